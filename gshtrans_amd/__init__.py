@@ -1,0 +1,357 @@
+"""gshtrans_amd -- MI355X (gfx950) GSH transform hot path.
+
+Python host layer over the C ABI in ``include/gshtrans_amd.h``
+(``gshtrans_amd/libgshtrans_hip.so``).  It mirrors the reference's
+``GaussLegendreGrid`` interface for the transform path
+(GSHTrans/src/GaussLegendreGrid.h:24-333, GridBase.h:12-180) with the same
+method names and argument meaning; device arrays are torch tensors, used only
+for memory and streams.  There is no CPU fallback: every transform runs the
+HIP kernels and fails loudly when the library or a gfx950 device is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgshtrans_hip.so")
+
+GSH_OK = 0
+STATUS = {
+    0: "GSH_OK", 1: "GSH_ERR_INVALID_ARGUMENT", 2: "GSH_ERR_UPPER_INDEX",
+    3: "GSH_ERR_DEGREE", 4: "GSH_ERR_VALUE_KIND", 5: "GSH_ERR_UNSUPPORTED",
+    6: "GSH_ERR_DEVICE", 7: "GSH_ERR_NO_DEVICE", 8: "GSH_ERR_OUT_OF_MEMORY",
+}
+RANGE = {"All": 0, "NonNegative": 1, "Single": 2}
+PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16x6": 3}
+STORAGE = {"fp64": 0, "fp32": 1}
+REAL, COMPLEX = 0, 1
+
+# Every symbol declared in include/gshtrans_amd.h.
+EXPORTS = [
+    "gsh_status_string", "gsh_version", "gsh_gl_nodes", "gsh_coefficient_size",
+    "gsh_wigner_seed", "gsh_grid_create", "gsh_grid_destroy", "gsh_grid_info",
+    "gsh_grid_colatitudes", "gsh_grid_prepare", "gsh_inverse", "gsh_forward",
+    "gsh_adjoint_inverse", "gsh_inverse_host", "gsh_forward_host",
+]
+
+
+class GSHError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        msg = _lib().gsh_status_string(status).decode() if _LIB is not None else STATUS.get(status)
+        super().__init__(f"{what}: {STATUS.get(status, status)} ({msg})")
+
+
+_LIB = None
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m gshtrans_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        i, l, d, vp = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.gsh_status_string.restype = ctypes.c_char_p
+        L.gsh_status_string.argtypes = [i]
+        L.gsh_version.restype = i
+        L.gsh_gl_nodes.argtypes = [i, dp, dp]
+        L.gsh_coefficient_size.restype = l
+        L.gsh_coefficient_size.argtypes = [i, i, i]
+        L.gsh_wigner_seed.argtypes = [i, i, d, dp]
+        L.gsh_grid_create.argtypes = [i, i, i, i, ctypes.POINTER(vp)]
+        L.gsh_grid_destroy.argtypes = [vp]
+        L.gsh_grid_info.argtypes = [vp, ip, ip, ip, ip, ip]
+        L.gsh_grid_colatitudes.argtypes = [vp, dp, dp]
+        L.gsh_grid_prepare.argtypes = [vp, i]
+        L.gsh_inverse.argtypes = [vp, i, i, i, i, i, i, vp, l, vp, l, vp]
+        L.gsh_forward.argtypes = [vp, i, i, i, i, i, i, vp, l, vp, l, i, vp]
+        L.gsh_adjoint_inverse.argtypes = [vp, i, i, i, i, i, i, vp, l, vp, l, vp]
+        L.gsh_inverse_host.argtypes = [vp, i, i, i, dp, dp]
+        L.gsh_forward_host.argtypes = [vp, i, i, i, dp, dp, i]
+        for name in ("gsh_gl_nodes", "gsh_wigner_seed", "gsh_grid_create", "gsh_grid_destroy",
+                     "gsh_grid_info", "gsh_grid_colatitudes", "gsh_grid_prepare",
+                     "gsh_inverse", "gsh_forward", "gsh_adjoint_inverse",
+                     "gsh_inverse_host", "gsh_forward_host"):
+            getattr(L, name).restype = i
+        _LIB = L
+    return _LIB
+
+
+def library():
+    """The loaded ctypes library (raises if the HIP library is not built)."""
+    return _lib()
+
+
+def _check(status, what):
+    if status != GSH_OK:
+        raise GSHError(status, what)
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+# ---------------------------------------------------------------------------
+# Host-side helpers (no device)
+# ---------------------------------------------------------------------------
+def gl_nodes(lmax):
+    """Colatitudes theta_i = acos(-x_i) (ascending) and Gauss-Legendre weights
+    of the (lmax+1)-point grid (GaussLegendreGrid.h:47-49)."""
+    th = np.zeros(lmax + 1)
+    w = np.zeros(lmax + 1)
+    _check(_lib().gsh_gl_nodes(lmax, _dp(th), _dp(w)), "gsh_gl_nodes")
+    return th, w
+
+
+def coefficient_size(lmax, n, real=False):
+    """GSHIndices<All|NonNegative>(lmax, lmax, n).Size() (Indexing.h:148-150)."""
+    s = _lib().gsh_coefficient_size(lmax, n, REAL if real else COMPLEX)
+    if s < 0:
+        raise ValueError("invalid (lmax, n)")
+    return int(s)
+
+
+def coefficient_index(lmax, n, l, m, real=False):
+    """GSHIndices<All|NonNegative>(lmax, lmax, n).Index(l, m) (Indexing.h:143-146)."""
+    na = abs(n)
+    if real:
+        return l * (l + 1) // 2 - na * (na + 1) // 2 + m
+    return l * l - na * na + m + l
+
+
+def wigner_seed(l, n, theta):
+    """WignerMinOrder(l, n, theta) (Wigner.h:52-80)."""
+    v = ctypes.c_double()
+    _check(_lib().gsh_wigner_seed(l, n, float(theta), ctypes.byref(v)), "gsh_wigner_seed")
+    return v.value
+
+
+def random_coefficients(lmax, n, real=False, seed=0, decay=1.0, grid_lmax=None,
+                        batch=None):
+    """Seeded synthetic coefficients with the reference generator's masks
+    (GridBase.h:115-172): complex normal c_lm scaled by (l+1)^-decay;
+    (L, L) zeroed for complex fields when lmax equals the grid's L; Im = 0 at
+    m = 0 and at (L, L) for real fields.  Counter-based splitmix64 ->
+    Box-Muller, so every host produces the same values for a seed."""
+    if grid_lmax is None:
+        grid_lmax = lmax
+    size = coefficient_size(lmax, n, real)
+    shape = (size,) if batch is None else (batch, size)
+    count = int(np.prod(shape))
+    golden = np.uint64(0x9E3779B97F4A7C15)
+
+    def mix(z):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+    with np.errstate(over="ignore"):
+        base = mix(np.uint64(seed) + golden)
+        z = mix(base + (np.arange(2 * count, dtype=np.uint64) + np.uint64(1)) * golden)
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    u1, u2 = u[0::2], u[1::2]
+    r = np.sqrt(-2.0 * np.log1p(-u1))
+    c = (r * np.cos(2 * np.pi * u2) + 1j * r * np.sin(2 * np.pi * u2)).reshape(shape)
+    na = abs(n)
+    ls = np.zeros(size, dtype=np.int64)
+    ms = np.zeros(size, dtype=np.int64)
+    k = 0
+    for l in range(na, lmax + 1):
+        cnt = l + 1 if real else 2 * l + 1
+        ls[k:k + cnt] = l
+        ms[k:k + cnt] = np.arange(0, l + 1) if real else np.arange(-l, l + 1)
+        k += cnt
+    c = c * (ls + 1.0) ** (-decay)
+    if real:
+        c[..., ms == 0] = c[..., ms == 0].real
+        if lmax == grid_lmax:
+            c[..., -1] = c[..., -1].real
+    elif lmax == grid_lmax:
+        c[..., -1] = 0
+    return c
+
+
+# ---------------------------------------------------------------------------
+# Grid
+# ---------------------------------------------------------------------------
+def _ptr(t):
+    import torch
+    if isinstance(t, torch.Tensor):
+        if not t.is_cuda:
+            raise ValueError("device tensor expected")
+        if not t.is_contiguous():
+            raise ValueError("contiguous tensor expected")
+        return ctypes.c_void_p(t.data_ptr())
+    raise TypeError("torch tensor expected")
+
+
+def _stream(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class GaussLegendreGrid:
+    """GaussLegendreGrid<double, MRange, NRange>(lMax, nMax) on one MI355X.
+
+    Mirrors GSHTrans/src/GaussLegendreGrid.h:24-333: colatitudes are the
+    Gauss-Legendre points theta = acos(-x) ascending, longitudes are the 2L
+    points j*pi/L, fields are row-major (theta, phi).
+    """
+
+    def __init__(self, lmax, nmax, mrange="All", nrange="All"):
+        self._h = ctypes.c_void_p()
+        _check(_lib().gsh_grid_create(lmax, nmax, RANGE[mrange], RANGE[nrange],
+                                      ctypes.byref(self._h)), "gsh_grid_create")
+        self._lmax, self._nmax = lmax, nmax
+        self.mrange, self.nrange = mrange, nrange
+        th = np.zeros(lmax + 1)
+        w = np.zeros(lmax + 1)
+        _check(_lib().gsh_grid_colatitudes(self._h, _dp(th), _dp(w)), "gsh_grid_colatitudes")
+        self._theta, self._w = th, w
+
+    def close(self):
+        if self._h:
+            _lib().gsh_grid_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # GaussLegendreGrid.h:86-103 / GridBase.h:17-77
+    def MaxDegree(self):
+        return self._lmax
+
+    def MaxUpperIndex(self):
+        return self._nmax
+
+    def MinUpperIndex(self):
+        return {"All": -self._nmax, "NonNegative": 0, "Single": self._nmax}[self.nrange]
+
+    def UpperIndices(self):
+        return range(self.MinUpperIndex(), self._nmax + 1)
+
+    def CoLatitudes(self):
+        return self._theta.copy()
+
+    def CoLatitudeWeights(self):
+        return self._w.copy()
+
+    def Longitudes(self):
+        L = self._lmax
+        return np.arange(2 * L) * (2 * math.pi / (2 * L)) if L > 0 else np.zeros(0)
+
+    def LongitudeWeights(self):
+        L = self._lmax
+        return np.full(2 * L, 2 * math.pi / (2 * L)) if L > 0 else np.zeros(0)
+
+    def NumberOfCoLatitudes(self):
+        return self._lmax + 1
+
+    def NumberOfLongitudes(self):
+        return 2 * self._lmax
+
+    def FieldSize(self):
+        return self.NumberOfCoLatitudes() * self.NumberOfLongitudes()
+
+    def ComplexCoefficientSize(self, lmax, n=None):
+        if n is None:
+            lmax, n = self._lmax, lmax
+        return coefficient_size(lmax, n, False)
+
+    def RealCoefficientSize(self, lmax, n=None):
+        if n is None:
+            lmax, n = self._lmax, lmax
+        return coefficient_size(lmax, n, True)
+
+    def prepare(self, n):
+        _check(_lib().gsh_grid_prepare(self._h, n), "gsh_grid_prepare")
+
+    # ---- batched device transforms --------------------------------------
+    def inverse(self, lmax, n, coeffs, field=None, real=False, precision="fp64",
+                stream=None):
+        """Synthesis of a batch: coeffs (B, C) complex128 cuda tensor ->
+        field (B, ntheta, nphi) float64 (real) or complex128."""
+        import torch
+        c = coeffs if coeffs.dim() == 2 else coeffs.unsqueeze(0)
+        B, C = c.shape
+        if C != coefficient_size(lmax, n, real):
+            raise ValueError("coefficient size mismatch")
+        if field is None:
+            field = torch.empty((B, self._lmax + 1, 2 * self._lmax),
+                                dtype=torch.float64 if real else torch.complex128,
+                                device=c.device)
+        _check(_lib().gsh_inverse(self._h, lmax, n, REAL if real else COMPLEX,
+                                  PRECISION[precision], STORAGE["fp64"], B, _ptr(c), C,
+                                  _ptr(field), self.FieldSize(), _stream(stream)),
+               "gsh_inverse")
+        return field if coeffs.dim() == 2 else field[0]
+
+    def forward(self, lmax, n, field, coeffs=None, real=False, precision="fp64",
+                accumulate=False, stream=None):
+        """Analysis of a batch: field (B, ntheta, nphi) -> coeffs (B, C)."""
+        import torch
+        f = field if field.dim() == 3 else field.unsqueeze(0)
+        B = f.shape[0]
+        C = coefficient_size(lmax, n, real)
+        if coeffs is None:
+            coeffs = torch.zeros((B, C), dtype=torch.complex128, device=f.device)
+        _check(_lib().gsh_forward(self._h, lmax, n, REAL if real else COMPLEX,
+                                  PRECISION[precision], STORAGE["fp64"], B, _ptr(f),
+                                  self.FieldSize(), _ptr(coeffs), C, 1 if accumulate else 0,
+                                  _stream(stream)), "gsh_forward")
+        return coeffs if field.dim() == 3 else coeffs[0]
+
+    # ---- reference-named single-field calls ------------------------------
+    def InverseTransformation(self, lmax, n, inp, out):
+        """GaussLegendreGrid.h:234: out = synthesis of inp.  Host numpy arrays
+        are staged through the device; torch cuda tensors stay resident."""
+        real = not np.iscomplexobj(out) if isinstance(out, np.ndarray) else \
+            not out.is_complex()
+        if isinstance(inp, np.ndarray):
+            c = np.ascontiguousarray(inp, dtype=np.complex128)
+            if c.size != coefficient_size(lmax, n, real):
+                raise ValueError("in.size() != coefficient size")
+            if out.size != self.FieldSize() or not out.flags.c_contiguous:
+                raise ValueError("out must be a contiguous array of FieldSize()")
+            _check(_lib().gsh_inverse_host(self._h, lmax, n, REAL if real else COMPLEX,
+                                           _dp(c.view(np.float64)), _dp(out.view(np.float64))),
+                   "InverseTransformation")
+            return out
+        self.inverse(lmax, n, inp.reshape(1, -1),
+                     field=out.reshape(1, self._lmax + 1, 2 * self._lmax), real=real)
+        return out
+
+    def ForwardTransformation(self, lmax, n, inp, out):
+        """GaussLegendreGrid.h:120: out += analysis of inp (the reference
+        accumulates into `out`)."""
+        real = not np.iscomplexobj(inp) if isinstance(inp, np.ndarray) else \
+            not inp.is_complex()
+        if isinstance(inp, np.ndarray):
+            f = np.ascontiguousarray(inp, dtype=np.float64 if real else np.complex128)
+            if f.size != self.FieldSize():
+                raise ValueError("in.size() != FieldSize()")
+            if out.size != coefficient_size(lmax, n, real) or out.dtype != np.complex128:
+                raise ValueError("out must be complex128 of the coefficient size")
+            _check(_lib().gsh_forward_host(self._h, lmax, n, REAL if real else COMPLEX,
+                                           _dp(f.view(np.float64)), _dp(out.view(np.float64)),
+                                           1), "ForwardTransformation")
+            return out
+        self.forward(lmax, n, inp.reshape(1, self._lmax + 1, 2 * self._lmax),
+                     coeffs=out.reshape(1, -1), real=real, accumulate=True)
+        return out

@@ -1,0 +1,595 @@
+// gsh_capi.cpp -- host layer of the MI355X GSH transform library: the C ABI
+// declared in include/gshtrans_amd.h, grid set-up (Gauss-Legendre nodes,
+// per-upper-index recurrence data) and launch orchestration.
+//
+// Reference counterparts (paths relative to the reference tree):
+//   GaussLegendreGrid ctor            GSHTrans/src/GaussLegendreGrid.h:38-73
+//   Wigner seeds (closed forms)       GSHTrans/src/Wigner.h:24-95, 329-344
+//   Wigner recurrence coefficients    Wigner.h:347-472 (restated per order m)
+//   Forward/InverseTransformation     GaussLegendreGrid.h:108-321
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/gshtrans_amd.h"
+#include "gsh_internal.h"
+
+namespace {
+
+using gsh::FFTParams;
+using gsh::LegParams;
+
+constexpr long double kPiL = 3.141592653589793238462643383279502884L;
+
+// ---------------------------------------------------------------------------
+// Host math
+// ---------------------------------------------------------------------------
+
+// Gauss-Legendre nodes x ascending on [-1, 1] and weights, by Newton's method
+// on P_{npts} in long double (the quadrature GaussLegendreGrid.h:47 obtains
+// from GaussQuad), so theta = acos(-x) ascends (GaussLegendreGrid.h:48).
+void gauss_legendre(int npts, std::vector<long double>& x, std::vector<long double>& w) {
+  x.assign(npts, 0.0L);
+  w.assign(npts, 0.0L);
+  auto legendre = [npts](long double z, long double& p1, long double& p2) {
+    p1 = 1;
+    p2 = 0;
+    for (int j = 1; j <= npts; j++) {
+      long double p3 = p2;
+      p2 = p1;
+      p1 = ((2 * j - 1) * z * p2 - (j - 1) * p3) / j;
+    }
+  };
+  for (int i = 0; i < (npts + 1) / 2; i++) {
+    long double z = std::cos(kPiL * (static_cast<long double>(i) + 0.75L) /
+                             (static_cast<long double>(npts) + 0.5L));
+    long double p1, p2;
+    for (int it = 0; it < 100; it++) {
+      legendre(z, p1, p2);
+      long double pp = npts * (z * p1 - p2) / (z * z - 1);
+      long double dz = p1 / pp;
+      z -= dz;
+      if (std::fabs(dz) <= 4 * std::numeric_limits<long double>::epsilon() && it > 1) break;
+    }
+    legendre(z, p1, p2);
+    long double pp = npts * (z * p1 - p2) / (z * z - 1);
+    long double wt = 2 / ((1 - z * z) * pp * pp);
+    x[npts - 1 - i] = z;
+    x[i] = -z;
+    w[npts - 1 - i] = wt;
+    w[i] = wt;
+  }
+  if (npts % 2 == 1) x[npts / 2] = 0;
+}
+
+// WignerDetails::Arguments (Wigner.h:24-50).
+struct SeedArgs {
+  double logSinHalf, logCosHalf;
+  bool atLeft, atRight;
+  explicit SeedArgs(double theta) {
+    const double half = 1.0 / 2.0;
+    logSinHalf = std::sin(half * theta);
+    logCosHalf = std::cos(half * theta);
+    atLeft = logSinHalf < std::numeric_limits<double>::min();
+    atRight = logCosHalf < std::numeric_limits<double>::min();
+    logSinHalf = atLeft ? 0.0 : std::log(logSinHalf);
+    logCosHalf = atRight ? 0.0 : std::log(logCosHalf);
+  }
+};
+
+// WignerMinOrder (Wigner.h:52-80), evaluated with the same libm expression so
+// the seeds agree with the reference's to the last bit.
+double min_order(long l, long n, const SeedArgs& a) {
+  if (l == 0) return 1.0;
+  if (a.atLeft) return n == -l ? 1.0 : 0.0;
+  if (a.atRight) return n == l ? 1.0 : 0.0;
+  const double half = 1.0 / 2.0;
+  const double Fl = static_cast<double>(l), Fn = static_cast<double>(n);
+  return std::exp(half * (std::lgamma(2 * Fl + 1) - std::lgamma(Fl - Fn + 1) -
+                          std::lgamma(Fl + Fn + 1)) +
+                  (Fl + Fn) * a.logSinHalf + (Fl - Fn) * a.logCosHalf);
+}
+double sign_pow(long k) { return (k % 2) ? -1.0 : 1.0; }  // Utility.h:8-10
+double max_order(long l, long n, const SeedArgs& a) { return sign_pow(n + l) * min_order(l, -n, a); }
+
+// d^{l0}_{m n}(theta) at the lowest degree l0 = max(|m|, |n|) of order m,
+// picking the same closed form the reference's Compute uses there
+// (Wigner.h:330-344 at l = |n|; :358-363 and :398-404 for m = -l; :379-382
+// and :458-459 for m = l).
+double seed_value(long m, long n, const SeedArgs& a) {
+  const long nabs = std::labs(n);
+  const long mabs = std::labs(m);
+  if (mabs <= nabs) {
+    const long l = nabs;
+    return n >= 0 ? min_order(l, -m, a)          // WignerMaxUpperIndex
+                  : max_order(l, -m, a);         // WignerMinUpperIndex
+  }
+  const long l = mabs;
+  return m < 0 ? min_order(l, n, a) : max_order(l, n, a);
+}
+
+// Ortho factor applied as in Wigner.h:475-485.
+double ortho_scale(long l, double v) {
+  const double factor = 0.564189583547756286948079451560772585844050629328998856844 / 2.0;
+  return factor * std::sqrt(static_cast<double>(2 * l + 1)) * v;
+}
+
+template <typename F>
+void parallel_for(long n, F f) {
+  unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  if (n < 256) nt = 1;
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; t++)
+    th.emplace_back([=, &f]() {
+      for (long i = t; i < n; i += nt) f(i);
+    });
+  for (auto& x : th) x.join();
+}
+
+// ---------------------------------------------------------------------------
+// Grid state
+// ---------------------------------------------------------------------------
+struct NData {
+  double* seeds = nullptr;  // [(m+L)*ntheta + t]
+  double* RA = nullptr;
+  double* RB = nullptr;
+  double* RC = nullptr;
+  long* roff = nullptr;
+};
+
+#define GSH_HIP(call)                                  \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return GSH_ERR_DEVICE;       \
+  } while (0)
+
+}  // namespace
+
+struct gsh_grid_s {
+  int L = 0, nmax = 0, mrange = 0, nrange = 0, device = 0;
+  std::vector<double> theta, weight;
+  double* d_cos = nullptr;
+  double* d_wq = nullptr;
+  double2* d_tw = nullptr;
+  std::map<int, NData> ndata;
+  double* work = nullptr;
+  size_t work_bytes = 0;
+  double* stage = nullptr;  // host-call staging
+  size_t stage_bytes = 0;
+  std::mutex mu;
+
+  int min_upper() const { return nrange == GSH_ALL ? -nmax : (nrange == GSH_NONNEGATIVE ? 0 : nmax); }
+  int ntheta() const { return L + 1; }
+  int nphi() const { return 2 * L; }
+};
+
+namespace {
+
+int ensure_ndata(gsh_grid_s* g, int n, NData** out) {
+  auto it = g->ndata.find(n);
+  if (it != g->ndata.end()) {
+    *out = &it->second;
+    return GSH_OK;
+  }
+  const int L = g->L, nt = g->ntheta();
+  const long nabs = std::labs(n);
+  const long norders = 2L * L + 1;
+  // Seeds, Ortho-normalised.
+  std::vector<double> seeds(norders * nt);
+  parallel_for(nt, [&](long t) {
+    SeedArgs a(g->theta[t]);
+    for (long m = -L; m <= L; m++) {
+      const long l0 = std::max(std::labs(m), nabs);
+      seeds[(m + L) * nt + t] = ortho_scale(l0, seed_value(m, n, a));
+    }
+  });
+  // Recurrence coefficients per (m, l), l0 < l <= L, in long double:
+  //   d_l = [(2l-1)(l(l-1)cos - m n) d_{l-1} - l sqrt(((l-1)^2-n^2)((l-1)^2-m^2)) d_{l-2}]
+  //         / ((l-1) sqrt((l^2-n^2)(l^2-m^2)))            (Wigner.h:407-445)
+  // with the l = |n|+1 one-term form (Wigner.h:368-376) as its n m / (l-1) -> sign(n) m
+  // limit, and the Ortho ratio sqrt(2l+1) folded in.
+  std::vector<long> roff(norders);
+  long total = 0;
+  for (long m = -L; m <= L; m++) {
+    roff[m + L] = total;
+    total += L - std::max(std::labs(m), nabs) + 1;
+  }
+  std::vector<double> RA(total, 0.0), RB(total, 0.0), RC(total, 0.0);
+  parallel_for(norders, [&](long mi) {
+    const long m = mi - L;
+    const long l0 = std::max(std::labs(m), nabs);
+    for (long l = l0 + 1; l <= L; l++) {
+      const long double ll = l, mm = m, nn = n;
+      const long double den = std::sqrt((ll * ll - nn * nn) * (ll * ll - mm * mm));
+      const long double r1 = std::sqrt((2 * ll + 1) / (2 * ll - 1));
+      const long double a = (2 * ll - 1) * ll / den;
+      long double b = 0;
+      if (m * n != 0) b = (2 * ll - 1) * mm * nn / ((ll - 1) * den);
+      long double c = 0;
+      if (l >= l0 + 2) {
+        const long double r2 = std::sqrt((2 * ll + 1) / (2 * ll - 3));
+        c = ll * std::sqrt(((ll - 1) * (ll - 1) - nn * nn) * ((ll - 1) * (ll - 1) - mm * mm)) /
+            ((ll - 1) * den) * r2;
+      }
+      const long idx = roff[mi] + (l - l0);
+      RA[idx] = static_cast<double>(a * r1);
+      RB[idx] = static_cast<double>(b * r1);
+      RC[idx] = static_cast<double>(c);
+    }
+  });
+  NData d;
+  GSH_HIP(hipSetDevice(g->device));
+  GSH_HIP(hipMalloc(&d.seeds, seeds.size() * sizeof(double)));
+  GSH_HIP(hipMalloc(&d.RA, total * sizeof(double)));
+  GSH_HIP(hipMalloc(&d.RB, total * sizeof(double)));
+  GSH_HIP(hipMalloc(&d.RC, total * sizeof(double)));
+  GSH_HIP(hipMalloc(&d.roff, norders * sizeof(long)));
+  GSH_HIP(hipMemcpy(d.seeds, seeds.data(), seeds.size() * sizeof(double), hipMemcpyHostToDevice));
+  GSH_HIP(hipMemcpy(d.RA, RA.data(), total * sizeof(double), hipMemcpyHostToDevice));
+  GSH_HIP(hipMemcpy(d.RB, RB.data(), total * sizeof(double), hipMemcpyHostToDevice));
+  GSH_HIP(hipMemcpy(d.RC, RC.data(), total * sizeof(double), hipMemcpyHostToDevice));
+  GSH_HIP(hipMemcpy(d.roff, roff.data(), norders * sizeof(long), hipMemcpyHostToDevice));
+  auto res = g->ndata.emplace(n, d);
+  *out = &res.first->second;
+  return GSH_OK;
+}
+
+int ensure_work(gsh_grid_s* g, size_t bytes) {
+  if (bytes <= g->work_bytes) return GSH_OK;
+  if (g->work) {
+    GSH_HIP(hipDeviceSynchronize());
+    GSH_HIP(hipFree(g->work));
+    g->work = nullptr;
+    g->work_bytes = 0;
+  }
+  if (hipMalloc(&g->work, bytes) != hipSuccess) {
+    g->work = nullptr;
+    return GSH_ERR_OUT_OF_MEMORY;
+  }
+  g->work_bytes = bytes;
+  return GSH_OK;
+}
+
+int check_call(const gsh_grid_s* g, int lmax, int n, int value_kind, int precision, int storage,
+               int batch) {
+  if (!g) return GSH_ERR_INVALID_ARGUMENT;
+  if (value_kind != GSH_REAL && value_kind != GSH_COMPLEX) return GSH_ERR_INVALID_ARGUMENT;
+  if (precision < GSH_FP64 || precision > GSH_BF16X6) return GSH_ERR_INVALID_ARGUMENT;
+  if (storage != GSH_STORE_FP64 && storage != GSH_STORE_FP32) return GSH_ERR_INVALID_ARGUMENT;
+  if (batch < 0) return GSH_ERR_INVALID_ARGUMENT;
+  if (lmax < 0 || lmax > g->L) return GSH_ERR_DEGREE;
+  if (n < g->min_upper() || n > g->nmax || std::abs(n) > lmax) return GSH_ERR_UPPER_INDEX;
+  if (value_kind == GSH_COMPLEX && g->mrange != GSH_ALL) return GSH_ERR_VALUE_KIND;
+  return GSH_OK;
+}
+
+LegParams base_params(gsh_grid_s* g, const NData* d, int lmax, int n, int real, int batch) {
+  LegParams p{};
+  p.L = g->L;
+  p.ntheta = g->ntheta();
+  p.lmax = lmax;
+  p.n = n;
+  p.nabs = std::abs(n);
+  p.real = real;
+  p.m_first = real ? 0 : -lmax;
+  p.nslot = real ? lmax + 1 : 2 * lmax + 1;
+  p.nfield = batch;
+  p.ncol = 2 * batch;
+  p.cosv = g->d_cos;
+  p.seeds = d->seeds;
+  p.RA = d->RA;
+  p.RB = d->RB;
+  p.RC = d->RC;
+  p.roff = d->roff;
+  p.F = g->work;
+  return p;
+}
+
+FFTParams fft_params(gsh_grid_s* g, int lmax, int real, int batch) {
+  FFTParams q{};
+  q.N = g->nphi();
+  q.L = g->L;
+  q.ntheta = g->ntheta();
+  q.lmax = lmax;
+  q.real = real;
+  q.m_first = real ? 0 : -lmax;
+  q.nslot = real ? lmax + 1 : 2 * lmax + 1;
+  q.nfield = batch;
+  q.ncol = 2 * batch;
+  q.tw = g->d_tw;
+  q.wq = g->d_wq;
+  q.F = g->work;
+  return q;
+}
+
+bool fft_supported(int N) { return N >= 2 && N <= 4096 && (N & (N - 1)) == 0; }
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* gsh_status_string(int status) {
+  switch (status) {
+    case GSH_OK: return "ok";
+    case GSH_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case GSH_ERR_UPPER_INDEX: return "upper index n outside the grid's UpperIndices() or |n| > lMax";
+    case GSH_ERR_DEGREE: return "degree lMax outside [0, grid lMax]";
+    case GSH_ERR_VALUE_KIND: return "complex field on an MRange=NonNegative grid";
+    case GSH_ERR_UNSUPPORTED: return "configuration not supported by this build";
+    case GSH_ERR_DEVICE: return "HIP runtime error";
+    case GSH_ERR_NO_DEVICE: return "no gfx950 device";
+    case GSH_ERR_OUT_OF_MEMORY: return "out of device memory";
+    default: return "unknown status";
+  }
+}
+
+int gsh_version(void) { return 1; }
+
+int gsh_gl_nodes(int lmax, double* theta, double* weight) {
+  if (lmax < 0 || !theta || !weight) return GSH_ERR_INVALID_ARGUMENT;
+  std::vector<long double> x, w;
+  gauss_legendre(lmax + 1, x, w);
+  for (int i = 0; i <= lmax; i++) {
+    const double xi = static_cast<double>(x[i]);
+    theta[i] = std::acos(-xi);
+    weight[i] = static_cast<double>(w[i]);
+  }
+  return GSH_OK;
+}
+
+long gsh_coefficient_size(int lmax, int n, int value_kind) {
+  if (lmax < 0 || std::abs(n) > lmax) return -1;
+  const long l = lmax, na = std::abs(n);
+  if (value_kind == GSH_REAL) return (l + 1) * (l + 2) / 2 - na * (na + 1) / 2;
+  return (l + 1) * (l + 1) - na * na;
+}
+
+int gsh_wigner_seed(int l, int n, double theta, double* value) {
+  if (!value || l < 0 || std::abs(n) > l) return GSH_ERR_INVALID_ARGUMENT;
+  SeedArgs a(theta);
+  *value = min_order(l, n, a);
+  return GSH_OK;
+}
+
+int gsh_grid_create(int lmax, int nmax, int mrange, int nrange, gsh_grid_t* grid) {
+  if (!grid) return GSH_ERR_INVALID_ARGUMENT;
+  *grid = nullptr;
+  if (lmax < 0 || nmax < 0 || nmax > lmax) return GSH_ERR_INVALID_ARGUMENT;
+  if (mrange != GSH_ALL && mrange != GSH_NONNEGATIVE) return GSH_ERR_INVALID_ARGUMENT;
+  if (nrange != GSH_ALL && nrange != GSH_NONNEGATIVE && nrange != GSH_SINGLE)
+    return GSH_ERR_INVALID_ARGUMENT;
+  if (lmax > 0 && !fft_supported(2 * lmax)) return GSH_ERR_UNSUPPORTED;
+  if ((lmax + 1 + 63) / 64 > 36) return GSH_ERR_UNSUPPORTED;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return GSH_ERR_NO_DEVICE;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return GSH_ERR_NO_DEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return GSH_ERR_NO_DEVICE;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return GSH_ERR_NO_DEVICE;
+
+  auto g = std::make_unique<gsh_grid_s>();
+  g->L = lmax;
+  g->nmax = nmax;
+  g->mrange = mrange;
+  g->nrange = nrange;
+  g->device = dev;
+  g->theta.resize(lmax + 1);
+  g->weight.resize(lmax + 1);
+  gsh_gl_nodes(lmax, g->theta.data(), g->weight.data());
+  const int N = g->nphi();
+  std::vector<double> cs(lmax + 1), wq(lmax + 1);
+  // scaleFactor = 2 pi / nPhi (GaussLegendreGrid.h:144-145)
+  const double scale = N > 0 ? 2.0 * 3.14159265358979323846 / static_cast<double>(N) : 0.0;
+  for (int i = 0; i <= lmax; i++) {
+    cs[i] = std::cos(g->theta[i]);
+    wq[i] = g->weight[i] * scale;
+  }
+  std::vector<double2> tw(std::max(N, 1));
+  for (int k = 0; k < N; k++) {
+    const long double a = -2.0L * kPiL * k / N;
+    tw[k] = make_double2(static_cast<double>(std::cos(a)), static_cast<double>(std::sin(a)));
+  }
+  GSH_HIP(hipMalloc(&g->d_cos, cs.size() * sizeof(double)));
+  GSH_HIP(hipMalloc(&g->d_wq, wq.size() * sizeof(double)));
+  GSH_HIP(hipMalloc(&g->d_tw, tw.size() * sizeof(double2)));
+  GSH_HIP(hipMemcpy(g->d_cos, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice));
+  GSH_HIP(hipMemcpy(g->d_wq, wq.data(), wq.size() * sizeof(double), hipMemcpyHostToDevice));
+  GSH_HIP(hipMemcpy(g->d_tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
+  *grid = g.release();
+  return GSH_OK;
+}
+
+int gsh_grid_destroy(gsh_grid_t g) {
+  if (!g) return GSH_ERR_INVALID_ARGUMENT;
+  hipSetDevice(g->device);
+  hipDeviceSynchronize();
+  for (auto& kv : g->ndata) {
+    hipFree(kv.second.seeds);
+    hipFree(kv.second.RA);
+    hipFree(kv.second.RB);
+    hipFree(kv.second.RC);
+    hipFree(kv.second.roff);
+  }
+  hipFree(g->d_cos);
+  hipFree(g->d_wq);
+  hipFree(g->d_tw);
+  if (g->work) hipFree(g->work);
+  if (g->stage) hipFree(g->stage);
+  delete g;
+  return GSH_OK;
+}
+
+int gsh_grid_info(gsh_grid_t g, int* lmax, int* nmax, int* min_upper, int* ntheta, int* nphi) {
+  if (!g) return GSH_ERR_INVALID_ARGUMENT;
+  if (lmax) *lmax = g->L;
+  if (nmax) *nmax = g->nmax;
+  if (min_upper) *min_upper = g->min_upper();
+  if (ntheta) *ntheta = g->ntheta();
+  if (nphi) *nphi = g->nphi();
+  return GSH_OK;
+}
+
+int gsh_grid_colatitudes(gsh_grid_t g, double* theta, double* weight) {
+  if (!g) return GSH_ERR_INVALID_ARGUMENT;
+  if (theta) std::copy(g->theta.begin(), g->theta.end(), theta);
+  if (weight) std::copy(g->weight.begin(), g->weight.end(), weight);
+  return GSH_OK;
+}
+
+int gsh_grid_prepare(gsh_grid_t g, int n) {
+  if (!g) return GSH_ERR_INVALID_ARGUMENT;
+  if (n < g->min_upper() || n > g->nmax) return GSH_ERR_UPPER_INDEX;
+  std::lock_guard<std::mutex> lock(g->mu);
+  NData* d;
+  return ensure_ndata(g, n, &d);
+}
+
+int gsh_inverse(gsh_grid_t g, int lmax, int n, int value_kind, int precision, int storage,
+                int batch, const void* coeffs, long coeff_stride, void* field, long field_stride,
+                void* stream) {
+  int st = check_call(g, lmax, n, value_kind, precision, storage, batch);
+  if (st) return st;
+  if (batch == 0) return GSH_OK;
+  if (!coeffs || !field) return GSH_ERR_INVALID_ARGUMENT;
+  if (precision != GSH_FP64 || storage != GSH_STORE_FP64) return GSH_ERR_UNSUPPORTED;
+  const int real = value_kind == GSH_REAL;
+  const long csize = gsh_coefficient_size(lmax, n, value_kind);
+  const long fsize = static_cast<long>(g->ntheta()) * g->nphi();
+  if (batch > 1 && (coeff_stride < csize || field_stride < fsize)) return GSH_ERR_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> lock(g->mu);
+  GSH_HIP(hipSetDevice(g->device));
+  if (lmax == 0) {
+    GSH_HIP(gsh::launch_lmax0(1, real, batch, static_cast<const double*>(coeffs), coeff_stride,
+                              static_cast<double*>(field), field_stride, 0, s));
+    return GSH_OK;
+  }
+  NData* d;
+  if ((st = ensure_ndata(g, n, &d))) return st;
+  const int nslot = real ? lmax + 1 : 2 * lmax + 1;
+  if ((st = ensure_work(g, sizeof(double) * g->ntheta() * nslot * 2 * (size_t)batch))) return st;
+  LegParams p = base_params(g, d, lmax, n, real, batch);
+  p.cin = static_cast<const double*>(coeffs);
+  p.cstride = coeff_stride;
+  GSH_HIP(gsh::launch_leg_synth_f64(p, s));
+  FFTParams q = fft_params(g, lmax, real, batch);
+  q.fout = field;
+  q.fstride = field_stride;
+  GSH_HIP(gsh::launch_fft_synth(q, s));
+  return GSH_OK;
+}
+
+int gsh_forward(gsh_grid_t g, int lmax, int n, int value_kind, int precision, int storage,
+                int batch, const void* field, long field_stride, void* coeffs, long coeff_stride,
+                int accumulate, void* stream) {
+  int st = check_call(g, lmax, n, value_kind, precision, storage, batch);
+  if (st) return st;
+  if (batch == 0) return GSH_OK;
+  if (!coeffs || !field) return GSH_ERR_INVALID_ARGUMENT;
+  if (precision != GSH_FP64 || storage != GSH_STORE_FP64) return GSH_ERR_UNSUPPORTED;
+  const int real = value_kind == GSH_REAL;
+  const long csize = gsh_coefficient_size(lmax, n, value_kind);
+  const long fsize = static_cast<long>(g->ntheta()) * g->nphi();
+  if (batch > 1 && (coeff_stride < csize || field_stride < fsize)) return GSH_ERR_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> lock(g->mu);
+  GSH_HIP(hipSetDevice(g->device));
+  if (lmax == 0) {
+    GSH_HIP(gsh::launch_lmax0(0, real, batch, static_cast<const double*>(field), field_stride,
+                              static_cast<double*>(coeffs), coeff_stride, accumulate, s));
+    return GSH_OK;
+  }
+  NData* d;
+  if ((st = ensure_ndata(g, n, &d))) return st;
+  const int nslot = real ? lmax + 1 : 2 * lmax + 1;
+  if ((st = ensure_work(g, sizeof(double) * g->ntheta() * nslot * 2 * (size_t)batch))) return st;
+  FFTParams q = fft_params(g, lmax, real, batch);
+  q.fin = field;
+  q.fstride = field_stride;
+  GSH_HIP(gsh::launch_fft_anal(q, s));
+  LegParams p = base_params(g, d, lmax, n, real, batch);
+  p.cout = static_cast<double*>(coeffs);
+  p.cstride = coeff_stride;
+  p.accumulate = accumulate ? 1 : 0;
+  p.zero_last = (!real && lmax == g->L) ? 1 : 0;
+  GSH_HIP(gsh::launch_leg_anal_f64(p, s));
+  return GSH_OK;
+}
+
+int gsh_adjoint_inverse(gsh_grid_t, int, int, int, int, int, int, const void*, long, void*,
+                        long, void*) {
+  return GSH_ERR_UNSUPPORTED;
+}
+
+static int stage_ensure(gsh_grid_s* g, size_t bytes) {
+  if (bytes <= g->stage_bytes) return GSH_OK;
+  if (g->stage) GSH_HIP(hipFree(g->stage));
+  g->stage = nullptr;
+  g->stage_bytes = 0;
+  if (hipMalloc(&g->stage, bytes) != hipSuccess) return GSH_ERR_OUT_OF_MEMORY;
+  g->stage_bytes = bytes;
+  return GSH_OK;
+}
+
+int gsh_inverse_host(gsh_grid_t g, int lmax, int n, int value_kind, const double* coeffs,
+                     double* field) {
+  int st = check_call(g, lmax, n, value_kind, GSH_FP64, GSH_STORE_FP64, 1);
+  if (st) return st;
+  if (!coeffs || !field) return GSH_ERR_INVALID_ARGUMENT;
+  const size_t cbytes = 16 * (size_t)gsh_coefficient_size(lmax, n, value_kind);
+  const size_t fbytes = (value_kind == GSH_REAL ? 8 : 16) * (size_t)g->ntheta() * g->nphi();
+  {
+    std::lock_guard<std::mutex> lock(g->mu);
+    GSH_HIP(hipSetDevice(g->device));
+    if ((st = stage_ensure(g, cbytes + fbytes))) return st;
+    GSH_HIP(hipMemcpy(g->stage, coeffs, cbytes, hipMemcpyHostToDevice));
+    // lMax == 0 writes element 0 only; keep the rest of the caller's field.
+    if (lmax == 0) GSH_HIP(hipMemcpy(reinterpret_cast<char*>(g->stage) + cbytes, field, fbytes,
+                                     hipMemcpyHostToDevice));
+  }
+  char* dfield = reinterpret_cast<char*>(g->stage) + cbytes;
+  st = gsh_inverse(g, lmax, n, value_kind, GSH_FP64, GSH_STORE_FP64, 1, g->stage, 0, dfield, 0,
+                   nullptr);
+  if (st) return st;
+  GSH_HIP(hipMemcpy(field, dfield, fbytes, hipMemcpyDeviceToHost));
+  return GSH_OK;
+}
+
+int gsh_forward_host(gsh_grid_t g, int lmax, int n, int value_kind, const double* field,
+                     double* coeffs, int accumulate) {
+  int st = check_call(g, lmax, n, value_kind, GSH_FP64, GSH_STORE_FP64, 1);
+  if (st) return st;
+  if (!coeffs || !field) return GSH_ERR_INVALID_ARGUMENT;
+  const size_t cbytes = 16 * (size_t)gsh_coefficient_size(lmax, n, value_kind);
+  const size_t fbytes = (value_kind == GSH_REAL ? 8 : 16) * (size_t)g->ntheta() * g->nphi();
+  {
+    std::lock_guard<std::mutex> lock(g->mu);
+    GSH_HIP(hipSetDevice(g->device));
+    if ((st = stage_ensure(g, cbytes + fbytes))) return st;
+    GSH_HIP(hipMemcpy(g->stage, field, fbytes, hipMemcpyHostToDevice));
+    GSH_HIP(hipMemcpy(reinterpret_cast<char*>(g->stage) + fbytes, coeffs, cbytes,
+                      hipMemcpyHostToDevice));
+  }
+  char* dco = reinterpret_cast<char*>(g->stage) + fbytes;
+  st = gsh_forward(g, lmax, n, value_kind, GSH_FP64, GSH_STORE_FP64, 1, g->stage, 0, dco, 0,
+                   accumulate, nullptr);
+  if (st) return st;
+  GSH_HIP(hipMemcpy(coeffs, dco, cbytes, hipMemcpyDeviceToHost));
+  return GSH_OK;
+}
+
+}  // extern "C"

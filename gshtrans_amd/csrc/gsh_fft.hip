@@ -1,0 +1,279 @@
+// gsh_fft.hip -- batched longitude-ring FFTs for the Gauss-Legendre grid.
+//
+// The reference runs one FFTW plan per ring (GaussLegendreGrid.h:147-174 for
+// the analysis, :266-319 for the synthesis): an unnormalised c2c/c2r backward
+// transform of the order bins F_m placed at k = m (m >= 0) and k = N + m
+// (m < 0), and an unnormalised c2c/r2c forward transform of each ring.  Here
+// each workgroup loads `rings` rings of N = 2L points (rings * N = 4096
+// complex values, 64 KiB) into LDS, runs a Stockham radix-4 (+ one radix-2)
+// autosort FFT in place (values go through registers between barriers), and
+// writes the rings back.  The synthesis kernel also assembles the bins from
+// the Legendre work buffer F[theta][slot][col]; the analysis kernel scatters
+// X_m * w_theta * 2 pi / N back into F for the Legendre contraction.
+#include "gsh_internal.h"
+
+namespace gsh {
+namespace {
+
+constexpr int kFFTThreads = 256;
+constexpr int kFFTElems = 4096;  // complex values per workgroup
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ double2 csub(double2 a, double2 b) {
+  return make_double2(a.x - b.x, a.y - b.y);
+}
+// multiply by s*i (s = +1 or -1)
+__device__ __forceinline__ double2 cmuli(double2 a, int s) {
+  return s > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+// Stockham autosort passes over `rings` rings of length N held in buf
+// (ring q at buf[q*N]).  sign = -1 forward, +1 backward.
+template <int SIGN>
+__device__ void stockham(double2* buf, int N, int rings, const double2* __restrict__ tw) {
+  int logN = 31 - __clz(N);
+  int Ns = 1;
+  // radix-4 passes
+  while (logN >= 2) {
+    const int quarter = N >> 2;
+    const int total = rings * quarter;  // butterflies
+    constexpr int kPer = kFFTElems / 4 / kFFTThreads;
+    double2 v[kPer][4];
+    int dst[kPer];
+    int qq[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; i++) {
+      const int bi = threadIdx.x + i * kFFTThreads;
+      dst[i] = -1;
+      if (bi >= total) continue;
+      const int q = bi / quarter;
+      const int j = bi - q * quarter;
+      const int k = j & (Ns - 1);
+      const double2* x = buf + q * N;
+      double2 a0 = x[j], a1 = x[j + quarter], a2 = x[j + 2 * quarter], a3 = x[j + 3 * quarter];
+      if (Ns > 1) {
+        const int step = N / (Ns * 4);
+        double2 w1 = tw[k * step], w2 = tw[2 * k * step], w3 = tw[3 * k * step];
+        if (SIGN > 0) {
+          w1.y = -w1.y;
+          w2.y = -w2.y;
+          w3.y = -w3.y;
+        }
+        a1 = cmul(a1, w1);
+        a2 = cmul(a2, w2);
+        a3 = cmul(a3, w3);
+      }
+      const double2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+      const double2 t2 = cadd(a1, a3), t3 = cmuli(csub(a1, a3), SIGN);
+      v[i][0] = cadd(t0, t2);
+      v[i][1] = cadd(t1, t3);
+      v[i][2] = csub(t0, t2);
+      v[i][3] = csub(t1, t3);
+      dst[i] = q * N + (j / Ns) * Ns * 4 + k;
+      qq[i] = q;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kPer; i++) {
+      if (dst[i] < 0) continue;
+      buf[dst[i]] = v[i][0];
+      buf[dst[i] + Ns] = v[i][1];
+      buf[dst[i] + 2 * Ns] = v[i][2];
+      buf[dst[i] + 3 * Ns] = v[i][3];
+    }
+    (void)qq;
+    __syncthreads();
+    Ns *= 4;
+    logN -= 2;
+  }
+  if (logN == 1) {
+    const int half = N >> 1;
+    const int total = rings * half;
+    constexpr int kPer = kFFTElems / 2 / kFFTThreads;
+    double2 v[kPer][2];
+    int dst[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; i++) {
+      const int bi = threadIdx.x + i * kFFTThreads;
+      dst[i] = -1;
+      if (bi >= total) continue;
+      const int q = bi / half;
+      const int j = bi - q * half;
+      const int k = j & (Ns - 1);
+      const double2* x = buf + q * N;
+      double2 a0 = x[j], a1 = x[j + half];
+      if (Ns > 1) {
+        double2 w1 = tw[k * (N / (Ns * 2))];
+        if (SIGN > 0) w1.y = -w1.y;
+        a1 = cmul(a1, w1);
+      }
+      v[i][0] = cadd(a0, a1);
+      v[i][1] = csub(a0, a1);
+      dst[i] = q * N + (j / Ns) * Ns * 2 + k;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kPer; i++) {
+      if (dst[i] < 0) continue;
+      buf[dst[i]] = v[i][0];
+      buf[dst[i] + Ns] = v[i][1];
+    }
+    __syncthreads();
+  }
+}
+
+// Synthesis: bins from F, backward FFT, write field rings.
+__global__ __launch_bounds__(kFFTThreads) void fft_synth(FFTParams p) {
+  __shared__ double2 buf[kFFTElems];
+  const int theta = blockIdx.x / p.nfg;
+  const int fg = blockIdx.x % p.nfg;
+  const int N = p.N, R = p.rings;
+  const long frow = (long)theta * p.nslot * p.ncol;
+  // Fill: consecutive threads read consecutive fields of one bin.
+  for (int idx = threadIdx.x; idx < R * N; idx += kFFTThreads) {
+    const int q = idx % R;
+    const int k = idx / R;
+    const int f = fg * R + q;
+    double2 X = make_double2(0.0, 0.0);
+    if (f < p.nfield) {
+      const double* Fp = p.F + frow + 2 * f;
+      if (!p.real) {
+        if (k <= p.lmax) {
+          const int s = k - p.m_first;
+          X = make_double2(Fp[(long)s * p.ncol], Fp[(long)s * p.ncol + 1]);
+        }
+        if (k >= N - p.lmax) {
+          const int s = (k - N) - p.m_first;
+          X = cadd(X, make_double2(Fp[(long)s * p.ncol], Fp[(long)s * p.ncol + 1]));
+        }
+      } else {
+        // Hermitian spectrum from bins 0..N/2 (c2r semantics).
+        const int kk = k <= N / 2 ? k : N - k;
+        if (kk <= p.lmax) {
+          const int s = kk - p.m_first;
+          X = make_double2(Fp[(long)s * p.ncol], Fp[(long)s * p.ncol + 1]);
+          if (kk == 0 || kk == N / 2) X.y = 0.0;
+          else if (k > N / 2) X.y = -X.y;
+        }
+      }
+    }
+    buf[q * N + k] = X;
+  }
+  __syncthreads();
+  stockham<+1>(buf, N, R, p.tw);
+  for (int idx = threadIdx.x; idx < R * N; idx += kFFTThreads) {
+    const int q = idx / N;
+    const int j = idx - q * N;
+    const int f = fg * R + q;
+    if (f >= p.nfield) continue;
+    const long o = (long)f * p.fstride + (long)theta * N + j;
+    if (p.real)
+      static_cast<double*>(p.fout)[o] = buf[idx].x;
+    else
+      static_cast<double2*>(p.fout)[o] = buf[idx];
+  }
+}
+
+// Analysis: load rings, forward FFT, scatter weighted bins into F.
+__global__ __launch_bounds__(kFFTThreads) void fft_anal(FFTParams p) {
+  __shared__ double2 buf[kFFTElems];
+  const int theta = blockIdx.x / p.nfg;
+  const int fg = blockIdx.x % p.nfg;
+  const int N = p.N, R = p.rings;
+  for (int idx = threadIdx.x; idx < R * N; idx += kFFTThreads) {
+    const int q = idx / N;
+    const int j = idx - q * N;
+    const int f = fg * R + q;
+    double2 x = make_double2(0.0, 0.0);
+    if (f < p.nfield) {
+      const long o = (long)f * p.fstride + (long)theta * N + j;
+      if (p.real)
+        x.x = static_cast<const double*>(p.fin)[o];
+      else
+        x = static_cast<const double2*>(p.fin)[o];
+    }
+    buf[idx] = x;
+  }
+  __syncthreads();
+  stockham<-1>(buf, N, R, p.tw);
+  const double w = p.wq[theta];
+  const long frow = (long)theta * p.nslot * p.ncol;
+  for (int idx = threadIdx.x; idx < R * p.nslot; idx += kFFTThreads) {
+    const int q = idx % R;
+    const int s = idx / R;
+    const int f = fg * R + q;
+    if (f >= p.nfield) continue;
+    const int m = s + p.m_first;
+    const int k = m >= 0 ? m : N + m;
+    const double2 X = buf[q * N + k];
+    double* Fp = p.F + frow + (long)s * p.ncol + 2 * f;
+    Fp[0] = X.x * w;
+    Fp[1] = X.y * w;
+  }
+}
+
+// lmax == 0 branches of the reference act on element 0 only
+// (GaussLegendreGrid.h:137-140 and :251-260).
+__global__ void lmax0_kernel(int inverse, int real, int nfield, const double* in, long istride,
+                             double* out, long ostride, int accumulate) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nfield) return;
+  const double inv_sqrtpi = 0.56418958354775628695;  // 1/sqrt(pi)
+  if (inverse) {
+    // in: complex coefficients; out: real or complex field
+    const double* c = in + 2 * (long)f * istride;
+    if (real) {
+      out[(long)f * ostride] = c[0] * 2.0 / inv_sqrtpi;
+    } else {
+      out[2 * (long)f * ostride] = c[0] * 2.0 / inv_sqrtpi;
+      out[2 * (long)f * ostride + 1] = c[1] * 2.0 / inv_sqrtpi;
+    }
+  } else {
+    // in: field; out: complex coefficients
+    double re, im;
+    if (real) {
+      re = in[(long)f * istride] * inv_sqrtpi / 2.0;
+      im = 0.0;
+    } else {
+      re = in[2 * (long)f * istride] * inv_sqrtpi / 2.0;
+      im = in[2 * (long)f * istride + 1] * inv_sqrtpi / 2.0;
+    }
+    // The reference assigns here even though its general branch accumulates.
+    double* c = out + 2 * (long)f * ostride;
+    (void)accumulate;
+    c[0] = re;
+    c[1] = im;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_fft_synth(const FFTParams& p0, hipStream_t s) {
+  FFTParams p = p0;
+  p.rings = kFFTElems / p.N;
+  p.nfg = (p.nfield + p.rings - 1) / p.rings;
+  hipLaunchKernelGGL(fft_synth, dim3((long)p.ntheta * p.nfg), dim3(kFFTThreads), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_fft_anal(const FFTParams& p0, hipStream_t s) {
+  FFTParams p = p0;
+  p.rings = kFFTElems / p.N;
+  p.nfg = (p.nfield + p.rings - 1) / p.rings;
+  hipLaunchKernelGGL(fft_anal, dim3((long)p.ntheta * p.nfg), dim3(kFFTThreads), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_lmax0(int inverse, int real, int nfield, const double* in, long istride,
+                        double* out, long ostride, int accumulate, hipStream_t s) {
+  hipLaunchKernelGGL(lmax0_kernel, dim3((nfield + 255) / 256), dim3(256), 0, s, inverse, real,
+                     nfield, in, istride, out, ostride, accumulate);
+  return hipGetLastError();
+}
+
+}  // namespace gsh

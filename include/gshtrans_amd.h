@@ -1,0 +1,128 @@
+/* gshtrans_amd.h -- C ABI of the MI355X (gfx950) GSH transform library.
+ *
+ * One shared library, gshtrans_amd/libgshtrans_hip.so, holds the HIP kernels
+ * and this plain-C entry layer: no C++ or torch types cross it, only
+ * integers, doubles, raw pointers and an opaque stream handle (hipStream_t
+ * passed as void*; NULL means the default stream).
+ *
+ * Each entry point replaces one piece of the reference's header-only C++ API
+ * (paths relative to the reference tree):
+ *
+ *   gsh_grid_create        GaussLegendreGrid(int lMax, int nMax, Flag)
+ *                          GSHTrans/src/GaussLegendreGrid.h:38-73
+ *   gsh_grid_colatitudes   GaussLegendreGrid::CoLatitudes / CoLatitudeWeights
+ *                          GaussLegendreGrid.h:89-92
+ *   gsh_inverse            GaussLegendreGrid::InverseTransformation
+ *                          GaussLegendreGrid.h:222-321 (synthesis)
+ *   gsh_forward            GaussLegendreGrid::ForwardTransformation
+ *                          GaussLegendreGrid.h:108-217 (analysis)
+ *   gsh_inverse_host /     the same two calls on host ranges, as the reference
+ *   gsh_forward_host       takes them (used by the C++ facade in include/GSHTrans)
+ *   gsh_coefficient_size   GridBase::{Real,Complex}CoefficientSize
+ *                          GSHTrans/src/GridBase.h:79-113 via
+ *                          GSHIndices::Size, GSHTrans/src/Indexing.h:148-150
+ *   gsh_gl_nodes           GaussQuad::GaussQuadrature + Transform(acos(-x))
+ *                          as called at GaussLegendreGrid.h:47-49
+ *   gsh_wigner_seed        WignerDetails::WignerMinOrder, Wigner.h:52-80
+ *   gsh_adjoint_inverse    no reference counterpart: the transpose of the
+ *                          synthesis, used for least-squares gradients
+ *
+ * Layouts are the reference's:
+ *   coefficients  GSHIndices<All>(lMax, lMax, n) for complex fields and
+ *                 GSHIndices<NonNegative>(lMax, lMax, n) for real fields
+ *                 (Indexing.h:104-146), interleaved complex (re, im)
+ *   field         row-major (iTheta, iPhi), (L+1) x 2L points
+ *                 (CanonicalComponentField.h:122-124, GaussLegendreGrid.h:94-103)
+ * A batch is `batch` such arrays spaced by a stride counted in elements of
+ * the array's own type (complex elements for coefficients and complex
+ * fields, doubles or floats for real fields).
+ *
+ * Errors: every entry returns a gsh_status; the reference only asserts
+ * (GaussLegendreGrid.h:41-44, 126-134, 240-248), which the C++ facade
+ * restores by asserting on non-zero status.
+ */
+#ifndef GSHTRANS_AMD_H
+#define GSHTRANS_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gsh_grid_s* gsh_grid_t;
+
+enum gsh_status {
+  GSH_OK = 0,
+  GSH_ERR_INVALID_ARGUMENT = 1, /* null pointer, negative size, bad enum   */
+  GSH_ERR_UPPER_INDEX = 2,      /* n not in UpperIndices() or |n| > lMax   */
+  GSH_ERR_DEGREE = 3,           /* lMax > grid lMax or lMax < 0            */
+  GSH_ERR_VALUE_KIND = 4,       /* complex field on an MRange=NonNegative grid */
+  GSH_ERR_UNSUPPORTED = 5,      /* precision/size combination not built    */
+  GSH_ERR_DEVICE = 6,           /* HIP runtime error                       */
+  GSH_ERR_NO_DEVICE = 7,        /* no gfx950 device visible                */
+  GSH_ERR_OUT_OF_MEMORY = 8
+};
+
+/* MRange / NRange tags (GSHTrans/src/Concepts.h:24-27). */
+enum gsh_range { GSH_ALL = 0, GSH_NONNEGATIVE = 1, GSH_SINGLE = 2 };
+
+/* Value type of the field (Concepts.h:53-54). */
+enum gsh_value_kind { GSH_REAL = 0, GSH_COMPLEX = 1 };
+
+/* Arithmetic of the Legendre contraction.  The Wigner recurrence, its seeds,
+ * the quadrature weights and the FFT are FP64 in every mode. */
+enum gsh_precision {
+  GSH_FP64 = 0,   /* FP64 MFMA, the reference's arithmetic                 */
+  GSH_FP32 = 1,   /* FP32 MFMA on FP32-rounded operands                     */
+  GSH_BF16X3 = 2, /* 2-way BF16 split, 3 products, FP32 accumulation        */
+  GSH_BF16X6 = 3  /* 3-way BF16 split, 6 products, FP32 accumulation with
+                     periodic FP64 flush                                     */
+};
+
+/* Storage type of a batched call's arrays. */
+enum gsh_storage { GSH_STORE_FP64 = 0, GSH_STORE_FP32 = 1 };
+
+const char* gsh_status_string(int status);
+int gsh_version(void);
+
+/* ---- host-only helpers (no device needed) ------------------------------ */
+int gsh_gl_nodes(int lmax, double* theta, double* weight);
+long gsh_coefficient_size(int lmax, int n, int value_kind);
+int gsh_wigner_seed(int l, int n, double theta, double* value);
+
+/* ---- grid -------------------------------------------------------------- */
+int gsh_grid_create(int lmax, int nmax, int mrange, int nrange, gsh_grid_t* grid);
+int gsh_grid_destroy(gsh_grid_t grid);
+int gsh_grid_info(gsh_grid_t grid, int* lmax, int* nmax, int* min_upper_index,
+                  int* ntheta, int* nphi);
+int gsh_grid_colatitudes(gsh_grid_t grid, double* theta, double* weight);
+/* Build the per-upper-index device data (seeds, recurrence coefficients)
+ * ahead of the first transform; transforms build it on demand otherwise. */
+int gsh_grid_prepare(gsh_grid_t grid, int n);
+
+/* ---- transforms on device memory -------------------------------------- */
+/* Synthesis: field = S_n(coeffs) for `batch` fields. */
+int gsh_inverse(gsh_grid_t grid, int lmax, int n, int value_kind, int precision,
+                int storage, int batch, const void* coeffs, long coeff_stride,
+                void* field, long field_stride, void* stream);
+/* Analysis: coeffs (+)= A_n(field).  accumulate != 0 adds into coeffs as the
+ * reference does (GaussLegendreGrid.h:192); 0 overwrites. */
+int gsh_forward(gsh_grid_t grid, int lmax, int n, int value_kind, int precision,
+                int storage, int batch, const void* field, long field_stride,
+                void* coeffs, long coeff_stride, int accumulate, void* stream);
+/* Adjoint of the synthesis: coeffs = S_n^H(field) (no quadrature weights). */
+int gsh_adjoint_inverse(gsh_grid_t grid, int lmax, int n, int value_kind,
+                        int precision, int storage, int batch, const void* field,
+                        long field_stride, void* coeffs, long coeff_stride,
+                        void* stream);
+
+/* ---- transforms on host memory (staged through device buffers) -------- */
+int gsh_inverse_host(gsh_grid_t grid, int lmax, int n, int value_kind,
+                     const double* coeffs, double* field);
+int gsh_forward_host(gsh_grid_t grid, int lmax, int n, int value_kind,
+                     const double* field, double* coeffs, int accumulate);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSHTRANS_AMD_H */
