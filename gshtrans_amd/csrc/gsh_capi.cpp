@@ -415,20 +415,20 @@ int gsh_grid_create(int lmax, int nmax, int mrange, int nrange, gsh_grid_t* grid
 
 int gsh_grid_destroy(gsh_grid_t g) {
   if (!g) return GSH_ERR_INVALID_ARGUMENT;
-  hipSetDevice(g->device);
-  hipDeviceSynchronize();
+  (void)hipSetDevice(g->device);
+  (void)hipDeviceSynchronize();
   for (auto& kv : g->ndata) {
-    hipFree(kv.second.seeds);
-    hipFree(kv.second.RA);
-    hipFree(kv.second.RB);
-    hipFree(kv.second.RC);
-    hipFree(kv.second.roff);
+    (void)hipFree(kv.second.seeds);
+    (void)hipFree(kv.second.RA);
+    (void)hipFree(kv.second.RB);
+    (void)hipFree(kv.second.RC);
+    (void)hipFree(kv.second.roff);
   }
-  hipFree(g->d_cos);
-  hipFree(g->d_wq);
-  hipFree(g->d_tw);
-  if (g->work) hipFree(g->work);
-  if (g->stage) hipFree(g->stage);
+  (void)hipFree(g->d_cos);
+  (void)hipFree(g->d_wq);
+  (void)hipFree(g->d_tw);
+  if (g->work) (void)hipFree(g->work);
+  if (g->stage) (void)hipFree(g->stage);
   delete g;
   return GSH_OK;
 }

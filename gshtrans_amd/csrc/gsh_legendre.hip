@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void leg_synth_f64(LegParams p) {
     for (int j = 0; j < T; j++) B[j] = Bn[j];
   }
 
-  // C layout of the 16x16 f64 MFMA: lane holds rows 4*(lane/16)+r, column lane%16.
+  // C layout of the 16x16x4 f64 MFMA: lane holds rows lane/16 + 4r, column lane%16.
 #pragma unroll
   for (int g = 0; g < 4; g++)
 #pragma unroll
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void leg_synth_f64(LegParams p) {
       if (col >= p.ncol) continue;
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int row = tsb * 256 + wave * 64 + g * 16 + 4 * krow + r;
+        const int row = tsb * 256 + wave * 64 + g * 16 + krow + 4 * r;
         if (row < p.ntheta) p.F[((long)row * p.nslot + slot) * p.ncol + col] = acc[g][j][r];
       }
     }
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
       for (int r = 0; r < 4; r++) {
         const double s = ((red[0][lt][r][lane] + red[1][lt][r][lane]) + red[2][lt][r][lane]) +
                          red[3][lt][r][lane];
-        const int l = lc + 16 * lt + 4 * krow + r;
+        const int l = lc + 16 * lt + krow + 4 * r;
         if (cv && l <= p.lmax) {
           const int f = col >> 1, pt = col & 1;
           const long ci = ((long)f * p.cstride + coef_index(l, m, p.nabs, p.real)) * 2 + pt;

@@ -105,7 +105,8 @@ def test_real_fields(torch, L, lmax, n):
 @pytest.mark.parametrize("L,lmax,n", [(64, 40, 1), (32, 5, -2), (8, 8, 2), (2, 2, 1),
                                       (1, 1, 0), (4, 2, 2), (256, 255, 0)])
 def test_truncation_and_small_grids(torch, L, lmax, n):
-    check_pair(torch, L, 2, lmax, n, False, 1.0, SEED0 + 11, grid=_grid(L, 2))
+    nmax = min(2, L)
+    check_pair(torch, L, nmax, lmax, n, False, 1.0, SEED0 + 11, grid=_grid(L, nmax))
 
 
 def test_nonnegative_grid_real(torch):
