@@ -1,0 +1,258 @@
+#!/usr/bin/env python3
+"""GSH transform throughput on one MI355X.
+
+Workload (the precision configuration the headline metric is quoted on):
+GaussLegendreGrid lMax=1024, nMax=2 (1025 x 2048 grid); a batch of 64
+real-valued n=0 fields and a batch of 64 complex n=2 fields, coefficients
+normal with (l+1)^-1 decay (seeded, synthetic).  One step = synthesis then
+analysis of every field = 256 transforms.  Inputs are resident in HBM before
+timing starts.
+
+Printed: one JSON line.  `value` is transforms/s of the headline mode (the
+mixed-precision Legendre contraction when built, else FP64); the FP64 figure,
+the speedup and the per-field relative L2 error of each mode against FP64 are
+reported beside it.  `roofline` is the dominant kernel's algorithmic rate
+(HIP events around each launch on its own stream) against the MI355X peak;
+`cpu_baseline` times the CPU oracle (a restatement of the reference's
+serial transform with its precomputed Wigner table) on a bounded sample.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GSH transforms/sec at lMax=1024, B=64: mixed vs FP64 GPU; rel. L2 err"
+L = 1024
+NMAX = 2
+BATCH = 64
+SEED = 20261202 + 2
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3, "bf16x6": 2500.0 / 6}
+HBM_GBPS = 8000.0
+MIXED_CANDIDATES = ["bf16x6", "fp32", "bf16x3"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_counts():
+    """Per-field algorithmic work: Legendre flops 4*ntheta*C (complex) or
+    4*ntheta*R (real); ring-FFT HBM bytes (field read/write + F bins)."""
+    import gshtrans_amd as G
+    nt, N = L + 1, 2 * L
+    R0 = G.coefficient_size(L, 0, real=True)
+    C2 = G.coefficient_size(L, 2, real=False)
+    leg = {"real": 4.0 * nt * R0, "complex": 4.0 * nt * C2}
+    fft_bytes = {"real": nt * N * 8 + nt * (L + 1) * 16,
+                 "complex": nt * N * 16 + nt * (2 * L + 1) * 16}
+    return leg, fft_bytes
+
+
+def run_gpu(args):
+    import torch
+    import gshtrans_amd as G
+
+    dev = torch.device("cuda:0")
+    grid = G.GaussLegendreGrid(L, NMAX)
+    grid.prepare(0)
+    grid.prepare(2)
+    log("[bench] generating inputs")
+    cr = torch.from_numpy(G.random_coefficients(L, 0, real=True, seed=SEED, batch=BATCH)).to(dev)
+    cc = torch.from_numpy(G.random_coefficients(L, 2, real=False, seed=SEED + 1,
+                                                batch=BATCH)).to(dev)
+    fr = torch.empty((BATCH, L + 1, 2 * L), dtype=torch.float64, device=dev)
+    fc = torch.empty((BATCH, L + 1, 2 * L), dtype=torch.complex128, device=dev)
+    outs = {}
+
+    def step(prec, o):
+        grid.inverse(L, 0, cr, field=o["fr"], real=True, precision=prec)
+        grid.forward(L, 0, o["fr"], coeffs=o["cr"], real=True, precision=prec)
+        grid.inverse(L, 2, cc, field=o["fc"], real=False, precision=prec)
+        grid.forward(L, 2, o["fc"], coeffs=o["cc"], real=False, precision=prec)
+
+    def buffers(prec):
+        if prec == "fp64":
+            return {"fr": fr, "fc": fc, "cr": torch.empty_like(cr), "cc": torch.empty_like(cc)}
+        return {"fr": torch.empty_like(fr), "fc": torch.empty_like(fc),
+                "cr": torch.empty_like(cr), "cc": torch.empty_like(cc)}
+
+    modes = ["fp64"]
+    mixed = None
+    for m in MIXED_CANDIDATES if args.precision is None else [args.precision]:
+        if m == "fp64":
+            break
+        try:
+            o = buffers(m)
+            step(m, o)
+            torch.cuda.synchronize()
+            mixed = m
+            outs[m] = o
+            break
+        except G.GSHError as e:
+            log(f"[bench] {m} not available: {e}")
+    if mixed:
+        modes.append(mixed)
+    outs["fp64"] = buffers("fp64")
+
+    results = {}
+    headline = mixed or "fp64"
+    for m in modes:
+        o = outs[m]
+        for _ in range(args.warmup):
+            step(m, o)
+        torch.cuda.synchronize()
+        if m == headline:
+            grid.timing_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(m, o)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        kt = grid.timing_read() if m == headline else None
+        grid.timing_enable(False)
+        tps = args.steps * 4 * BATCH / dt
+        results[m] = {"transforms_per_s": tps, "ms_per_step": 1e3 * dt / args.steps,
+                      "kernels": kt}
+        log(f"[bench] {m}: {tps:.1f} transforms/s, {1e3 * dt / args.steps:.2f} ms/step")
+
+    # Accuracy of each mode against FP64 per field (relative L2).
+    def per_field(a, b):
+        d = (a - b).reshape(a.shape[0], -1)
+        return (torch.linalg.vector_norm(d, dim=1) /
+                torch.linalg.vector_norm(b.reshape(b.shape[0], -1), dim=1)).cpu().numpy()
+
+    errors = {}
+    if mixed:
+        ref, o = outs["fp64"], outs[mixed]
+        e = {k: per_field(o[k], ref[k]) for k in ("fr", "cr", "fc", "cc")}
+        errors[mixed] = {
+            "synthesis_real_max": float(e["fr"].max()),
+            "analysis_real_max": float(e["cr"].max()),
+            "synthesis_complex_max": float(e["fc"].max()),
+            "analysis_complex_max": float(e["cc"].max()),
+            "max": float(max(v.max() for v in e.values())),
+            "median": float(sorted(x for v in e.values() for x in v)[
+                sum(len(v) for v in e.values()) // 2]),
+        }
+    # FP64 round trip as a size-independent self-check.
+    rt_r = per_field(outs["fp64"]["cr"], cr)
+    rt_c = per_field(outs["fp64"]["cc"], cc)
+    errors["fp64_round_trip_max"] = float(max(rt_r.max(), rt_c.max()))
+    return results, headline, mixed, errors
+
+
+def roofline(results, headline):
+    leg, fftb = algorithmic_counts()
+    kt = results[headline]["kernels"]
+    # per-step work: real batch + complex batch, one launch each per kernel
+    work = {
+        "legendre_synthesis": ("mfma", BATCH * (leg["real"] + leg["complex"])),
+        "legendre_analysis": ("mfma", BATCH * (leg["real"] + leg["complex"])),
+        "fft_synthesis": ("hbm", BATCH * (fftb["real"] + fftb["complex"])),
+        "fft_analysis": ("hbm", BATCH * (fftb["real"] + fftb["complex"])),
+    }
+    name = max(kt, key=lambda k: kt[k][0])
+    ms, launches = kt[name]
+    bound, per_step = work[name]
+    steps = launches / 2.0
+    total = per_step * steps
+    sec = ms / 1e3
+    if bound == "mfma":
+        achieved = total / sec / 1e12
+        peak = PEAK_TFLOPS[headline]
+        unit = "TFLOP/s"
+    else:
+        achieved = total / sec / 1e9
+        peak = HBM_GBPS
+        unit = "GB/s"
+    per_kernel = {k: {"avg_launch_ms": v[0] / max(v[1], 1), "launches": v[1]}
+                  for k, v in kt.items()}
+    return {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "traffic": None,
+            "avg_launch_ms": ms / max(launches, 1), "per_kernel": per_kernel}
+
+
+def cpu_baseline(args):
+    """The CPU oracle restating the reference's path: Wigner table built with
+    OpenMP on all host cores (setup, reported separately), then serial
+    per-field transforms streaming the table, as GaussLegendreGrid.h:163, 281.
+    Sample: one real n=0 field and one complex n=2 field, synthesis and
+    analysis each (4 transforms), on NRange=Single grids (8.6 GB table each)."""
+    import numpy as np
+    import gshtrans_amd as G
+    import oracle
+    t0 = time.perf_counter()
+    g0 = oracle.OracleGrid(L, 0, nrange="Single", table=True)
+    g2 = oracle.OracleGrid(L, 2, nrange="Single", table=True)
+    setup = time.perf_counter() - t0
+    cr = G.random_coefficients(L, 0, real=True, seed=SEED)
+    cc = G.random_coefficients(L, 2, real=False, seed=SEED + 1)
+    t0 = time.perf_counter()
+    f = g0.inverse(L, 0, cr, real=True)
+    g0.forward(L, 0, f, real=True)
+    f = g2.inverse(L, 2, cc)
+    g2.forward(L, 2, f)
+    dt = time.perf_counter() - t0
+    g0.close()
+    g2.close()
+    return {"value": 4.0 / dt, "unit": "transforms/s", "cores": 1, "kind": "port",
+            "sample": "lMax=1024: 1 real n=0 + 1 complex n=2 field, synthesis + analysis "
+                      "(4 transforms, serial as the reference) with precomputed Wigner tables",
+            "setup_s": setup, "setup_threads": oracle.lib().gsho_num_threads()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default=None,
+                    help="mixed mode to report (default: best available)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
+              "(the transform batch shards over GPUs without exchange; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    results, headline, mixed, errors = run_gpu(args)
+    rl = roofline(results, headline)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args)
+    head = results[headline]
+    fp64 = results["fp64"]
+    line = {
+        "metric": METRIC,
+        "value": head["transforms_per_s"],
+        "unit": "transforms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if headline == "fp64" else f"{headline} Legendre products, f64 recurrence/FFT",
+        "data": "synthetic (seeded normal coefficients, (l+1)^-1 decay)",
+        "config": {"workload": "GaussLegendreGrid lMax=1024 nMax=2: 64 real n=0 + 64 complex "
+                               "n=2 fields, synthesis then analysis (256 transforms/step)",
+                   "lmax": L, "batch_real": BATCH, "batch_complex": BATCH,
+                   "mode": headline},
+        "fp64_transforms_per_s": fp64["transforms_per_s"],
+        "mixed_transforms_per_s": results[mixed]["transforms_per_s"] if mixed else None,
+        "speedup_mixed_vs_fp64": (results[mixed]["transforms_per_s"] /
+                                  fp64["transforms_per_s"]) if mixed else None,
+        "rel_l2": errors,
+        "roofline": rl,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -36,7 +36,9 @@ EXPORTS = [
     "gsh_wigner_seed", "gsh_grid_create", "gsh_grid_destroy", "gsh_grid_info",
     "gsh_grid_colatitudes", "gsh_grid_prepare", "gsh_inverse", "gsh_forward",
     "gsh_adjoint_inverse", "gsh_inverse_host", "gsh_forward_host",
+    "gsh_timing_enable", "gsh_timing_read",
 ]
+KERNELS = ["legendre_synthesis", "fft_synthesis", "fft_analysis", "legendre_analysis"]
 
 
 class GSHError(RuntimeError):
@@ -76,10 +78,13 @@ def _lib():
         L.gsh_adjoint_inverse.argtypes = [vp, i, i, i, i, i, i, vp, l, vp, l, vp]
         L.gsh_inverse_host.argtypes = [vp, i, i, i, dp, dp]
         L.gsh_forward_host.argtypes = [vp, i, i, i, dp, dp, i]
+        L.gsh_timing_enable.argtypes = [vp, i]
+        L.gsh_timing_read.argtypes = [vp, dp, ctypes.POINTER(l), i]
         for name in ("gsh_gl_nodes", "gsh_wigner_seed", "gsh_grid_create", "gsh_grid_destroy",
                      "gsh_grid_info", "gsh_grid_colatitudes", "gsh_grid_prepare",
                      "gsh_inverse", "gsh_forward", "gsh_adjoint_inverse",
-                     "gsh_inverse_host", "gsh_forward_host"):
+                     "gsh_inverse_host", "gsh_forward_host", "gsh_timing_enable",
+                     "gsh_timing_read"):
             getattr(L, name).restype = i
         _LIB = L
     return _LIB
@@ -281,6 +286,17 @@ class GaussLegendreGrid:
 
     def prepare(self, n):
         _check(_lib().gsh_grid_prepare(self._h, n), "gsh_grid_prepare")
+
+    def timing_enable(self, enable=True):
+        """Record HIP events around every kernel launch (resets the totals)."""
+        _check(_lib().gsh_timing_enable(self._h, 1 if enable else 0), "gsh_timing_enable")
+
+    def timing_read(self):
+        """{kernel: (total_ms, launches)} since timing_enable()."""
+        ms = np.zeros(len(KERNELS))
+        cnt = (ctypes.c_long * len(KERNELS))()
+        _check(_lib().gsh_timing_read(self._h, _dp(ms), cnt, len(KERNELS)), "gsh_timing_read")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNELS)}
 
     # ---- batched device transforms --------------------------------------
     def inverse(self, lmax, n, coeffs, field=None, real=False, precision="fp64",

@@ -167,6 +167,12 @@ struct gsh_grid_s {
   double* stage = nullptr;  // host-call staging
   size_t stage_bytes = 0;
   std::mutex mu;
+  // Kernel timing (gsh_timing_*): event pairs per kernel id.
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[GSH_KERNEL_COUNT];
+  std::vector<hipEvent_t> event_pool;
+  double total_ms[GSH_KERNEL_COUNT] = {0, 0, 0, 0};
+  long launches[GSH_KERNEL_COUNT] = {0, 0, 0, 0};
 
   int min_upper() const { return nrange == GSH_ALL ? -nmax : (nrange == GSH_NONNEGATIVE ? 0 : nmax); }
   int ntheta() const { return L + 1; }
@@ -242,6 +248,29 @@ int ensure_ndata(gsh_grid_s* g, int n, NData** out) {
   auto res = g->ndata.emplace(n, d);
   *out = &res.first->second;
   return GSH_OK;
+}
+
+hipEvent_t take_event(gsh_grid_s* g) {
+  if (!g->event_pool.empty()) {
+    hipEvent_t e = g->event_pool.back();
+    g->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Runs `launch` and, when timing is on, brackets it with events on stream s.
+template <typename F>
+hipError_t timed(gsh_grid_s* g, int kid, hipStream_t s, F launch) {
+  if (!g->timing) return launch();
+  hipEvent_t a = take_event(g), b = take_event(g);
+  if (a) (void)hipEventRecord(a, s);
+  hipError_t e = launch();
+  if (b) (void)hipEventRecord(b, s);
+  if (a && b) g->pending[kid].emplace_back(a, b);
+  return e;
 }
 
 int ensure_work(gsh_grid_s* g, size_t bytes) {
@@ -427,6 +456,12 @@ int gsh_grid_destroy(gsh_grid_t g) {
   (void)hipFree(g->d_cos);
   (void)hipFree(g->d_wq);
   (void)hipFree(g->d_tw);
+  for (int k = 0; k < GSH_KERNEL_COUNT; k++)
+    for (auto& pr : g->pending[k]) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+  for (auto e : g->event_pool) (void)hipEventDestroy(e);
   if (g->work) (void)hipFree(g->work);
   if (g->stage) (void)hipFree(g->stage);
   delete g;
@@ -485,11 +520,12 @@ int gsh_inverse(gsh_grid_t g, int lmax, int n, int value_kind, int precision, in
   LegParams p = base_params(g, d, lmax, n, real, batch);
   p.cin = static_cast<const double*>(coeffs);
   p.cstride = coeff_stride;
-  GSH_HIP(gsh::launch_leg_synth_f64(p, s));
+  GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_SYNTHESIS, s,
+                [&] { return gsh::launch_leg_synth_f64(p, s); }));
   FFTParams q = fft_params(g, lmax, real, batch);
   q.fout = field;
   q.fstride = field_stride;
-  GSH_HIP(gsh::launch_fft_synth(q, s));
+  GSH_HIP(timed(g, GSH_KERNEL_FFT_SYNTHESIS, s, [&] { return gsh::launch_fft_synth(q, s); }));
   return GSH_OK;
 }
 
@@ -520,13 +556,53 @@ int gsh_forward(gsh_grid_t g, int lmax, int n, int value_kind, int precision, in
   FFTParams q = fft_params(g, lmax, real, batch);
   q.fin = field;
   q.fstride = field_stride;
-  GSH_HIP(gsh::launch_fft_anal(q, s));
+  GSH_HIP(timed(g, GSH_KERNEL_FFT_ANALYSIS, s, [&] { return gsh::launch_fft_anal(q, s); }));
   LegParams p = base_params(g, d, lmax, n, real, batch);
   p.cout = static_cast<double*>(coeffs);
   p.cstride = coeff_stride;
   p.accumulate = accumulate ? 1 : 0;
   p.zero_last = (!real && lmax == g->L) ? 1 : 0;
-  GSH_HIP(gsh::launch_leg_anal_f64(p, s));
+  GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_ANALYSIS, s,
+                [&] { return gsh::launch_leg_anal_f64(p, s); }));
+  return GSH_OK;
+}
+
+int gsh_timing_enable(gsh_grid_t g, int enable) {
+  if (!g) return GSH_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(g->mu);
+  for (int k = 0; k < GSH_KERNEL_COUNT; k++) {
+    for (auto& pr : g->pending[k]) {
+      (void)hipEventSynchronize(pr.second);
+      g->event_pool.push_back(pr.first);
+      g->event_pool.push_back(pr.second);
+    }
+    g->pending[k].clear();
+    g->total_ms[k] = 0;
+    g->launches[k] = 0;
+  }
+  g->timing = enable != 0;
+  return GSH_OK;
+}
+
+int gsh_timing_read(gsh_grid_t g, double* total_ms, long* launches, int nkernels) {
+  if (!g || nkernels < 0) return GSH_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(g->mu);
+  for (int k = 0; k < GSH_KERNEL_COUNT; k++) {
+    for (auto& pr : g->pending[k]) {
+      GSH_HIP(hipEventSynchronize(pr.second));
+      float ms = 0;
+      GSH_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+      g->total_ms[k] += ms;
+      g->launches[k] += 1;
+      g->event_pool.push_back(pr.first);
+      g->event_pool.push_back(pr.second);
+    }
+    g->pending[k].clear();
+  }
+  for (int k = 0; k < nkernels && k < GSH_KERNEL_COUNT; k++) {
+    if (total_ms) total_ms[k] = g->total_ms[k];
+    if (launches) launches[k] = g->launches[k];
+  }
   return GSH_OK;
 }
 

@@ -26,6 +26,9 @@
  *   gsh_wigner_seed        WignerDetails::WignerMinOrder, Wigner.h:52-80
  *   gsh_adjoint_inverse    no reference counterpart: the transpose of the
  *                          synthesis, used for least-squares gradients
+ *   gsh_timing_enable /    no reference counterpart (the reference has no
+ *   gsh_timing_read        timing code): HIP events around each kernel
+ *                          launch, on the launch stream, for benchmarks
  *
  * Layouts are the reference's:
  *   coefficients  GSHIndices<All>(lMax, lMax, n) for complex fields and
@@ -114,6 +117,21 @@ int gsh_adjoint_inverse(gsh_grid_t grid, int lmax, int n, int value_kind,
                         int precision, int storage, int batch, const void* field,
                         long field_stride, void* coeffs, long coeff_stride,
                         void* stream);
+
+/* ---- per-kernel timing ------------------------------------------------- */
+enum gsh_kernel {
+  GSH_KERNEL_LEGENDRE_SYNTHESIS = 0,
+  GSH_KERNEL_FFT_SYNTHESIS = 1,
+  GSH_KERNEL_FFT_ANALYSIS = 2,
+  GSH_KERNEL_LEGENDRE_ANALYSIS = 3,
+  GSH_KERNEL_COUNT = 4
+};
+/* enable != 0 records a HIP event pair around every kernel launch of this
+ * grid (on the launch stream) and clears the accumulated totals. */
+int gsh_timing_enable(gsh_grid_t grid, int enable);
+/* Waits for the recorded events and returns, per kernel id, the summed
+ * duration in milliseconds and the number of launches since enabling. */
+int gsh_timing_read(gsh_grid_t grid, double* total_ms, long* launches, int nkernels);
 
 /* ---- transforms on host memory (staged through device buffers) -------- */
 int gsh_inverse_host(gsh_grid_t grid, int lmax, int n, int value_kind,
