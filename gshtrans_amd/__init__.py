@@ -306,7 +306,8 @@ class GaussLegendreGrid:
         import torch
         c = coeffs if coeffs.dim() == 2 else coeffs.unsqueeze(0)
         B, C = c.shape
-        if C != coefficient_size(lmax, n, real):
+        expect = _lib().gsh_coefficient_size(lmax, n, REAL if real else COMPLEX)
+        if expect >= 0 and C != expect:
             raise ValueError("coefficient size mismatch")
         if field is None:
             field = torch.empty((B, self._lmax + 1, 2 * self._lmax),
@@ -324,7 +325,7 @@ class GaussLegendreGrid:
         import torch
         f = field if field.dim() == 3 else field.unsqueeze(0)
         B = f.shape[0]
-        C = coefficient_size(lmax, n, real)
+        C = max(int(_lib().gsh_coefficient_size(lmax, n, REAL if real else COMPLEX)), 1)
         if coeffs is None:
             coeffs = torch.zeros((B, C), dtype=torch.complex128, device=f.device)
         _check(_lib().gsh_forward(self._h, lmax, n, REAL if real else COMPLEX,

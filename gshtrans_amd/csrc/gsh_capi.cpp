@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstring>
 #include <limits>
@@ -141,9 +142,7 @@ void parallel_for(long n, F f) {
 // ---------------------------------------------------------------------------
 struct NData {
   double* seeds = nullptr;  // [(m+L)*ntheta + t]
-  double* RA = nullptr;
-  double* RB = nullptr;
-  double* RC = nullptr;
+  double4* rec = nullptr;  // {a, b, c, 0} per (m, l), see gsh_internal.h
   long* roff = nullptr;
 };
 
@@ -161,6 +160,10 @@ struct gsh_grid_s {
   double* d_cos = nullptr;
   double* d_wq = nullptr;
   double2* d_tw = nullptr;
+  double2* d_chirp = nullptr;  // Bluestein tables (ring length not a power of two)
+  double2* d_hf = nullptr;
+  double2* d_hb = nullptr;
+  int M = 0;                   // FFT length in LDS
   std::map<int, NData> ndata;
   double* work = nullptr;
   size_t work_bytes = 0;
@@ -208,9 +211,9 @@ int ensure_ndata(gsh_grid_s* g, int n, NData** out) {
   long total = 0;
   for (long m = -L; m <= L; m++) {
     roff[m + L] = total;
-    total += L - std::max(std::labs(m), nabs) + 1;
+    total += L - std::max(std::labs(m), nabs) + 1 + gsh::kRecPad;
   }
-  std::vector<double> RA(total, 0.0), RB(total, 0.0), RC(total, 0.0);
+  std::vector<double4> rec(total, make_double4(0.0, 0.0, 0.0, 0.0));
   parallel_for(norders, [&](long mi) {
     const long m = mi - L;
     const long l0 = std::max(std::labs(m), nabs);
@@ -228,22 +231,17 @@ int ensure_ndata(gsh_grid_s* g, int n, NData** out) {
             ((ll - 1) * den) * r2;
       }
       const long idx = roff[mi] + (l - l0);
-      RA[idx] = static_cast<double>(a * r1);
-      RB[idx] = static_cast<double>(b * r1);
-      RC[idx] = static_cast<double>(c);
+      rec[idx] = make_double4(static_cast<double>(a * r1), static_cast<double>(b * r1),
+                              static_cast<double>(c), 0.0);
     }
   });
   NData d;
   GSH_HIP(hipSetDevice(g->device));
   GSH_HIP(hipMalloc(&d.seeds, seeds.size() * sizeof(double)));
-  GSH_HIP(hipMalloc(&d.RA, total * sizeof(double)));
-  GSH_HIP(hipMalloc(&d.RB, total * sizeof(double)));
-  GSH_HIP(hipMalloc(&d.RC, total * sizeof(double)));
+  GSH_HIP(hipMalloc(&d.rec, total * sizeof(double4)));
   GSH_HIP(hipMalloc(&d.roff, norders * sizeof(long)));
   GSH_HIP(hipMemcpy(d.seeds, seeds.data(), seeds.size() * sizeof(double), hipMemcpyHostToDevice));
-  GSH_HIP(hipMemcpy(d.RA, RA.data(), total * sizeof(double), hipMemcpyHostToDevice));
-  GSH_HIP(hipMemcpy(d.RB, RB.data(), total * sizeof(double), hipMemcpyHostToDevice));
-  GSH_HIP(hipMemcpy(d.RC, RC.data(), total * sizeof(double), hipMemcpyHostToDevice));
+  GSH_HIP(hipMemcpy(d.rec, rec.data(), total * sizeof(double4), hipMemcpyHostToDevice));
   GSH_HIP(hipMemcpy(d.roff, roff.data(), norders * sizeof(long), hipMemcpyHostToDevice));
   auto res = g->ndata.emplace(n, d);
   *out = &res.first->second;
@@ -316,9 +314,7 @@ LegParams base_params(gsh_grid_s* g, const NData* d, int lmax, int n, int real, 
   p.ncol = 2 * batch;
   p.cosv = g->d_cos;
   p.seeds = d->seeds;
-  p.RA = d->RA;
-  p.RB = d->RB;
-  p.RC = d->RC;
+  p.rec = d->rec;
   p.roff = d->roff;
   p.F = g->work;
   return p;
@@ -327,6 +323,10 @@ LegParams base_params(gsh_grid_s* g, const NData* d, int lmax, int n, int real, 
 FFTParams fft_params(gsh_grid_s* g, int lmax, int real, int batch) {
   FFTParams q{};
   q.N = g->nphi();
+  q.M = g->M;
+  q.chirp = g->d_chirp;
+  q.hf = g->d_hf;
+  q.hb = g->d_hb;
   q.L = g->L;
   q.ntheta = g->ntheta();
   q.lmax = lmax;
@@ -341,7 +341,37 @@ FFTParams fft_params(gsh_grid_s* g, int lmax, int real, int batch) {
   return q;
 }
 
-bool fft_supported(int N) { return N >= 2 && N <= 4096 && (N & (N - 1)) == 0; }
+bool is_pow2(int N) { return N >= 1 && (N & (N - 1)) == 0; }
+
+// LDS transform length for ring length N: N itself when a power of two,
+// else the Bluestein convolution length (smallest power of two >= 2N - 1).
+int fft_length(int N) {
+  if (is_pow2(N)) return N;
+  int M = 1;
+  while (M < 2 * N - 1) M <<= 1;
+  return M;
+}
+
+// In-place radix-2 FFT in long double (host, set-up only), sign -1.
+void fft_ld(std::vector<std::complex<long double>>& a) {
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; i++) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    const long double ang = -2.0L * kPiL / len;
+    for (size_t i = 0; i < n; i += len)
+      for (size_t k = 0; k < len / 2; k++) {
+        const std::complex<long double> w(std::cos(ang * k), std::sin(ang * k));
+        const auto u = a[i + k], v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+  }
+}
 
 }  // namespace
 
@@ -400,7 +430,7 @@ int gsh_grid_create(int lmax, int nmax, int mrange, int nrange, gsh_grid_t* grid
   if (mrange != GSH_ALL && mrange != GSH_NONNEGATIVE) return GSH_ERR_INVALID_ARGUMENT;
   if (nrange != GSH_ALL && nrange != GSH_NONNEGATIVE && nrange != GSH_SINGLE)
     return GSH_ERR_INVALID_ARGUMENT;
-  if (lmax > 0 && !fft_supported(2 * lmax)) return GSH_ERR_UNSUPPORTED;
+  if (lmax > 0 && fft_length(2 * lmax) > 4096) return GSH_ERR_UNSUPPORTED;
   if ((lmax + 1 + 63) / 64 > 36) return GSH_ERR_UNSUPPORTED;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return GSH_ERR_NO_DEVICE;
@@ -427,10 +457,47 @@ int gsh_grid_create(int lmax, int nmax, int mrange, int nrange, gsh_grid_t* grid
     cs[i] = std::cos(g->theta[i]);
     wq[i] = g->weight[i] * scale;
   }
-  std::vector<double2> tw(std::max(N, 1));
-  for (int k = 0; k < N; k++) {
-    const long double a = -2.0L * kPiL * k / N;
+  const int M = N > 0 ? fft_length(N) : 1;
+  g->M = M;
+  std::vector<double2> tw(M);
+  for (int k = 0; k < M; k++) {
+    const long double a = -2.0L * kPiL * k / M;
     tw[k] = make_double2(static_cast<double>(std::cos(a)), static_cast<double>(std::sin(a)));
+  }
+  if (N > 0 && M != N) {
+    // Bluestein: chirp w_k = exp(-i pi k^2 / N); forward filter conj(w),
+    // backward filter w, both circularly extended to M and transformed.
+    std::vector<std::complex<long double>> w(N), hf(M), hb(M);
+    for (int k = 0; k < N; k++) {
+      const long long kk = (static_cast<long long>(k) * k) % (2LL * N);
+      const long double a = -kPiL * kk / N;
+      w[k] = std::complex<long double>(std::cos(a), std::sin(a));
+    }
+    for (int k = 0; k < N; k++) {
+      hf[k] = std::conj(w[k]);
+      hb[k] = w[k];
+      if (k > 0) {
+        hf[M - k] = std::conj(w[k]);
+        hb[M - k] = w[k];
+      }
+    }
+    fft_ld(hf);
+    fft_ld(hb);
+    std::vector<double2> cw(N), chf(M), chb(M);
+    for (int k = 0; k < N; k++)
+      cw[k] = make_double2(static_cast<double>(w[k].real()), static_cast<double>(w[k].imag()));
+    for (int k = 0; k < M; k++) {
+      chf[k] = make_double2(static_cast<double>(hf[k].real() / M),
+                            static_cast<double>(hf[k].imag() / M));
+      chb[k] = make_double2(static_cast<double>(hb[k].real() / M),
+                            static_cast<double>(hb[k].imag() / M));
+    }
+    GSH_HIP(hipMalloc(&g->d_chirp, N * sizeof(double2)));
+    GSH_HIP(hipMalloc(&g->d_hf, M * sizeof(double2)));
+    GSH_HIP(hipMalloc(&g->d_hb, M * sizeof(double2)));
+    GSH_HIP(hipMemcpy(g->d_chirp, cw.data(), N * sizeof(double2), hipMemcpyHostToDevice));
+    GSH_HIP(hipMemcpy(g->d_hf, chf.data(), M * sizeof(double2), hipMemcpyHostToDevice));
+    GSH_HIP(hipMemcpy(g->d_hb, chb.data(), M * sizeof(double2), hipMemcpyHostToDevice));
   }
   GSH_HIP(hipMalloc(&g->d_cos, cs.size() * sizeof(double)));
   GSH_HIP(hipMalloc(&g->d_wq, wq.size() * sizeof(double)));
@@ -448,14 +515,15 @@ int gsh_grid_destroy(gsh_grid_t g) {
   (void)hipDeviceSynchronize();
   for (auto& kv : g->ndata) {
     (void)hipFree(kv.second.seeds);
-    (void)hipFree(kv.second.RA);
-    (void)hipFree(kv.second.RB);
-    (void)hipFree(kv.second.RC);
+    (void)hipFree(kv.second.rec);
     (void)hipFree(kv.second.roff);
   }
   (void)hipFree(g->d_cos);
   (void)hipFree(g->d_wq);
   (void)hipFree(g->d_tw);
+  if (g->d_chirp) (void)hipFree(g->d_chirp);
+  if (g->d_hf) (void)hipFree(g->d_hf);
+  if (g->d_hb) (void)hipFree(g->d_hb);
   for (int k = 0; k < GSH_KERNEL_COUNT; k++)
     for (auto& pr : g->pending[k]) {
       (void)hipEventDestroy(pr.first);
@@ -552,7 +620,8 @@ int gsh_forward(gsh_grid_t g, int lmax, int n, int value_kind, int precision, in
   NData* d;
   if ((st = ensure_ndata(g, n, &d))) return st;
   const int nslot = real ? lmax + 1 : 2 * lmax + 1;
-  if ((st = ensure_work(g, sizeof(double) * g->ntheta() * nslot * 2 * (size_t)batch))) return st;
+  const size_t fa = (size_t)nslot * ((2 * (size_t)batch + 15) / 16) * ((g->ntheta() + 3) / 4) * 64;
+  if ((st = ensure_work(g, sizeof(double) * fa))) return st;
   FFTParams q = fft_params(g, lmax, real, batch);
   q.fin = field;
   q.fstride = field_stride;

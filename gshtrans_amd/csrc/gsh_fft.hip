@@ -4,10 +4,13 @@
 // the analysis, :266-319 for the synthesis): an unnormalised c2c/c2r backward
 // transform of the order bins F_m placed at k = m (m >= 0) and k = N + m
 // (m < 0), and an unnormalised c2c/r2c forward transform of each ring.  Here
-// each workgroup loads `rings` rings of N = 2L points (rings * N = 4096
-// complex values, 64 KiB) into LDS, runs a Stockham radix-4 (+ one radix-2)
-// autosort FFT in place (values go through registers between barriers), and
-// writes the rings back.  The synthesis kernel also assembles the bins from
+// each workgroup loads `rings` rings of N = 2L points into LDS (64 KiB of
+// complex values), runs a Stockham radix-4 (+ one radix-2) autosort FFT in
+// place (values go through registers between barriers), and writes the rings
+// back.  A ring length that is not a power of two is transformed with
+// Bluestein's chirp-z identity jk = (j^2 + k^2 - (k-j)^2)/2: chirp multiply,
+// power-of-two FFT of length M >= 2N-1, multiply by the transformed chirp
+// filter, inverse FFT, chirp multiply (Bluestein 1970).  The synthesis kernel also assembles the bins from
 // the Legendre work buffer F[theta][slot][col]; the analysis kernel scatters
 // X_m * w_theta * 2 pi / N back into F for the Legendre contraction.
 #include "gsh_internal.h"
@@ -45,7 +48,6 @@ __device__ void stockham(double2* buf, int N, int rings, const double2* __restri
     constexpr int kPer = kFFTElems / 4 / kFFTThreads;
     double2 v[kPer][4];
     int dst[kPer];
-    int qq[kPer];
 #pragma unroll
     for (int i = 0; i < kPer; i++) {
       const int bi = threadIdx.x + i * kFFTThreads;
@@ -75,7 +77,6 @@ __device__ void stockham(double2* buf, int N, int rings, const double2* __restri
       v[i][2] = csub(t0, t2);
       v[i][3] = csub(t1, t3);
       dst[i] = q * N + (j / Ns) * Ns * 4 + k;
-      qq[i] = q;
     }
     __syncthreads();
 #pragma unroll
@@ -86,7 +87,6 @@ __device__ void stockham(double2* buf, int N, int rings, const double2* __restri
       buf[dst[i] + 2 * Ns] = v[i][2];
       buf[dst[i] + 3 * Ns] = v[i][3];
     }
-    (void)qq;
     __syncthreads();
     Ns *= 4;
     logN -= 2;
@@ -127,20 +127,51 @@ __device__ void stockham(double2* buf, int N, int rings, const double2* __restri
   }
 }
 
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+
+// Unnormalised DFT of length N of each ring (ring q at buf[q*M], zero from N
+// to M when Bluestein is used).  SIGN = -1 forward, +1 backward.
+template <int SIGN>
+__device__ void ring_dft(double2* buf, const FFTParams& p) {
+  const int N = p.N, M = p.M, R = p.rings;
+  if (M == N) {
+    stockham<SIGN>(buf, N, R, p.tw);
+    return;
+  }
+  for (int idx = threadIdx.x; idx < R * N; idx += kFFTThreads) {
+    const int q = idx / N, j = idx - q * N;
+    const double2 w = SIGN < 0 ? p.chirp[j] : cconj(p.chirp[j]);
+    buf[q * M + j] = cmul(buf[q * M + j], w);
+  }
+  __syncthreads();
+  stockham<-1>(buf, M, R, p.tw);
+  const double2* h = SIGN < 0 ? p.hf : p.hb;
+  for (int idx = threadIdx.x; idx < R * M; idx += kFFTThreads)
+    buf[idx] = cmul(buf[idx], h[idx & (M - 1)]);
+  __syncthreads();
+  stockham<+1>(buf, M, R, p.tw);
+  for (int idx = threadIdx.x; idx < R * N; idx += kFFTThreads) {
+    const int q = idx / N, j = idx - q * N;
+    const double2 w = SIGN < 0 ? p.chirp[j] : cconj(p.chirp[j]);
+    buf[q * M + j] = cmul(buf[q * M + j], w);
+  }
+  __syncthreads();
+}
+
 // Synthesis: bins from F, backward FFT, write field rings.
 __global__ __launch_bounds__(kFFTThreads) void fft_synth(FFTParams p) {
   __shared__ double2 buf[kFFTElems];
   const int theta = blockIdx.x / p.nfg;
   const int fg = blockIdx.x % p.nfg;
-  const int N = p.N, R = p.rings;
+  const int N = p.N, M = p.M, R = p.rings;
   const long frow = (long)theta * p.nslot * p.ncol;
   // Fill: consecutive threads read consecutive fields of one bin.
-  for (int idx = threadIdx.x; idx < R * N; idx += kFFTThreads) {
+  for (int idx = threadIdx.x; idx < R * M; idx += kFFTThreads) {
     const int q = idx % R;
     const int k = idx / R;
     const int f = fg * R + q;
     double2 X = make_double2(0.0, 0.0);
-    if (f < p.nfield) {
+    if (f < p.nfield && k < N) {
       const double* Fp = p.F + frow + 2 * f;
       if (!p.real) {
         if (k <= p.lmax) {
@@ -162,10 +193,10 @@ __global__ __launch_bounds__(kFFTThreads) void fft_synth(FFTParams p) {
         }
       }
     }
-    buf[q * N + k] = X;
+    buf[q * M + k] = X;
   }
   __syncthreads();
-  stockham<+1>(buf, N, R, p.tw);
+  ring_dft<+1>(buf, p);
   for (int idx = threadIdx.x; idx < R * N; idx += kFFTThreads) {
     const int q = idx / N;
     const int j = idx - q * N;
@@ -173,9 +204,9 @@ __global__ __launch_bounds__(kFFTThreads) void fft_synth(FFTParams p) {
     if (f >= p.nfield) continue;
     const long o = (long)f * p.fstride + (long)theta * N + j;
     if (p.real)
-      static_cast<double*>(p.fout)[o] = buf[idx].x;
+      static_cast<double*>(p.fout)[o] = buf[q * M + j].x;
     else
-      static_cast<double2*>(p.fout)[o] = buf[idx];
+      static_cast<double2*>(p.fout)[o] = buf[q * M + j];
   }
 }
 
@@ -184,13 +215,13 @@ __global__ __launch_bounds__(kFFTThreads) void fft_anal(FFTParams p) {
   __shared__ double2 buf[kFFTElems];
   const int theta = blockIdx.x / p.nfg;
   const int fg = blockIdx.x % p.nfg;
-  const int N = p.N, R = p.rings;
-  for (int idx = threadIdx.x; idx < R * N; idx += kFFTThreads) {
-    const int q = idx / N;
-    const int j = idx - q * N;
+  const int N = p.N, M = p.M, R = p.rings;
+  for (int idx = threadIdx.x; idx < R * M; idx += kFFTThreads) {
+    const int q = idx / M;
+    const int j = idx - q * M;
     const int f = fg * R + q;
     double2 x = make_double2(0.0, 0.0);
-    if (f < p.nfield) {
+    if (f < p.nfield && j < N) {
       const long o = (long)f * p.fstride + (long)theta * N + j;
       if (p.real)
         x.x = static_cast<const double*>(p.fin)[o];
@@ -200,9 +231,8 @@ __global__ __launch_bounds__(kFFTThreads) void fft_anal(FFTParams p) {
     buf[idx] = x;
   }
   __syncthreads();
-  stockham<-1>(buf, N, R, p.tw);
+  ring_dft<-1>(buf, p);
   const double w = p.wq[theta];
-  const long frow = (long)theta * p.nslot * p.ncol;
   for (int idx = threadIdx.x; idx < R * p.nslot; idx += kFFTThreads) {
     const int q = idx % R;
     const int s = idx / R;
@@ -210,10 +240,17 @@ __global__ __launch_bounds__(kFFTThreads) void fft_anal(FFTParams p) {
     if (f >= p.nfield) continue;
     const int m = s + p.m_first;
     const int k = m >= 0 ? m : N + m;
-    const double2 X = buf[q * N + k];
-    double* Fp = p.F + frow + (long)s * p.ncol + 2 * f;
+    const double2 X = buf[q * M + k];
+    double* Fp = p.F + fa_index(s, 2 * f, theta, p.n_cg, p.nquad);
     Fp[0] = X.x * w;
     Fp[1] = X.y * w;
+    // The last ring also zeroes the padding rows of the last colatitude
+    // quad, which the Legendre contraction reads as B operands.
+    for (int tp = theta + 1; theta == p.ntheta - 1 && tp < 4 * p.nquad; tp++) {
+      double* Zp = p.F + fa_index(s, 2 * f, tp, p.n_cg, p.nquad);
+      Zp[0] = 0.0;
+      Zp[1] = 0.0;
+    }
   }
 }
 
@@ -255,7 +292,8 @@ __global__ void lmax0_kernel(int inverse, int real, int nfield, const double* in
 
 hipError_t launch_fft_synth(const FFTParams& p0, hipStream_t s) {
   FFTParams p = p0;
-  p.rings = kFFTElems / p.N;
+  if (p.M > kFFTElems) return hipErrorInvalidValue;
+  p.rings = kFFTElems / p.M;
   p.nfg = (p.nfield + p.rings - 1) / p.rings;
   hipLaunchKernelGGL(fft_synth, dim3((long)p.ntheta * p.nfg), dim3(kFFTThreads), 0, s, p);
   return hipGetLastError();
@@ -263,7 +301,10 @@ hipError_t launch_fft_synth(const FFTParams& p0, hipStream_t s) {
 
 hipError_t launch_fft_anal(const FFTParams& p0, hipStream_t s) {
   FFTParams p = p0;
-  p.rings = kFFTElems / p.N;
+  p.n_cg = (p.ncol + 15) / 16;
+  p.nquad = (p.ntheta + 3) / 4;
+  if (p.M > kFFTElems) return hipErrorInvalidValue;
+  p.rings = kFFTElems / p.M;
   p.nfg = (p.nfield + p.rings - 1) / p.rings;
   hipLaunchKernelGGL(fft_anal, dim3((long)p.ntheta * p.nfg), dim3(kFFTThreads), 0, s, p);
   return hipGetLastError();

@@ -8,12 +8,23 @@
 
 namespace gsh {
 
+// Zero recurrence entries kept after degree L for every order, so that
+// fixed-size chunks of the l-loop may step past L without bounds checks.
+constexpr int kRecPad = 64;
+
 // Parameters of one Legendre-contraction launch (synthesis, analysis or
 // adjoint) for a batch of fields sharing (lmax, n, value kind).
 //
-// Work layout in HBM ("F"): F[theta][slot][col] doubles, where slot indexes
-// the order m (slot = m - m_first; m_first = -lmax for complex fields, 0 for
-// real fields) and col = 2*field + part (part 0 = real, 1 = imaginary).
+// Work layouts in HBM.  slot indexes the order m (slot = m - m_first;
+// m_first = -lmax for complex fields, 0 for real fields) and
+// col = 2*field + part (part 0 = real, 1 = imaginary).
+//   synthesis  F[theta][slot][col]: the Legendre kernel writes F_m(theta),
+//              the ring FFT reads all orders of one ring
+//   analysis   FA[slot][cg][quad][64]: the ring FFT writes the weighted bins
+//              X'_m(theta) in the lane order of the 4x16 MFMA B operand,
+//              quad = theta/4, cg = col/16, lane = 16*(theta%4) + col%16,
+//              so one colatitude quad of one column group is 512 contiguous
+//              bytes; see fa_index().
 struct LegParams {
   int L;          // grid degree; ntheta = L + 1
   int ntheta;
@@ -33,24 +44,38 @@ struct LegParams {
   // per-grid / per-n tables
   const double* cosv;   // cos(theta_t), t < ntheta
   const double* seeds;  // Ortho-normalised d^{l0}_{m n}(theta_t) at [(m+L)*ntheta + t]
-  const double* RA;     // recurrence: e_l = (RA*cos - RB) e_{l-1} - RC e_{l-2}
-  const double* RB;
-  const double* RC;
-  const long* roff;     // [m + L]: index of (m, l0) in RA/RB/RC
+  // Recurrence coefficients {a, b, c, 0} per (m, l):
+  //   e_l = (a cos - b) e_{l-1} - c e_{l-2}
+  // rec[roff[m + L] + (l - l0)], l0 <= l <= L + kRecPad, zero beyond L.
+  const double4* rec;
+  const long* roff;
   int accumulate;       // analysis: coefficients += result
   int zero_last;        // analysis: zero the (L, L) coefficient (complex, lmax == L)
   int n_tsb;            // synthesis: theta super-blocks of 256
   int n_cg;             // column groups
-  int nblk;             // analysis: theta blocks of 64
+  int nquad;            // analysis: colatitude quads, ceil(ntheta / 4)
 };
+
+// Index of (slot, col, theta) in the analysis work layout FA.
+__host__ __device__ inline long fa_index(int slot, int col, int theta, int n_cg, int nquad) {
+  return (((long)slot * n_cg + (col >> 4)) * nquad + (theta >> 2)) * 64 + 16 * (theta & 3) +
+         (col & 15);
+}
 
 // Parameters of one ring-FFT launch.
 struct FFTParams {
-  int N;            // ring length 2L (power of two)
+  int N;            // ring length 2L
+  int M;            // transform length in LDS: N if N is a power of two, else
+                    // the Bluestein convolution length (power of two >= 2N-1)
   int L, ntheta, lmax, nslot, ncol, nfield, real, m_first;
-  int rings;        // rings per workgroup (rings * N == 4096 or N == 4096)
+  int rings;        // rings per workgroup, rings * M <= 4096
   int nfg;          // field groups
-  const double2* tw;  // tw[k] = exp(-2 pi i k / N)
+  int n_cg;         // analysis: 16-column groups, ceil(ncol / 16)
+  int nquad;        // analysis: colatitude quads
+  const double2* tw;  // tw[k] = exp(-2 pi i k / M), k < M
+  const double2* chirp;  // Bluestein: exp(-i pi k^2 / N), k < N
+  const double2* hf;     // Bluestein: FFT_M of the forward filter / M
+  const double2* hb;     // Bluestein: FFT_M of the backward filter / M
   const double* wq;   // analysis weights w_t * 2 pi / N
   double* F;
   const void* fin;    // analysis input field

@@ -9,21 +9,32 @@
 // reading D from a precomputed O(L^3) table (Wigner.h:122-130, 305-314).
 // Here D is generated in registers by the three-term recurrence in l
 // (Wigner.h:427-445), one lane per colatitude, and never touches HBM:
-//   e_l = (RA_l cos(theta) - RB_l) e_{l-1} - RC_l e_{l-2},   e_{l0-1} = 0,
+//   e_l = (a_l cos(theta) - b_l) e_{l-1} - c_l e_{l-2},   e_{l0-1} = 0,
 // with e_{l0} the Ortho-normalised seed of Wigner.h:52-95 (computed on the
-// host with the reference's libm expression) and RA/RB/RC the recurrence
+// host with the reference's libm expression) and (a, b, c) the recurrence
 // coefficients with the Ortho factor sqrt(2l+1)/(2 sqrt(pi)) folded in.
 //
 // The sums over l (synthesis) or theta (analysis) are dense contractions per
 // order m: (theta x l) . (l x columns) and (l x theta) . (theta x columns),
 // columns = (real, imaginary) x fields.  Recurrence lanes hold one theta
-// each; a per-wave LDS transpose turns them into MFMA operands.
+// each; a per-wave LDS transpose turns them into MFMA operands.  The loop
+// bodies are branch-free (coefficient tables are zero-padded past L and
+// loads past lmax are masked by selects), which keeps the MFMA accumulators
+// resident in AGPRs across iterations.
 #include "gsh_internal.h"
 
 namespace gsh {
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+// Recurrence tables are read through the constant address space: they are
+// never written while a kernel runs, and uniform loads from it become scalar
+// (s_load) loads even when the kernel stores to global memory in the loop.
+typedef const __attribute__((address_space(4))) d4 rec_t;
+
+__device__ __forceinline__ rec_t* rec_ptr(const double4* rec, long off) {
+  return (rec_t*)(rec) + off;
+}
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -45,12 +56,10 @@ __device__ __forceinline__ long coef_index(int l, int m, int nabs, int real) {
               : (long)l * l - (long)nabs * nabs + m + l;
 }
 
-// One recurrence step to degree l (uniform coefficients, per-lane cos).
-__device__ __forceinline__ void rec_step(const LegParams& p, long ridx, double cs, double& e0,
-                                         double& e1) {
-  const double a = p.RA[ridx], b = p.RB[ridx], c = p.RC[ridx];
-  const double t = fma(a, cs, -b);
-  const double en = fma(t, e0, -(c * e1));
+// One recurrence step with uniform coefficients q and per-lane cos.
+__device__ __forceinline__ void rec_step(const d4 q, double cs, double& e0, double& e1) {
+  const double t = fma(q.x, cs, -q.y);
+  const double en = fma(t, e0, -(q.z * e1));
   e1 = e0;
   e0 = en;
 }
@@ -60,9 +69,10 @@ __device__ __forceinline__ void rec_step(const LegParams& p, long ridx, double c
 // order.  Per 4 degrees: each lane advances its recurrence 4 steps, the wave
 // transposes the 4 x 64 block through LDS into four 16x4 A operands (one per
 // 16-colatitude group), the coefficients of those 4 degrees form the 4x16 B
-// operands, and 4*T MFMAs accumulate F[theta][col].
+// operands, and 4*T MFMAs accumulate F[theta][col].  Coefficients of the
+// next 4 degrees are loaded one chunk ahead.
 // ---------------------------------------------------------------------------
-template <int T>
+template <int T, int REAL>
 __global__ __launch_bounds__(256) void leg_synth_f64(LegParams p) {
   __shared__ double tr[4][4][66];
   const int lane = threadIdx.x & 63;
@@ -72,27 +82,36 @@ __global__ __launch_bounds__(256) void leg_synth_f64(LegParams p) {
   blk /= p.n_tsb;
   const int cg = blk % p.n_cg;
   const int mw = blk / p.n_cg;
-  const int m = order_of(mw, p.real);
+  const int m = order_of(mw, REAL);
   const int slot = m - p.m_first;
   const int l0 = max(abs(m), p.nabs);
+  const int lmax = p.lmax;
   const int t = tsb * 256 + wave * 64 + lane;
   const bool tv = t < p.ntheta;
   const double cs = tv ? p.cosv[t] : 0.0;
   double e0 = tv ? p.seeds[(long)(m + p.L) * p.ntheta + t] : 0.0;
   double e1 = 0.0;
-  const long rbase = p.roff[m + p.L] - l0;
+  rec_t* rc = rec_ptr(p.rec, p.roff[m + p.L] - l0);  // rc[l]: step to degree l
+  const int krow = lane >> 4;
 
-  // B-operand column of this lane for each column tile.
-  int fld[T], part[T];
+  // B-operand column of this lane per column tile.  Every load reads a valid
+  // address (column and degree clamped); out-of-range values are zeroed by a
+  // select when the operand is used, so loads stay in flight across a chunk.
+  const double* cptr[T];
   bool cv[T];
 #pragma unroll
   for (int j = 0; j < T; j++) {
     const int col = cg * 16 * T + 16 * j + (lane & 15);
     cv[j] = col < p.ncol;
-    fld[j] = col >> 1;
-    part[j] = col & 1;
+    const int cc = cv[j] ? col : 0;
+    cptr[j] = p.cin + (long)(cc >> 1) * p.cstride * 2 + (cc & 1);
   }
-  const int krow = lane >> 4;
+  auto loadB = [&](int l, double* out) {
+    const int lk = min(l + krow, lmax);
+    const long ci = 2 * coef_index(lk, m, p.nabs, REAL);
+#pragma unroll
+    for (int j = 0; j < T; j++) out[j] = cptr[j][ci];
+  };
 
   d4 acc[4][T];
 #pragma unroll
@@ -100,42 +119,41 @@ __global__ __launch_bounds__(256) void leg_synth_f64(LegParams p) {
 #pragma unroll
     for (int j = 0; j < T; j++) acc[g][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-  // Prefetched B operands for the current 4 degrees.
-  double B[T];
-  auto loadB = [&](int l, double* out) {
-    const int lk = l + krow;
-    const bool lv = lk <= p.lmax;
-    const long ci = coef_index(lk, m, p.nabs, p.real);
-#pragma unroll
-    for (int j = 0; j < T; j++)
-      out[j] = (lv && cv[j]) ? p.cin[((long)fld[j] * p.cstride + ci) * 2 + part[j]] : 0.0;
-  };
-  loadB(l0, B);
-
-  for (int l = l0; l <= p.lmax; l += 4) {
-    double Bn[T];
-    loadB(l + 4, Bn);
-    double v[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int lk = l + k;
-      v[k] = (lk <= p.lmax) ? e0 : 0.0;
-      if (lk + 1 <= p.lmax) rec_step(p, rbase + lk + 1, cs, e0, e1);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++) tr[wave][k][lane] = v[k];
+  // One 4-degree chunk: recurrence, LDS transpose, B mask, 4*T MFMAs.
+  auto chunk = [&](int l, double* B) {
+    const d4 q1 = rc[l + 1], q2 = rc[l + 2], q3 = rc[l + 3], q4 = rc[l + 4];
+    tr[wave][0][lane] = e0;
+    rec_step(q1, cs, e0, e1);
+    tr[wave][1][lane] = e0;
+    rec_step(q2, cs, e0, e1);
+    tr[wave][2][lane] = e0;
+    rec_step(q3, cs, e0, e1);
+    tr[wave][3][lane] = e0;
+    rec_step(q4, cs, e0, e1);
     wave_sync();
     double A[4];
 #pragma unroll
     for (int g = 0; g < 4; g++) A[g] = tr[wave][krow][g * 16 + (lane & 15)];
     wave_sync();
+    const bool lv = l + krow <= lmax;
 #pragma unroll
     for (int g = 0; g < 4; g++)
 #pragma unroll
       for (int j = 0; j < T; j++)
-        acc[g][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[g], B[j], acc[g][j], 0, 0, 0);
-#pragma unroll
-    for (int j = 0; j < T; j++) B[j] = Bn[j];
+        acc[g][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[g], (lv && cv[j]) ? B[j] : 0.0,
+                                                         acc[g][j], 0, 0, 0);
+  };
+
+  // Chunks are processed in pairs with ping-pong operand buffers, so each
+  // load has a full chunk of work to hide behind; an odd final chunk is
+  // paired with an all-zero one.
+  double B0[T], B1[T];
+  loadB(l0, B0);
+  for (int l = l0; l <= lmax; l += 8) {
+    loadB(l + 4, B1);
+    chunk(l, B0);
+    loadB(l + 8, B0);
+    chunk(l + 4, B1);
   }
 
   // C layout of the 16x16x4 f64 MFMA: lane holds rows lane/16 + 4r, column lane%16.
@@ -155,109 +173,161 @@ __global__ __launch_bounds__(256) void leg_synth_f64(LegParams p) {
 
 // ---------------------------------------------------------------------------
 // Analysis.  Workgroup = 4 waves for one (order, 16-column group), covering
-// every colatitude: wave w owns theta blocks w, w+4, ... of 64 lanes.  Per
-// chunk of 64 degrees, each wave walks its blocks: loads the block's B
-// operands (X'[theta][col], 16 theta-quads), advances each block's
-// recurrence 16 steps per degree tile, transposes through LDS into 16x4 A
-// operands (16 degrees x 4 colatitudes) and accumulates C[l][col].  The four
-// waves' partial sums are then added in LDS in a fixed order.
+// every colatitude.  The colatitudes are cut into quads (the K=4 of one
+// MFMA); each wave owns MB full blocks of 16 quads (64 lanes), wave w taking
+// blocks w, w+4, ...; the leftover quads (fewer than 64) are dealt round-robin
+// to the waves as a "tail" block.  Per chunk of 64 degrees each wave walks
+// its blocks: the block's B operands (X'[theta][col], one per quad) sit in
+// registers (the next block's are loaded while the current one runs), the
+// block's recurrence advances 16 steps per 16-degree tile, an LDS transpose
+// forms 16x4 A operands (16 degrees x 4 colatitudes), and MFMAs accumulate
+// C[l][col].  The tail block accumulates into its own registers so the main
+// accumulators never cross a branch.  The four waves' partial sums are then
+// added in LDS in a fixed order.  Degrees past lmax are computed but never
+// written; colatitudes past the grid have zero seeds, so their D vanish.
 // ---------------------------------------------------------------------------
-constexpr int kAnalMaxBlocks = 9;   // theta blocks per wave: L <= 2303
-constexpr int kAnalLT = 4;          // 16-degree tiles per chunk
+constexpr int kAnalLT = 4;  // 16-degree tiles per chunk
 
+template <int MB>
 __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
-  __shared__ double tr[4][16][66];
-  __shared__ double red[4][kAnalLT][4][64];
+  __shared__ double tr[4][16][66];  // also holds the cross-wave reduction
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int cg = blockIdx.x % p.n_cg;
   const int mw = blockIdx.x / p.n_cg;
   const int m = order_of(mw, p.real);
   const int l0 = max(abs(m), p.nabs);
-  const long rbase = p.roff[m + p.L] - l0;
+  const int lmax = p.lmax;
+  rec_t* rc = rec_ptr(p.rec, p.roff[m + p.L] - l0);
   const int slot = m - p.m_first;
   const int krow = lane >> 4;
   const int col = cg * 16 + (lane & 15);
   const bool cv = col < p.ncol;
+  const int nt = p.ntheta;
+  // B operands of this (order, column group): quad-major, 64 lanes per quad,
+  // read with raw buffer loads (one lane offset register, scalar quad
+  // offsets; quads past the end read as zero).
+  const __amdgpu_buffer_rsrc_t fa = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(p.F + ((long)slot * p.n_cg + cg) * p.nquad * 64), (short)0, p.nquad * 512,
+      0x00020000);
+  const int voff = lane * 8;
+  auto ldB = [&](int quad) -> double {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(fa, voff, quad * 512, 0));
+  };
 
-  // Per-block recurrence state.
-  double cs[kAnalMaxBlocks], e0[kAnalMaxBlocks], e1[kAnalMaxBlocks];
-  int nb = 0;
+  // Full blocks: block bi of this wave starts at colatitude 64*(wave + 4*bi).
+  double cs[MB > 0 ? MB : 1], e0[MB > 0 ? MB : 1], e1[MB > 0 ? MB : 1];
 #pragma unroll
-  for (int bi = 0; bi < kAnalMaxBlocks; bi++) {
-    const int b = wave + 4 * bi;
-    const int t = b * 64 + lane;
-    const bool tv = (b < p.nblk) && (t < p.ntheta);
-    cs[bi] = tv ? p.cosv[t] : 0.0;
-    e0[bi] = tv ? p.seeds[(long)(m + p.L) * p.ntheta + t] : 0.0;
+  for (int bi = 0; bi < MB; bi++) {
+    const int t = 64 * (wave + 4 * bi) + lane;
+    cs[bi] = p.cosv[t];
+    e0[bi] = p.seeds[(long)(m + p.L) * nt + t];
     e1[bi] = 0.0;
-    if (b < p.nblk) nb = bi + 1;
   }
+  // Tail: quads 4*w + 16*i + {0..3} after the full blocks; lane j covers
+  // quad i = j/4, colatitude tail0 + 16*(j/4) + 4*wave + j%4.
+  const int tail0 = 256 * MB;
+  const int ntailq = (nt - tail0 + 3) >> 2;
+  const int tq = (ntailq - wave + 3) >> 2;  // tail quads of this wave
+  const int tt = tail0 + 16 * (lane >> 2) + 4 * wave + (lane & 3);
+  const bool ttv = tt < nt;
+  double cst = ttv ? p.cosv[tt] : 0.0;
+  double e0t = ttv ? p.seeds[(long)(m + p.L) * nt + tt] : 0.0;
+  double e1t = 0.0;
 
-  for (int lc = l0; lc <= p.lmax; lc += 16 * kAnalLT) {
+  auto loadBlk = [&](int bi, double* Bq) {
+#pragma unroll
+    for (int q = 0; q < 16; q++) Bq[q] = ldB(16 * (wave + 4 * bi) + q);
+  };
+  // 16 recurrence steps of one block into the transpose buffer.
+  auto steps16 = [&](int lb, double cs_, double& e0_, double& e1_) {
+    rec_t* r = rc + (lb + 1);
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      tr[wave][k][lane] = e0_;
+      rec_step(r[k], cs_, e0_, e1_);
+    }
+  };
+
+  for (int lc = l0; lc <= lmax; lc += 16 * kAnalLT) {
     d4 acc[kAnalLT];
 #pragma unroll
     for (int lt = 0; lt < kAnalLT; lt++) acc[lt] = d4{0.0, 0.0, 0.0, 0.0};
 
+    double Ba[16], Bb[16];
+    if (MB > 0) loadBlk(0, Ba);
 #pragma unroll
-    for (int bi = 0; bi < kAnalMaxBlocks; bi++) {
-      if (bi >= nb) continue;
-      const int b = wave + 4 * bi;
-      // B operands: theta = b*64 + 4q + krow, column col.
-      double Bq[16];
-      const int nq = min(16, (p.ntheta - b * 64 + 3) >> 2);
-#pragma unroll
-      for (int q = 0; q < 16; q++) {
-        const int t = b * 64 + 4 * q + krow;
-        Bq[q] = (cv && t < p.ntheta) ? p.F[((long)t * p.nslot + slot) * p.ncol + col] : 0.0;
-      }
+    for (int bi = 0; bi < MB; bi++) {
+      double* Bcur = (bi & 1) ? Bb : Ba;
+      double* Bnext = (bi & 1) ? Ba : Bb;
+      if (bi + 1 < MB) loadBlk(bi + 1, Bnext);
 #pragma unroll
       for (int lt = 0; lt < kAnalLT; lt++) {
-        const int lb = lc + 16 * lt;
-        if (lb > p.lmax) continue;
-#pragma unroll
-        for (int k = 0; k < 16; k++) {
-          const int lk = lb + k;
-          tr[wave][k][lane] = (lk <= p.lmax) ? e0[bi] : 0.0;
-          if (lk + 1 <= p.lmax) rec_step(p, rbase + lk + 1, cs[bi], e0[bi], e1[bi]);
-        }
+        steps16(lc + 16 * lt, cs[bi], e0[bi], e1[bi]);
         wave_sync();
 #pragma unroll
         for (int q = 0; q < 16; q++) {
-          if (q >= nq) continue;
           const double A = tr[wave][lane & 15][4 * q + krow];
-          acc[lt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A, Bq[q], acc[lt], 0, 0, 0);
+          acc[lt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A, Bcur[q], acc[lt], 0, 0, 0);
         }
         wave_sync();
       }
     }
 
-    // Cross-wave reduction in a fixed order, then write the coefficients.
+    if (tq > 0) {
+      d4 acct[kAnalLT];
+#pragma unroll
+      for (int lt = 0; lt < kAnalLT; lt++) acct[lt] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int lt = 0; lt < kAnalLT; lt++) {
+        steps16(lc + 16 * lt, cst, e0t, e1t);
+        wave_sync();
+        for (int i = 0; i < tq; i++) {
+          const int q = (tail0 >> 2) + 4 * i + wave;  // global quad index
+          const double Bv = ldB(q);
+          const double A = tr[wave][lane & 15][4 * i + krow];
+          acct[lt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A, Bv, acct[lt], 0, 0, 0);
+        }
+        wave_sync();
+      }
+#pragma unroll
+      for (int lt = 0; lt < kAnalLT; lt++) acc[lt] += acct[lt];
+    }
+
+    // Cross-wave reduction in a fixed order; wave w sums degree tile w.
+    double* red = &tr[0][0][0];
+    __syncthreads();
 #pragma unroll
     for (int lt = 0; lt < kAnalLT; lt++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) red[wave][lt][r][lane] = acc[lt][r];
+      for (int r = 0; r < 4; r++) red[((wave * kAnalLT + lt) * 4 + r) * 64 + lane] = acc[lt][r];
     __syncthreads();
-    {
-      // Thread -> (lt, r) pair with lt = wave.
-      const int lt = wave;
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const double s = ((red[0][lt][r][lane] + red[1][lt][r][lane]) + red[2][lt][r][lane]) +
-                         red[3][lt][r][lane];
-        const int l = lc + 16 * lt + krow + 4 * r;
-        if (cv && l <= p.lmax) {
-          const int f = col >> 1, pt = col & 1;
-          const long ci = ((long)f * p.cstride + coef_index(l, m, p.nabs, p.real)) * 2 + pt;
-          double v = s;
-          if (p.accumulate) v += p.cout[ci];
-          if (p.zero_last && l == p.lmax && m == p.lmax) v = 0.0;
-          p.cout[ci] = v;
-        }
+    for (int r = 0; r < 4; r++) {
+      const int lt = wave;
+      double s = red[((0 * kAnalLT + lt) * 4 + r) * 64 + lane];
+      s += red[((1 * kAnalLT + lt) * 4 + r) * 64 + lane];
+      s += red[((2 * kAnalLT + lt) * 4 + r) * 64 + lane];
+      s += red[((3 * kAnalLT + lt) * 4 + r) * 64 + lane];
+      const int l = lc + 16 * lt + krow + 4 * r;
+      if (cv && l <= lmax) {
+        const long ci = ((long)(col >> 1) * p.cstride + coef_index(l, m, p.nabs, p.real)) * 2 +
+                        (col & 1);
+        double v = s;
+        if (p.accumulate) v += p.cout[ci];
+        if (p.zero_last && l == lmax && m == lmax) v = 0.0;
+        p.cout[ci] = v;
       }
     }
     __syncthreads();
   }
+}
+
+template <int MB>
+hipError_t launch_anal(const LegParams& p, hipStream_t s) {
+  const long nblocks = (long)p.nslot * p.n_cg;
+  hipLaunchKernelGGL(leg_anal_f64<MB>, dim3(nblocks), dim3(256), 0, s, p);
+  return hipGetLastError();
 }
 
 }  // namespace
@@ -268,21 +338,35 @@ hipError_t launch_leg_synth_f64(const LegParams& p0, hipStream_t s) {
   const int T = p.ncol > 16 ? 2 : 1;
   p.n_cg = (p.ncol + 16 * T - 1) / (16 * T);
   const long nblocks = (long)p.nslot * p.n_cg * p.n_tsb;
-  if (T == 2)
-    hipLaunchKernelGGL(leg_synth_f64<2>, dim3(nblocks), dim3(256), 0, s, p);
+  if (T == 2 && p.real)
+    hipLaunchKernelGGL((leg_synth_f64<2, 1>), dim3(nblocks), dim3(256), 0, s, p);
+  else if (T == 2)
+    hipLaunchKernelGGL((leg_synth_f64<2, 0>), dim3(nblocks), dim3(256), 0, s, p);
+  else if (p.real)
+    hipLaunchKernelGGL((leg_synth_f64<1, 1>), dim3(nblocks), dim3(256), 0, s, p);
   else
-    hipLaunchKernelGGL(leg_synth_f64<1>, dim3(nblocks), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((leg_synth_f64<1, 0>), dim3(nblocks), dim3(256), 0, s, p);
   return hipGetLastError();
 }
 
 hipError_t launch_leg_anal_f64(const LegParams& p0, hipStream_t s) {
   LegParams p = p0;
   p.n_cg = (p.ncol + 15) / 16;
-  p.nblk = (p.ntheta + 63) / 64;
-  if ((p.nblk + 3) / 4 > kAnalMaxBlocks) return hipErrorInvalidValue;
-  const long nblocks = (long)p.nslot * p.n_cg;
-  hipLaunchKernelGGL(leg_anal_f64, dim3(nblocks), dim3(256), 0, s, p);
-  return hipGetLastError();
+  p.nquad = (p.ntheta + 3) / 4;
+  const int mb = p.nquad / 64;  // full 16-quad blocks per wave
+  switch (mb) {
+    case 0: return launch_anal<0>(p, s);
+    case 1: return launch_anal<1>(p, s);
+    case 2: return launch_anal<2>(p, s);
+    case 3: return launch_anal<3>(p, s);
+    case 4: return launch_anal<4>(p, s);
+    case 5: return launch_anal<5>(p, s);
+    case 6: return launch_anal<6>(p, s);
+    case 7: return launch_anal<7>(p, s);
+    case 8: return launch_anal<8>(p, s);
+    case 9: return launch_anal<9>(p, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace gsh

@@ -109,6 +109,15 @@ def test_truncation_and_small_grids(torch, L, lmax, n):
     check_pair(torch, L, nmax, lmax, n, False, 1.0, SEED0 + 11, grid=_grid(L, nmax))
 
 
+@pytest.mark.parametrize("L,lmax,n,real", [(24, 24, 0, False), (15, 15, 1, True),
+                                           (100, 80, -1, False), (33, 33, 2, False),
+                                           (48, 48, 0, True), (300, 300, 2, False)])
+def test_bluestein_ring_lengths(torch, L, lmax, n, real):
+    """Ring lengths 2L that are not powers of two (Bluestein path)."""
+    nmax = min(2, L)
+    check_pair(torch, L, nmax, lmax, n, real, 1.0, SEED0 + 13, grid=_grid(L, nmax))
+
+
 def test_nonnegative_grid_real(torch):
     g = G.GaussLegendreGrid(32, 2, mrange="NonNegative")
     og = oracle.OracleGrid(32, 2, mrange="NonNegative", table=False)
@@ -160,7 +169,7 @@ def test_strided_batch_via_capi(torch):
 def test_forward_accumulates_like_reference(torch):
     """ForwardTransformation adds into `out` (GaussLegendreGrid.h:192), and
     zeroes (L, L) for complex fields at lMax == L (:209-215)."""
-    L, n = 24, 0
+    L, n = 32, 0
     g = _grid(L, 2)
     og = oracle_grid(L, 2)
     f = og.inverse(L, n, G.random_coefficients(L, n, seed=8))
