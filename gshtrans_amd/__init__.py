@@ -38,7 +38,8 @@ EXPORTS = [
     "gsh_adjoint_inverse", "gsh_inverse_host", "gsh_forward_host",
     "gsh_timing_enable", "gsh_timing_read",
 ]
-KERNELS = ["legendre_synthesis", "fft_synthesis", "fft_analysis", "legendre_analysis"]
+KERNELS = ["legendre_synthesis", "fft_synthesis", "fft_analysis", "legendre_analysis",
+           "coef_split"]
 
 
 class GSHError(RuntimeError):

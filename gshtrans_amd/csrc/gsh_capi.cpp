@@ -167,6 +167,11 @@ struct gsh_grid_s {
   std::map<int, NData> ndata;
   double* work = nullptr;
   size_t work_bytes = 0;
+  void* work2 = nullptr;  // split-BF16 operands of the mixed-precision path
+  size_t work2_bytes = 0;
+  // Per (lmax, n, value kind): first 32-degree chunk of each order in the
+  // split coefficient buffer (device), and the total chunk count.
+  std::map<long, std::pair<long*, long>> cso;
   double* stage = nullptr;  // host-call staging
   size_t stage_bytes = 0;
   std::mutex mu;
@@ -174,8 +179,8 @@ struct gsh_grid_s {
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[GSH_KERNEL_COUNT];
   std::vector<hipEvent_t> event_pool;
-  double total_ms[GSH_KERNEL_COUNT] = {0, 0, 0, 0};
-  long launches[GSH_KERNEL_COUNT] = {0, 0, 0, 0};
+  double total_ms[GSH_KERNEL_COUNT] = {};
+  long launches[GSH_KERNEL_COUNT] = {};
 
   int min_upper() const { return nrange == GSH_ALL ? -nmax : (nrange == GSH_NONNEGATIVE ? 0 : nmax); }
   int ntheta() const { return L + 1; }
@@ -284,6 +289,52 @@ int ensure_work(gsh_grid_s* g, size_t bytes) {
     return GSH_ERR_OUT_OF_MEMORY;
   }
   g->work_bytes = bytes;
+  return GSH_OK;
+}
+
+int ensure_work2(gsh_grid_s* g, size_t bytes) {
+  if (bytes <= g->work2_bytes) return GSH_OK;
+  if (g->work2) {
+    GSH_HIP(hipDeviceSynchronize());
+    GSH_HIP(hipFree(g->work2));
+    g->work2 = nullptr;
+    g->work2_bytes = 0;
+  }
+  if (hipMalloc(&g->work2, bytes) != hipSuccess) {
+    g->work2 = nullptr;
+    return GSH_ERR_OUT_OF_MEMORY;
+  }
+  g->work2_bytes = bytes;
+  return GSH_OK;
+}
+
+// Chunk offsets of the split coefficient layout: order m (slot m - m_first)
+// owns ceil((lmax - l0 + 1) / 32) chunks of 32 degrees, l0 = max(|m|, |n|).
+int ensure_cso(gsh_grid_s* g, int lmax, int n, int real, long** dev, long* total) {
+  const long key = ((long)lmax * 4096 + (n + 2048)) * 2 + real;
+  auto it = g->cso.find(key);
+  if (it != g->cso.end()) {
+    *dev = it->second.first;
+    *total = it->second.second;
+    return GSH_OK;
+  }
+  const int nslot = real ? lmax + 1 : 2 * lmax + 1;
+  const int m_first = real ? 0 : -lmax;
+  std::vector<long> h(nslot + 1);
+  long acc = 0;
+  for (int sl = 0; sl < nslot; sl++) {
+    h[sl] = acc;
+    const int m = sl + m_first;
+    const int l0 = std::max(std::abs(m), std::abs(n));
+    acc += (lmax - l0 + 32) / 32;
+  }
+  h[nslot] = acc;
+  long* d = nullptr;
+  GSH_HIP(hipMalloc(&d, h.size() * sizeof(long)));
+  GSH_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(long), hipMemcpyHostToDevice));
+  g->cso.emplace(key, std::make_pair(d, acc));
+  *dev = d;
+  *total = acc;
   return GSH_OK;
 }
 
@@ -531,6 +582,8 @@ int gsh_grid_destroy(gsh_grid_t g) {
     }
   for (auto e : g->event_pool) (void)hipEventDestroy(e);
   if (g->work) (void)hipFree(g->work);
+  if (g->work2) (void)hipFree(g->work2);
+  for (auto& kv : g->cso) (void)hipFree(kv.second.first);
   if (g->stage) (void)hipFree(g->stage);
   delete g;
   return GSH_OK;
@@ -568,7 +621,7 @@ int gsh_inverse(gsh_grid_t g, int lmax, int n, int value_kind, int precision, in
   if (st) return st;
   if (batch == 0) return GSH_OK;
   if (!coeffs || !field) return GSH_ERR_INVALID_ARGUMENT;
-  if (precision != GSH_FP64 || storage != GSH_STORE_FP64) return GSH_ERR_UNSUPPORTED;
+  if (precision == GSH_FP32 || storage != GSH_STORE_FP64) return GSH_ERR_UNSUPPORTED;
   const int real = value_kind == GSH_REAL;
   const long csize = gsh_coefficient_size(lmax, n, value_kind);
   const long fsize = static_cast<long>(g->ntheta()) * g->nphi();
@@ -588,8 +641,25 @@ int gsh_inverse(gsh_grid_t g, int lmax, int n, int value_kind, int precision, in
   LegParams p = base_params(g, d, lmax, n, real, batch);
   p.cin = static_cast<const double*>(coeffs);
   p.cstride = coeff_stride;
-  GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_SYNTHESIS, s,
-                [&] { return gsh::launch_leg_synth_f64(p, s); }));
+  if (precision == GSH_FP64) {
+    GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_SYNTHESIS, s,
+                  [&] { return gsh::launch_leg_synth_f64(p, s); }));
+  } else {
+    const int ns = precision == GSH_BF16X6 ? 3 : 2;
+    long* cso = nullptr;
+    long nchunks = 0;
+    if ((st = ensure_cso(g, lmax, n, real, &cso, &nchunks))) return st;
+    p.ntile = (p.ncol + 15) / 16;
+    p.cso = cso;
+    const long nblocks = nchunks * p.ntile;
+    if ((st = ensure_work2(g, (size_t)nblocks * ns * 64 * 16))) return st;
+    p.bs = static_cast<const uint4*>(g->work2);
+    p.bs_out = static_cast<uint4*>(g->work2);
+    GSH_HIP(timed(g, GSH_KERNEL_COEF_SPLIT, s,
+                  [&] { return gsh::launch_coef_split(p, ns, nblocks, s); }));
+    GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_SYNTHESIS, s,
+                  [&] { return gsh::launch_leg_synth_bf16(p, ns, s); }));
+  }
   FFTParams q = fft_params(g, lmax, real, batch);
   q.fout = field;
   q.fstride = field_stride;
@@ -604,7 +674,7 @@ int gsh_forward(gsh_grid_t g, int lmax, int n, int value_kind, int precision, in
   if (st) return st;
   if (batch == 0) return GSH_OK;
   if (!coeffs || !field) return GSH_ERR_INVALID_ARGUMENT;
-  if (precision != GSH_FP64 || storage != GSH_STORE_FP64) return GSH_ERR_UNSUPPORTED;
+  if (precision == GSH_FP32 || storage != GSH_STORE_FP64) return GSH_ERR_UNSUPPORTED;
   const int real = value_kind == GSH_REAL;
   const long csize = gsh_coefficient_size(lmax, n, value_kind);
   const long fsize = static_cast<long>(g->ntheta()) * g->nphi();
@@ -620,19 +690,38 @@ int gsh_forward(gsh_grid_t g, int lmax, int n, int value_kind, int precision, in
   NData* d;
   if ((st = ensure_ndata(g, n, &d))) return st;
   const int nslot = real ? lmax + 1 : 2 * lmax + 1;
-  const size_t fa = (size_t)nslot * ((2 * (size_t)batch + 15) / 16) * ((g->ntheta() + 3) / 4) * 64;
-  if ((st = ensure_work(g, sizeof(double) * fa))) return st;
+  const int ncol = 2 * batch;
   FFTParams q = fft_params(g, lmax, real, batch);
   q.fin = field;
   q.fstride = field_stride;
-  GSH_HIP(timed(g, GSH_KERNEL_FFT_ANALYSIS, s, [&] { return gsh::launch_fft_anal(q, s); }));
   LegParams p = base_params(g, d, lmax, n, real, batch);
   p.cout = static_cast<double*>(coeffs);
   p.cstride = coeff_stride;
   p.accumulate = accumulate ? 1 : 0;
   p.zero_last = (!real && lmax == g->L) ? 1 : 0;
+  if (precision == GSH_FP64) {
+    const size_t fa = (size_t)nslot * ((ncol + 15) / 16) * ((g->ntheta() + 3) / 4) * 64;
+    if ((st = ensure_work(g, sizeof(double) * fa))) return st;
+    q.F = p.F = g->work;
+    GSH_HIP(timed(g, GSH_KERNEL_FFT_ANALYSIS, s, [&] { return gsh::launch_fft_anal(q, s); }));
+    GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_ANALYSIS, s,
+                  [&] { return gsh::launch_leg_anal_f64(p, s); }));
+    return GSH_OK;
+  }
+  const int ns = precision == GSH_BF16X6 ? 3 : 2;
+  const int nkc = (g->ntheta() + 31) / 32;
+  const int ntile = (ncol + 15) / 16;
+  if ((st = ensure_work2(g, (size_t)nslot * nkc * ntile * ns * 64 * 16))) return st;
+  q.ns = ns;
+  q.nkc = nkc;
+  q.ntile = ntile;
+  q.xs = static_cast<__bf16*>(g->work2);
+  p.bs = static_cast<const uint4*>(g->work2);
+  p.nkc = nkc;
+  p.ntile = ntile;
+  GSH_HIP(timed(g, GSH_KERNEL_FFT_ANALYSIS, s, [&] { return gsh::launch_fft_anal(q, s); }));
   GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_ANALYSIS, s,
-                [&] { return gsh::launch_leg_anal_f64(p, s); }));
+                [&] { return gsh::launch_leg_anal_bf16(p, ns, s); }));
   return GSH_OK;
 }
 

@@ -241,6 +241,24 @@ __global__ __launch_bounds__(kFFTThreads) void fft_anal(FFTParams p) {
     const int m = s + p.m_first;
     const int k = m >= 0 ? m : N + m;
     const double2 X = buf[q * M + k];
+    if (p.ns > 0) {
+      // Split X' into ns bf16 planes (f64 -> f32 -> bf16 remainders).
+      float v[2] = {(float)(X.x * w), (float)(X.y * w)};
+      asm volatile("" : "+v"(v[0]), "+v"(v[1]));
+      for (int pl = 0; pl < p.ns; pl++) {
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const __bf16 b = (__bf16)v[u];
+          p.xs[xs_index(s, 2 * f + u, theta, pl, p.ns, p.nkc, p.ntile)] = b;
+          v[u] -= (float)b;
+        }
+      }
+      for (int tp = theta + 1; theta == p.ntheta - 1 && tp < 32 * p.nkc; tp++)
+        for (int pl = 0; pl < p.ns; pl++)
+          for (int u = 0; u < 2; u++)
+            p.xs[xs_index(s, 2 * f + u, tp, pl, p.ns, p.nkc, p.ntile)] = (__bf16)0.0f;
+      continue;
+    }
     double* Fp = p.F + fa_index(s, 2 * f, theta, p.n_cg, p.nquad);
     Fp[0] = X.x * w;
     Fp[1] = X.y * w;

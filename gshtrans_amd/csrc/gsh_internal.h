@@ -54,6 +54,16 @@ struct LegParams {
   int n_tsb;            // synthesis: theta super-blocks of 256
   int n_cg;             // column groups
   int nquad;            // analysis: colatitude quads, ceil(ntheta / 4)
+  // Mixed precision (split-BF16 MFMA):
+  //   synthesis: bs holds the split coefficients, block (cso[slot] + chunk,
+  //   tile, split) = 64 lanes x 8 bf16, see coef_split in gsh_mixed.hip;
+  //   analysis: bs holds the split X' in the same lane order with 32-colatitude
+  //   chunks, block (slot, kchunk, tile, split).
+  const uint4* bs;
+  uint4* bs_out;
+  const long* cso;      // per slot: first chunk of that order in bs
+  int ntile;            // 16-column tiles, ceil(ncol / 16)
+  int nkc;              // analysis: 32-colatitude chunks, ceil(ntheta / 32)
 };
 
 // Index of (slot, col, theta) in the analysis work layout FA.
@@ -81,12 +91,32 @@ struct FFTParams {
   const void* fin;    // analysis input field
   void* fout;         // synthesis output field
   long fstride;       // elements between fields
+  // Mixed-precision analysis: ns > 0 writes the weighted bins as ns bf16
+  // planes in the split layout of leg_anal_bf16 (xs) instead of FP64 FA.
+  int ns;
+  int nkc;            // 32-colatitude chunks
+  int ntile;          // 16-column tiles
+  __bf16* xs;
 };
+
+// bf16 index of (slot, col, theta, plane s) in the split analysis layout:
+// block ((slot * nkc + theta/32) * ntile + col/16) * ns + s of 64 lanes x 8,
+// lane = 16 * ((theta % 32) / 8) + col % 16, element theta % 8.
+__host__ __device__ inline long xs_index(int slot, int col, int theta, int s, int ns, int nkc,
+                                         int ntile) {
+  const long blk = (((long)slot * nkc + (theta >> 5)) * ntile + (col >> 4)) * ns + s;
+  return (blk * 64 + 16 * ((theta & 31) >> 3) + (col & 15)) * 8 + (theta & 7);
+}
 
 hipError_t launch_leg_synth_f64(const LegParams& p, hipStream_t s);
 hipError_t launch_leg_anal_f64(const LegParams& p, hipStream_t s);
 hipError_t launch_fft_synth(const FFTParams& p, hipStream_t s);
 hipError_t launch_fft_anal(const FFTParams& p, hipStream_t s);
+// Mixed precision kernels (gsh_mixed.hip); ns = 2 (bf16x3) or 3 (bf16x6).
+hipError_t launch_coef_split(const LegParams& p, int ns, long nblocks, hipStream_t s);
+hipError_t launch_leg_synth_bf16(const LegParams& p, int ns, hipStream_t s);
+hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s);
+
 hipError_t launch_lmax0(int inverse, int real, int nfield, const double* in, long istride,
                         double* out, long ostride, int accumulate, hipStream_t s);
 

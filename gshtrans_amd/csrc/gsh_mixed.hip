@@ -1,0 +1,464 @@
+// gsh_mixed.hip -- mixed-precision Legendre contraction: split-BF16 MFMA
+// products (v_mfma_f32_16x16x32_bf16) with the Wigner recurrence, its seeds
+// and the quadrature weights kept in FP64.
+//
+// Splitting.  A double x is rounded to float f, then f = h + m + l with h, m,
+// l bf16 (each the round-to-nearest bf16 of the remainder).  bf16x6 keeps
+// the products h*h, h*m, m*h, h*l, m*m, l*h of the two operands (every term
+// down to 2^-16 of the leading one); bf16x3 splits in two and keeps h*h,
+// h*m, m*h.
+//
+// Accumulation.  Each MFMA rounds its 32-product sum into an FP32
+// accumulator.  Per 32-term chunk the small cross products go in first and
+// the leading h*h product last, so the only rounding at the size of the chunk
+// sum is the last one; the chunk result is then added into an FP64
+// accumulator.  The error therefore stays near one FP32 rounding of each
+// chunk sum instead of growing with the number of chunks.
+//
+// Sharing.  Generating and splitting D costs ~10 VALU instructions per value,
+// more than the MFMAs need when a value feeds few columns.  A workgroup of 8
+// waves therefore covers 256 colatitudes x 128 columns of one order: waves
+// 0-3 run the recurrence for 64 colatitudes each and write the bf16 splits
+// to LDS (double-buffered, one barrier per 32-degree chunk); every wave
+// multiplies them into its own 128 colatitudes x 32 columns.
+#include "gsh_internal.h"
+
+namespace gsh {
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(4))) d4 rec_t;
+
+__device__ __forceinline__ rec_t* rec_ptr(const double4* rec, long off) {
+  return (rec_t*)(rec) + off;
+}
+
+__device__ __forceinline__ void rec_step(const d4 q, double cs, double& e0, double& e1) {
+  const double t = fma(q.x, cs, -q.y);
+  const double en = fma(t, e0, -(q.z * e1));
+  e1 = e0;
+  e0 = en;
+}
+
+__device__ __forceinline__ int order_of(int mw, int real) {
+  return real ? mw : ((mw & 1) ? -((mw + 1) >> 1) : (mw >> 1));
+}
+
+__device__ __forceinline__ long coef_index(int l, int m, int nabs, int real) {
+  return real ? (long)l * (l + 1) / 2 - (long)nabs * (nabs + 1) / 2 + m
+              : (long)l * l - (long)nabs * nabs + m + l;
+}
+
+__device__ __forceinline__ unsigned pack2(__bf16 a, __bf16 b) {
+  bf16x2 v = {a, b};
+  return __builtin_bit_cast(unsigned, v);
+}
+
+// Split two doubles (consecutive K entries) into NS bf16 planes; plane s of
+// the pair is returned packed in one dword (element 0 in the low half).
+template <int NS>
+__device__ __forceinline__ void split_pair(double x0, double x1, unsigned* out) {
+  float f0 = (float)x0, f1 = (float)x1;
+  // Keep the f64->f32 rounding separate from the bf16 rounding below.
+  asm volatile("" : "+v"(f0), "+v"(f1));
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const __bf16 b0 = (__bf16)f0, b1 = (__bf16)f1;
+    out[s] = pack2(b0, b1);
+    if (s + 1 < NS) {
+      f0 -= (float)b0;
+      f1 -= (float)b1;
+    }
+  }
+}
+
+__device__ __forceinline__ bf16x8 as_bf8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+__device__ __forceinline__ f4 mfma(uint4 a, uint4 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(a), as_bf8(b), c, 0, 0, 0);
+}
+
+// Split product of one A and one B operand (NS planes each) into the FP64
+// accumulator; cross products first, leading product last.
+template <int NS>
+__device__ __forceinline__ void split_mma(const uint4* A, const uint4* B, d4& acc) {
+  f4 c = {0.f, 0.f, 0.f, 0.f};
+  if (NS == 3) {
+    c = mfma(A[1], B[1], c);
+    c = mfma(A[0], B[2], c);
+    c = mfma(A[2], B[0], c);
+  }
+  c = mfma(A[0], B[1], c);
+  c = mfma(A[1], B[0], c);
+  c = mfma(A[0], B[0], c);
+  acc[0] += (double)c[0];
+  acc[1] += (double)c[1];
+  acc[2] += (double)c[2];
+  acc[3] += (double)c[3];
+}
+
+// ---------------------------------------------------------------------------
+// Coefficient split: synthesis B operands.  Block (cso[slot] + chunk, tile,
+// s) holds, for lane = 16*kr + c, the bf16 plane s of the coefficients of
+// degrees l0 + 32*chunk + 8*kr + e (e = 0..7) and column 16*tile + c; degrees
+// past lmax and columns past ncol are zero.
+// ---------------------------------------------------------------------------
+template <int NS, int REAL>
+__global__ __launch_bounds__(256) void coef_split(LegParams p, long nblocks) {
+  const long b = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (b >= nblocks) return;
+  const int lane = threadIdx.x & 63;
+  const long cb = b / p.ntile;
+  const int tile = (int)(b - cb * p.ntile);
+  // slot: largest s with cso[s] <= cb
+  int lo = 0, hi = p.nslot - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (p.cso[mid] <= cb) lo = mid; else hi = mid - 1;
+  }
+  const int slot = lo;
+  const int m = slot + p.m_first;
+  const int l0 = max(abs(m), p.nabs);
+  const int chunk = (int)(cb - p.cso[slot]);
+  const int kr = lane >> 4;
+  const int col = tile * 16 + (lane & 15);
+  const bool cv = col < p.ncol;
+  const double* cp = p.cin + (long)(cv ? col >> 1 : 0) * p.cstride * 2 + (col & 1);
+  unsigned planes[NS][4];
+#pragma unroll
+  for (int e2 = 0; e2 < 4; e2++) {
+    double x[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int l = l0 + 32 * chunk + 8 * kr + 2 * e2 + u;
+      const double v = cp[2 * coef_index(min(l, p.lmax), m, p.nabs, REAL)];
+      x[u] = (cv && l <= p.lmax) ? v : 0.0;
+    }
+    unsigned o[NS];
+    split_pair<NS>(x[0], x[1], o);
+#pragma unroll
+    for (int s = 0; s < NS; s++) planes[s][e2] = o[s];
+  }
+#pragma unroll
+  for (int s = 0; s < NS; s++)
+    p.bs_out[(b * NS + s) * 64 + lane] =
+        make_uint4(planes[s][0], planes[s][1], planes[s][2], planes[s][3]);
+}
+
+// ---------------------------------------------------------------------------
+// Synthesis.  Workgroup = 8 waves: (256 colatitudes of super-block tsb) x
+// (128 columns of column super-group csg) for one order m.  Wave w < 4
+// produces the split D of colatitudes 64w..64w+63; wave w consumes
+// colatitude half h = w/4 (128 colatitudes = 8 MFMA row groups) for column
+// group w%4 (2 tiles of 16 columns).
+// ---------------------------------------------------------------------------
+template <int NS, int REAL>
+__global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
+  __shared__ uint4 trb[2][NS][4][256];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  int blk = blockIdx.x;
+  const int tsb = blk % p.n_tsb;
+  blk /= p.n_tsb;
+  const int csg = blk % p.n_cg;
+  const int mw = blk / p.n_cg;
+  const int m = order_of(mw, REAL);
+  const int slot = m - p.m_first;
+  const int l0 = max(abs(m), p.nabs);
+  const int lmax = p.lmax;
+  const int nchunk = (lmax - l0 + 32) >> 5;
+  const bool prod = wave < 4;
+
+  // Producer state: colatitude tp.
+  const int tp = tsb * 256 + (wave & 3) * 64 + lane;
+  const bool tpv = prod && tp < p.ntheta;
+  const double cs = tpv ? p.cosv[tp] : 0.0;
+  double e0 = tpv ? p.seeds[(long)(m + p.L) * p.ntheta + tp] : 0.0;
+  double e1 = 0.0;
+  rec_t* rc = rec_ptr(p.rec, p.roff[m + p.L] - l0);
+
+  auto produce = [&](int chunk, int buf) {
+    const int lb = l0 + 32 * chunk;
+#pragma unroll
+    for (int kr = 0; kr < 4; kr++) {
+      unsigned pl[NS][4];
+#pragma unroll
+      for (int e2 = 0; e2 < 4; e2++) {
+        const int l = lb + 8 * kr + 2 * e2;
+        const double v0 = e0;
+        rec_step(rc[l + 1], cs, e0, e1);
+        const double v1 = e0;
+        rec_step(rc[l + 2], cs, e0, e1);
+        unsigned o[NS];
+        split_pair<NS>(v0, v1, o);
+#pragma unroll
+        for (int s = 0; s < NS; s++) pl[s][e2] = o[s];
+      }
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+        trb[buf][s][kr][(wave & 3) * 64 + lane] = make_uint4(pl[s][0], pl[s][1], pl[s][2], pl[s][3]);
+    }
+  };
+
+  // Consumer role.
+  const int h = wave >> 2;
+  const int tile0 = csg * 8 + (wave & 3) * 2;
+  const bool tv0 = tile0 < p.ntile, tv1 = tile0 + 1 < p.ntile;
+  const uint4* bbase = p.bs + (p.cso[slot] * (long)p.ntile) * NS * 64 + lane;
+  auto loadB = [&](int chunk, uint4 (&B)[2][NS]) {
+    const long c = (long)min(chunk, nchunk - 1) * p.ntile;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const uint4 z = make_uint4(0, 0, 0, 0);
+      const uint4 b0 = bbase[((c + (tv0 ? tile0 : 0)) * NS + s) * 64];
+      const uint4 b1 = bbase[((c + (tv1 ? tile0 + 1 : 0)) * NS + s) * 64];
+      B[0][s] = tv0 ? b0 : z;
+      B[1][s] = tv1 ? b1 : z;
+    }
+  };
+
+  d4 acc[8][2];
+#pragma unroll
+  for (int g = 0; g < 8; g++)
+#pragma unroll
+    for (int t = 0; t < 2; t++) acc[g][t] = d4{0.0, 0.0, 0.0, 0.0};
+
+  if (prod) produce(0, 0);
+  uint4 B[2][NS];
+  loadB(0, B);
+  __syncthreads();
+  for (int k = 0; k < nchunk; k++) {
+    if (prod && k + 1 < nchunk) produce(k + 1, (k + 1) & 1);
+    uint4 Bn[2][NS];
+    loadB(k + 1, Bn);
+    const int buf = k & 1;
+#pragma unroll
+    for (int g = 0; g < 8; g++) {
+      uint4 A[NS];
+#pragma unroll
+      for (int s = 0; s < NS; s++) A[s] = trb[buf][s][lane >> 4][h * 128 + g * 16 + (lane & 15)];
+      split_mma<NS>(A, B[0], acc[g][0]);
+      split_mma<NS>(A, B[1], acc[g][1]);
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      B[0][s] = Bn[0][s];
+      B[1][s] = Bn[1][s];
+    }
+    __syncthreads();
+  }
+
+  // C layout of the 16x16x32 MFMA: lane holds rows 4*(lane/16)+r, column lane%16.
+#pragma unroll
+  for (int g = 0; g < 8; g++)
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const int col = (tile0 + t) * 16 + (lane & 15);
+      if (col >= p.ncol) continue;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = tsb * 256 + h * 128 + g * 16 + 4 * (lane >> 4) + r;
+        if (row < p.ntheta) p.F[((long)row * p.nslot + slot) * p.ncol + col] = acc[g][t][r];
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Analysis.  Workgroup = 8 waves for one order m and 128 columns (column
+// super-group csg), covering every colatitude.  Degrees are processed in
+// passes of 64 (4 MFMA row tiles); within a pass the colatitudes are walked
+// in groups of 128 (two 64-lane blocks = four 32-colatitude K chunks).  The
+// recurrence of block b is advanced by wave b % 8, its state kept in LDS
+// between passes; the produced bf16 planes D[s][l][theta] are
+// double-buffered in LDS.  Wave w multiplies degree tiles 2*(w/4) and
+// 2*(w/4)+1 with column tiles 2*(w%4) and 2*(w%4)+1, so each wave ends a
+// pass with complete sums over all colatitudes and writes them directly.
+// ---------------------------------------------------------------------------
+constexpr int kAnalBf16MaxBlocks = 40;  // colatitude blocks: L <= 2559
+
+template <int NS, int REAL>
+__global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
+  __shared__ __bf16 Dl[2][NS][64][128];
+  __shared__ double2 st[kAnalBf16MaxBlocks][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int csg = blockIdx.x % p.n_cg;
+  const int mw = blockIdx.x / p.n_cg;
+  const int m = order_of(mw, REAL);
+  const int slot = m - p.m_first;
+  const int l0 = max(abs(m), p.nabs);
+  const int lmax = p.lmax;
+  const int nt = p.ntheta;
+  const int nblk = (nt + 63) >> 6;
+  const int ngrp = (nblk + 1) >> 1;
+  rec_t* rc = rec_ptr(p.rec, p.roff[m + p.L] - l0);
+  const double* seeds = p.seeds + (long)(m + p.L) * nt;
+
+  // Initial recurrence state of every block.
+  for (int b = wave; b < nblk; b += 8) {
+    const int t = b * 64 + lane;
+    st[b][lane] = make_double2(t < nt ? seeds[t] : 0.0, 0.0);
+  }
+
+  // Produce the bf16 planes of block b (degrees lp..lp+63) into buffer half bb.
+  auto produce = [&](int b, int lp, int buf, int bb) {
+    const int t = b * 64 + lane;
+    const double cs = t < nt ? p.cosv[t] : 0.0;
+    double2 s2 = st[b][lane];
+    double e0 = s2.x, e1 = s2.y;
+#pragma unroll 4
+    for (int k = 0; k < 64; k += 2) {
+      const double v0 = e0;
+      rec_step(rc[lp + k + 1], cs, e0, e1);
+      const double v1 = e0;
+      rec_step(rc[lp + k + 2], cs, e0, e1);
+      unsigned o[NS];
+      split_pair<NS>(v0, v1, o);
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        const bf16x2 pr = __builtin_bit_cast(bf16x2, o[s]);
+        Dl[buf][s][k][bb * 64 + lane] = pr[0];
+        Dl[buf][s][k + 1][bb * 64 + lane] = pr[1];
+      }
+    }
+    st[b][lane] = make_double2(e0, e1);
+  };
+  auto produce_group = [&](int g, int lp, int buf) {
+#pragma unroll
+    for (int bb = 0; bb < 2; bb++) {
+      const int b = 2 * g + bb;
+      if ((b & 7) == wave) {
+        if (b < nblk) {
+          produce(b, lp, buf, bb);
+        } else {
+#pragma unroll
+          for (int s = 0; s < NS; s++)
+            for (int k = 0; k < 64; k++) Dl[buf][s][k][bb * 64 + lane] = (__bf16)0.0f;
+        }
+      }
+    }
+  };
+
+  // Consumer role.
+  const int lh = wave >> 2;
+  const int tile0 = csg * 8 + (wave & 3) * 2;
+  const bool tv0 = tile0 < p.ntile, tv1 = tile0 + 1 < p.ntile;
+  const uint4* bslot = p.bs + (long)slot * p.nkc * p.ntile * NS * 64 + lane;
+  auto loadB = [&](int kc, uint4 (&B)[2][NS]) {
+    const bool kv = kc < p.nkc;
+    const long c = (long)(kv ? kc : 0) * p.ntile;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const uint4 z = make_uint4(0, 0, 0, 0);
+      const uint4 b0 = bslot[((c + (tv0 ? tile0 : 0)) * NS + s) * 64];
+      const uint4 b1 = bslot[((c + (tv1 ? tile0 + 1 : 0)) * NS + s) * 64];
+      B[0][s] = (kv && tv0) ? b0 : z;
+      B[1][s] = (kv && tv1) ? b1 : z;
+    }
+  };
+
+  __syncthreads();
+  for (int lp = l0; lp <= lmax; lp += 64) {
+    d4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+    produce_group(0, lp, 0);
+    __syncthreads();
+    for (int g = 0; g < ngrp; g++) {
+      if (g + 1 < ngrp) produce_group(g + 1, lp, (g + 1) & 1);
+      const int buf = g & 1;
+#pragma unroll
+      for (int kq = 0; kq < 4; kq++) {
+        uint4 B[2][NS];
+        loadB(4 * g + kq, B);
+#pragma unroll
+        for (int lt = 0; lt < 2; lt++) {
+          uint4 A[NS];
+#pragma unroll
+          for (int s = 0; s < NS; s++)
+            A[s] = *reinterpret_cast<const uint4*>(
+                &Dl[buf][s][(2 * lh + lt) * 16 + (lane & 15)][kq * 32 + 8 * (lane >> 4)]);
+          split_mma<NS>(A, B[0], acc[lt][0]);
+          split_mma<NS>(A, B[1], acc[lt][1]);
+        }
+      }
+      __syncthreads();
+    }
+
+    // Rows l = lp + 16*(2*lh + lt) + 4*(lane/16) + r, column 16*tile + lane%16.
+#pragma unroll
+    for (int lt = 0; lt < 2; lt++)
+#pragma unroll
+      for (int ct = 0; ct < 2; ct++) {
+        const int col = (tile0 + ct) * 16 + (lane & 15);
+        if (col >= p.ncol) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int l = lp + 16 * (2 * lh + lt) + 4 * (lane >> 4) + r;
+          if (l > lmax) continue;
+          const long ci = ((long)(col >> 1) * p.cstride + coef_index(l, m, p.nabs, REAL)) * 2 +
+                          (col & 1);
+          double v = acc[lt][ct][r];
+          if (p.accumulate) v += p.cout[ci];
+          if (p.zero_last && l == lmax && m == lmax) v = 0.0;
+          p.cout[ci] = v;
+        }
+      }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_coef_split(const LegParams& p, int ns, long nblocks, hipStream_t s) {
+  const dim3 grid((nblocks + 3) / 4);
+  if (ns == 3 && p.real)
+    hipLaunchKernelGGL((coef_split<3, 1>), grid, dim3(256), 0, s, p, nblocks);
+  else if (ns == 3)
+    hipLaunchKernelGGL((coef_split<3, 0>), grid, dim3(256), 0, s, p, nblocks);
+  else if (p.real)
+    hipLaunchKernelGGL((coef_split<2, 1>), grid, dim3(256), 0, s, p, nblocks);
+  else
+    hipLaunchKernelGGL((coef_split<2, 0>), grid, dim3(256), 0, s, p, nblocks);
+  return hipGetLastError();
+}
+
+hipError_t launch_leg_synth_bf16(const LegParams& p0, int ns, hipStream_t s) {
+  LegParams p = p0;
+  p.n_tsb = (p.ntheta + 255) / 256;
+  p.n_cg = (p.ntile + 7) / 8;  // column super-groups of 8 tiles
+  const dim3 grid((long)p.nslot * p.n_cg * p.n_tsb);
+  if (ns == 3 && p.real)
+    hipLaunchKernelGGL((leg_synth_bf16<3, 1>), grid, dim3(512), 0, s, p);
+  else if (ns == 3)
+    hipLaunchKernelGGL((leg_synth_bf16<3, 0>), grid, dim3(512), 0, s, p);
+  else if (p.real)
+    hipLaunchKernelGGL((leg_synth_bf16<2, 1>), grid, dim3(512), 0, s, p);
+  else
+    hipLaunchKernelGGL((leg_synth_bf16<2, 0>), grid, dim3(512), 0, s, p);
+  return hipGetLastError();
+}
+
+template <int NS>
+hipError_t launch_anal_bf16_t(const LegParams& p0, hipStream_t s) {
+  LegParams p = p0;
+  p.n_cg = (p.ntile + 7) / 8;
+  const dim3 grid((long)p.nslot * p.n_cg);
+  if (p.real)
+    hipLaunchKernelGGL((leg_anal_bf16<NS, 1>), grid, dim3(512), 0, s, p);
+  else
+    hipLaunchKernelGGL((leg_anal_bf16<NS, 0>), grid, dim3(512), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s) {
+  if ((p.ntheta + 63) / 64 > kAnalBf16MaxBlocks) return hipErrorInvalidValue;
+  return ns == 3 ? launch_anal_bf16_t<3>(p, s) : launch_anal_bf16_t<2>(p, s);
+}
+
+}  // namespace gsh

@@ -124,7 +124,8 @@ enum gsh_kernel {
   GSH_KERNEL_FFT_SYNTHESIS = 1,
   GSH_KERNEL_FFT_ANALYSIS = 2,
   GSH_KERNEL_LEGENDRE_ANALYSIS = 3,
-  GSH_KERNEL_COUNT = 4
+  GSH_KERNEL_COEF_SPLIT = 4,      /* mixed precision: coefficient split   */
+  GSH_KERNEL_COUNT = 5
 };
 /* enable != 0 records a HIP event pair around every kernel launch of this
  * grid (on the launch stream) and clears the accumulated totals. */

@@ -1,0 +1,94 @@
+"""Mixed-precision Legendre contraction (split-BF16 MFMA, FP64 recurrence,
+seeds, weights and FFT): relative L2 error of each field against the FP64
+GPU transform, which is itself pinned to the oracle at <= 1e-12 by
+test_gpu_parity.py.  Bars: bf16x6 <= 1e-7 (the precision target), bf16x3
+<= 1e-4 (two-way split, ~16 significant bits per operand)."""
+import sys
+
+import numpy as np
+import pytest
+
+import gshtrans_amd as G
+
+pytestmark = pytest.mark.gpu
+SEED0 = 20261202
+BARS = {"bf16x6": 1e-7, "bf16x3": 1e-4}
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    assert _t.cuda.is_available()
+    return _t
+
+
+_GRIDS = {}
+
+
+def grid(L):
+    if L not in _GRIDS:
+        _GRIDS[L] = G.GaussLegendreGrid(L, min(2, L))
+    return _GRIDS[L]
+
+
+def per_field_rel(torch, a, b):
+    d = (a - b).reshape(a.shape[0], -1)
+    return (torch.linalg.vector_norm(d, dim=1) /
+            torch.linalg.vector_norm(b.reshape(b.shape[0], -1), dim=1)).cpu().numpy()
+
+
+@pytest.mark.parametrize("prec", ["bf16x6", "bf16x3"])
+@pytest.mark.parametrize("L,n,real,batch", [(64, 0, False, 3), (64, 2, True, 5),
+                                            (512, 2, False, 8), (300, -1, False, 2),
+                                            (1024, 0, True, 16), (1024, 2, False, 16),
+                                            (100, 1, True, 70)])
+def test_mixed_synthesis_error(torch, prec, L, n, real, batch):
+    say(f"[mixed synthesis] {prec} L={L} n={n} real={real} batch={batch}")
+    g = grid(L)
+    c = torch.from_numpy(G.random_coefficients(L, n, real=real, seed=SEED0 + L + n,
+                                               batch=batch)).cuda()
+    f64 = g.inverse(L, n, c, real=real)
+    fmx = g.inverse(L, n, c, real=real, precision=prec)
+    e = per_field_rel(torch, fmx, f64)
+    say(f"    max {e.max():.2e} median {np.median(e):.2e}")
+    assert e.max() <= BARS[prec]
+    assert e.min() > 0 or prec == "fp64"
+
+
+@pytest.mark.parametrize("prec", ["bf16x6", "bf16x3"])
+@pytest.mark.parametrize("L,n,real,batch", [(64, 0, False, 3), (64, 2, True, 5),
+                                            (512, 2, False, 8), (300, -1, False, 2),
+                                            (1024, 0, True, 16), (1024, 2, False, 16),
+                                            (100, 1, True, 70), (24, 1, False, 4)])
+def test_mixed_analysis_error(torch, prec, L, n, real, batch):
+    say(f"[mixed analysis] {prec} L={L} n={n} real={real} batch={batch}")
+    g = grid(L)
+    c = torch.from_numpy(G.random_coefficients(L, n, real=real, seed=SEED0 + 7 * L + n,
+                                               batch=batch)).cuda()
+    f = g.inverse(L, n, c, real=real)
+    a64 = g.forward(L, n, f, real=real)
+    amx = g.forward(L, n, f, real=real, precision=prec)
+    e = per_field_rel(torch, amx, a64)
+    rt = per_field_rel(torch, g.forward(L, n, g.inverse(L, n, c, real=real, precision=prec),
+                                        real=real, precision=prec), c)
+    say(f"    analysis max {e.max():.2e} median {np.median(e):.2e}; round trip max {rt.max():.2e}")
+    assert e.max() <= BARS[prec]
+    assert rt.max() <= 2 * BARS[prec]
+
+
+def test_mixed_accumulate_and_zero_last(torch):
+    L, n = 32, 0
+    g = grid(L)
+    c = torch.from_numpy(G.random_coefficients(L, n, seed=3, batch=2)).cuda()
+    f = g.inverse(L, n, c)
+    base = torch.from_numpy(G.random_coefficients(L, n, seed=4, batch=2, grid_lmax=L + 1)).cuda()
+    out = base.clone()
+    g.forward(L, n, f, coeffs=out, precision="bf16x6", accumulate=True)
+    ref = base + g.forward(L, n, f)
+    ref[:, -1] = 0
+    assert float(per_field_rel(torch, out, ref).max()) < 1e-7
+    assert bool((out[:, -1] == 0).all())
