@@ -39,7 +39,7 @@ EXPORTS = [
     "gsh_timing_enable", "gsh_timing_read",
 ]
 KERNELS = ["legendre_synthesis", "fft_synthesis", "fft_analysis", "legendre_analysis",
-           "coef_split"]
+           "coef_split", "bin_split"]
 
 
 class GSHError(RuntimeError):
@@ -333,6 +333,24 @@ class GaussLegendreGrid:
                                   PRECISION[precision], STORAGE["fp64"], B, _ptr(f),
                                   self.FieldSize(), _ptr(coeffs), C, 1 if accumulate else 0,
                                   _stream(stream)), "gsh_forward")
+        return coeffs if field.dim() == 3 else coeffs[0]
+
+    def adjoint(self, lmax, n, field, coeffs=None, real=False, precision="fp64", stream=None):
+        """Adjoint of the synthesis, coeffs = S_n^H(field), for a batch: the
+        ring FFT and Legendre contraction of the analysis with unit
+        colatitude weights, no 2 pi / N and no (L, L) zeroing; for real
+        fields the adjoint of the c2r synthesis (weight 2 on orders
+        0 < m < N/2, real part only at m = 0 and N/2).  Overwrites coeffs."""
+        import torch
+        f = field if field.dim() == 3 else field.unsqueeze(0)
+        B = f.shape[0]
+        C = max(int(_lib().gsh_coefficient_size(lmax, n, REAL if real else COMPLEX)), 1)
+        if coeffs is None:
+            coeffs = torch.empty((B, C), dtype=torch.complex128, device=f.device)
+        _check(_lib().gsh_adjoint_inverse(self._h, lmax, n, REAL if real else COMPLEX,
+                                          PRECISION[precision], STORAGE["fp64"], B, _ptr(f),
+                                          self.FieldSize(), _ptr(coeffs), C, _stream(stream)),
+               "gsh_adjoint_inverse")
         return coeffs if field.dim() == 3 else coeffs[0]
 
     # ---- reference-named single-field calls ------------------------------

@@ -159,7 +159,9 @@ struct gsh_grid_s {
   std::vector<double> theta, weight;
   double* d_cos = nullptr;
   double* d_wq = nullptr;
+  double* d_one = nullptr;     // unit colatitude weights (adjoint of the synthesis)
   double2* d_tw = nullptr;
+  double2* d_twa = nullptr;    // wavefront FFT twiddles (N in {512, 1024, 2048})
   double2* d_chirp = nullptr;  // Bluestein tables (ring length not a power of two)
   double2* d_hf = nullptr;
   double2* d_hb = nullptr;
@@ -387,6 +389,7 @@ FFTParams fft_params(gsh_grid_s* g, int lmax, int real, int batch) {
   q.nfield = batch;
   q.ncol = 2 * batch;
   q.tw = g->d_tw;
+  q.twa = g->d_twa;
   q.wq = g->d_wq;
   q.F = g->work;
   return q;
@@ -550,11 +553,28 @@ int gsh_grid_create(int lmax, int nmax, int mrange, int nrange, gsh_grid_t* grid
     GSH_HIP(hipMemcpy(g->d_hf, chf.data(), M * sizeof(double2), hipMemcpyHostToDevice));
     GSH_HIP(hipMemcpy(g->d_hb, chb.data(), M * sizeof(double2), hipMemcpyHostToDevice));
   }
+  if (N > 0 && gsh::wave_fft_supported(N)) {
+    // twa[k1 * 64 + j] = exp(-2 pi i j k1 / N), k1 < N / 64, j < 64
+    std::vector<double2> ta(N);
+    for (int k1 = 0; k1 < N / 64; k1++)
+      for (int j = 0; j < 64; j++) {
+        const long double a = -2.0L * kPiL * ((j * k1) % N) / N;
+        ta[k1 * 64 + j] =
+            make_double2(static_cast<double>(std::cos(a)), static_cast<double>(std::sin(a)));
+      }
+    GSH_HIP(hipMalloc(&g->d_twa, ta.size() * sizeof(double2)));
+    GSH_HIP(hipMemcpy(g->d_twa, ta.data(), ta.size() * sizeof(double2), hipMemcpyHostToDevice));
+  }
   GSH_HIP(hipMalloc(&g->d_cos, cs.size() * sizeof(double)));
   GSH_HIP(hipMalloc(&g->d_wq, wq.size() * sizeof(double)));
   GSH_HIP(hipMalloc(&g->d_tw, tw.size() * sizeof(double2)));
   GSH_HIP(hipMemcpy(g->d_cos, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice));
   GSH_HIP(hipMemcpy(g->d_wq, wq.data(), wq.size() * sizeof(double), hipMemcpyHostToDevice));
+  {
+    const std::vector<double> one(lmax + 1, 1.0);
+    GSH_HIP(hipMalloc(&g->d_one, one.size() * sizeof(double)));
+    GSH_HIP(hipMemcpy(g->d_one, one.data(), one.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   GSH_HIP(hipMemcpy(g->d_tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
   *grid = g.release();
   return GSH_OK;
@@ -571,7 +591,9 @@ int gsh_grid_destroy(gsh_grid_t g) {
   }
   (void)hipFree(g->d_cos);
   (void)hipFree(g->d_wq);
+  (void)hipFree(g->d_one);
   (void)hipFree(g->d_tw);
+  (void)hipFree(g->d_twa);
   if (g->d_chirp) (void)hipFree(g->d_chirp);
   if (g->d_hf) (void)hipFree(g->d_hf);
   if (g->d_hb) (void)hipFree(g->d_hb);
@@ -631,7 +653,7 @@ int gsh_inverse(gsh_grid_t g, int lmax, int n, int value_kind, int precision, in
   GSH_HIP(hipSetDevice(g->device));
   if (lmax == 0) {
     GSH_HIP(gsh::launch_lmax0(1, real, batch, static_cast<const double*>(coeffs), coeff_stride,
-                              static_cast<double*>(field), field_stride, 0, s));
+                              static_cast<double*>(field), field_stride, s));
     return GSH_OK;
   }
   NData* d;
@@ -663,13 +685,19 @@ int gsh_inverse(gsh_grid_t g, int lmax, int n, int value_kind, int precision, in
   FFTParams q = fft_params(g, lmax, real, batch);
   q.fout = field;
   q.fstride = field_stride;
-  GSH_HIP(timed(g, GSH_KERNEL_FFT_SYNTHESIS, s, [&] { return gsh::launch_fft_synth(q, s); }));
+  GSH_HIP(timed(g, GSH_KERNEL_FFT_SYNTHESIS, s, [&] {
+    return q.twa ? gsh::launch_fft_synth_wave(q, s) : gsh::launch_fft_synth(q, s);
+  }));
   return GSH_OK;
 }
 
-int gsh_forward(gsh_grid_t g, int lmax, int n, int value_kind, int precision, int storage,
-                int batch, const void* field, long field_stride, void* coeffs, long coeff_stride,
-                int accumulate, void* stream) {
+// Analysis (adjoint = 0) or the adjoint of the synthesis (adjoint = 1): the
+// same ring FFT and Legendre contraction, the adjoint with unit colatitude
+// weights, no 2 pi / N, no (L, L) zeroing, and for real fields the c2r
+// weights (2 on bins 0 < k < N/2, real part only on bins 0 and N/2).
+static int forward_impl(gsh_grid_t g, int lmax, int n, int value_kind, int precision, int storage,
+                        int batch, const void* field, long field_stride, void* coeffs,
+                        long coeff_stride, int accumulate, void* stream, int adjoint) {
   int st = check_call(g, lmax, n, value_kind, precision, storage, batch);
   if (st) return st;
   if (batch == 0) return GSH_OK;
@@ -683,8 +711,8 @@ int gsh_forward(gsh_grid_t g, int lmax, int n, int value_kind, int precision, in
   std::lock_guard<std::mutex> lock(g->mu);
   GSH_HIP(hipSetDevice(g->device));
   if (lmax == 0) {
-    GSH_HIP(gsh::launch_lmax0(0, real, batch, static_cast<const double*>(field), field_stride,
-                              static_cast<double*>(coeffs), coeff_stride, accumulate, s));
+    GSH_HIP(gsh::launch_lmax0(adjoint ? 2 : 0, real, batch, static_cast<const double*>(field),
+                              field_stride, static_cast<double*>(coeffs), coeff_stride, s));
     return GSH_OK;
   }
   NData* d;
@@ -694,16 +722,22 @@ int gsh_forward(gsh_grid_t g, int lmax, int n, int value_kind, int precision, in
   FFTParams q = fft_params(g, lmax, real, batch);
   q.fin = field;
   q.fstride = field_stride;
+  if (adjoint) {
+    q.wq = g->d_one;
+    q.adjoint = 1;
+  }
   LegParams p = base_params(g, d, lmax, n, real, batch);
   p.cout = static_cast<double*>(coeffs);
   p.cstride = coeff_stride;
   p.accumulate = accumulate ? 1 : 0;
-  p.zero_last = (!real && lmax == g->L) ? 1 : 0;
+  p.zero_last = (!adjoint && !real && lmax == g->L) ? 1 : 0;
+  const size_t fa = (size_t)nslot * ((ncol + 15) / 16) * ((g->ntheta() + 3) / 4) * 64;
+  if ((st = ensure_work(g, sizeof(double) * fa))) return st;
+  q.F = p.F = g->work;
+  GSH_HIP(timed(g, GSH_KERNEL_FFT_ANALYSIS, s, [&] {
+    return q.twa ? gsh::launch_fft_anal_wave(q, s) : gsh::launch_fft_anal(q, s);
+  }));
   if (precision == GSH_FP64) {
-    const size_t fa = (size_t)nslot * ((ncol + 15) / 16) * ((g->ntheta() + 3) / 4) * 64;
-    if ((st = ensure_work(g, sizeof(double) * fa))) return st;
-    q.F = p.F = g->work;
-    GSH_HIP(timed(g, GSH_KERNEL_FFT_ANALYSIS, s, [&] { return gsh::launch_fft_anal(q, s); }));
     GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_ANALYSIS, s,
                   [&] { return gsh::launch_leg_anal_f64(p, s); }));
     return GSH_OK;
@@ -711,18 +745,24 @@ int gsh_forward(gsh_grid_t g, int lmax, int n, int value_kind, int precision, in
   const int ns = precision == GSH_BF16X6 ? 3 : 2;
   const int nkc = (g->ntheta() + 31) / 32;
   const int ntile = (ncol + 15) / 16;
-  if ((st = ensure_work2(g, (size_t)nslot * nkc * ntile * ns * 64 * 16))) return st;
-  q.ns = ns;
-  q.nkc = nkc;
-  q.ntile = ntile;
-  q.xs = static_cast<__bf16*>(g->work2);
+  const long nblocks = (long)nslot * nkc * ntile;
+  if ((st = ensure_work2(g, (size_t)nblocks * ns * 64 * 16))) return st;
   p.bs = static_cast<const uint4*>(g->work2);
+  p.bs_out = static_cast<uint4*>(g->work2);
   p.nkc = nkc;
   p.ntile = ntile;
-  GSH_HIP(timed(g, GSH_KERNEL_FFT_ANALYSIS, s, [&] { return gsh::launch_fft_anal(q, s); }));
+  GSH_HIP(timed(g, GSH_KERNEL_BIN_SPLIT, s,
+                [&] { return gsh::launch_bin_split(p, ns, nblocks, s); }));
   GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_ANALYSIS, s,
                 [&] { return gsh::launch_leg_anal_bf16(p, ns, s); }));
   return GSH_OK;
+}
+
+int gsh_forward(gsh_grid_t g, int lmax, int n, int value_kind, int precision, int storage,
+                int batch, const void* field, long field_stride, void* coeffs, long coeff_stride,
+                int accumulate, void* stream) {
+  return forward_impl(g, lmax, n, value_kind, precision, storage, batch, field, field_stride,
+                      coeffs, coeff_stride, accumulate, stream, 0);
 }
 
 int gsh_timing_enable(gsh_grid_t g, int enable) {
@@ -764,9 +804,11 @@ int gsh_timing_read(gsh_grid_t g, double* total_ms, long* launches, int nkernels
   return GSH_OK;
 }
 
-int gsh_adjoint_inverse(gsh_grid_t, int, int, int, int, int, int, const void*, long, void*,
-                        long, void*) {
-  return GSH_ERR_UNSUPPORTED;
+int gsh_adjoint_inverse(gsh_grid_t g, int lmax, int n, int value_kind, int precision, int storage,
+                        int batch, const void* field, long field_stride, void* coeffs,
+                        long coeff_stride, void* stream) {
+  return forward_impl(g, lmax, n, value_kind, precision, storage, batch, field, field_stride,
+                      coeffs, coeff_stride, 0, stream, 1);
 }
 
 static int stage_ensure(gsh_grid_s* g, size_t bytes) {

@@ -240,24 +240,11 @@ __global__ __launch_bounds__(kFFTThreads) void fft_anal(FFTParams p) {
     if (f >= p.nfield) continue;
     const int m = s + p.m_first;
     const int k = m >= 0 ? m : N + m;
-    const double2 X = buf[q * M + k];
-    if (p.ns > 0) {
-      // Split X' into ns bf16 planes (f64 -> f32 -> bf16 remainders).
-      float v[2] = {(float)(X.x * w), (float)(X.y * w)};
-      asm volatile("" : "+v"(v[0]), "+v"(v[1]));
-      for (int pl = 0; pl < p.ns; pl++) {
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-          const __bf16 b = (__bf16)v[u];
-          p.xs[xs_index(s, 2 * f + u, theta, pl, p.ns, p.nkc, p.ntile)] = b;
-          v[u] -= (float)b;
-        }
-      }
-      for (int tp = theta + 1; theta == p.ntheta - 1 && tp < 32 * p.nkc; tp++)
-        for (int pl = 0; pl < p.ns; pl++)
-          for (int u = 0; u < 2; u++)
-            p.xs[xs_index(s, 2 * f + u, tp, pl, p.ns, p.nkc, p.ntile)] = (__bf16)0.0f;
-      continue;
+    double2 X = buf[q * M + k];
+    if (p.adjoint && p.real) {
+      // c2r bin weights: 2 for 0 < k < N/2, real part only at k = 0, N/2
+      if (k == 0 || 2 * k == N) X.y = 0.0;
+      else X = make_double2(2.0 * X.x, 2.0 * X.y);
     }
     double* Fp = p.F + fa_index(s, 2 * f, theta, p.n_cg, p.nquad);
     Fp[0] = X.x * w;
@@ -273,13 +260,16 @@ __global__ __launch_bounds__(kFFTThreads) void fft_anal(FFTParams p) {
 }
 
 // lmax == 0 branches of the reference act on element 0 only
-// (GaussLegendreGrid.h:137-140 and :251-260).
-__global__ void lmax0_kernel(int inverse, int real, int nfield, const double* in, long istride,
-                             double* out, long ostride, int accumulate) {
+// (GaussLegendreGrid.h:137-140 and :251-260): the synthesis writes
+// out[0] = c[0] 2 sqrt(pi), the analysis assigns c[0] = f[0] / (2 sqrt(pi))
+// (even though its general branch accumulates).  mode 2 is the adjoint of
+// that synthesis: c[0] = 2 sqrt(pi) f[0] (real part only for real fields).
+__global__ void lmax0_kernel(int mode, int real, int nfield, const double* in, long istride,
+                             double* out, long ostride) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nfield) return;
   const double inv_sqrtpi = 0.56418958354775628695;  // 1/sqrt(pi)
-  if (inverse) {
+  if (mode == 1) {
     // in: complex coefficients; out: real or complex field
     const double* c = in + 2 * (long)f * istride;
     if (real) {
@@ -292,17 +282,16 @@ __global__ void lmax0_kernel(int inverse, int real, int nfield, const double* in
     // in: field; out: complex coefficients
     double re, im;
     if (real) {
-      re = in[(long)f * istride] * inv_sqrtpi / 2.0;
+      re = in[(long)f * istride];
       im = 0.0;
     } else {
-      re = in[2 * (long)f * istride] * inv_sqrtpi / 2.0;
-      im = in[2 * (long)f * istride + 1] * inv_sqrtpi / 2.0;
+      re = in[2 * (long)f * istride];
+      im = in[2 * (long)f * istride + 1];
     }
-    // The reference assigns here even though its general branch accumulates.
+    const double sc = mode == 2 ? 2.0 / inv_sqrtpi : inv_sqrtpi / 2.0;
     double* c = out + 2 * (long)f * ostride;
-    (void)accumulate;
-    c[0] = re;
-    c[1] = im;
+    c[0] = re * sc;
+    c[1] = im * sc;
   }
 }
 
@@ -328,10 +317,10 @@ hipError_t launch_fft_anal(const FFTParams& p0, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_lmax0(int inverse, int real, int nfield, const double* in, long istride,
-                        double* out, long ostride, int accumulate, hipStream_t s) {
-  hipLaunchKernelGGL(lmax0_kernel, dim3((nfield + 255) / 256), dim3(256), 0, s, inverse, real,
-                     nfield, in, istride, out, ostride, accumulate);
+hipError_t launch_lmax0(int mode, int real, int nfield, const double* in, long istride,
+                        double* out, long ostride, hipStream_t s) {
+  hipLaunchKernelGGL(lmax0_kernel, dim3((nfield + 255) / 256), dim3(256), 0, s, mode, real,
+                     nfield, in, istride, out, ostride);
   return hipGetLastError();
 }
 

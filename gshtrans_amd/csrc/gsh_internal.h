@@ -66,6 +66,14 @@ struct LegParams {
   int nkc;              // analysis: 32-colatitude chunks, ceil(ntheta / 32)
 };
 
+// Load of a wave-uniform table entry through the constant address space, so
+// that it becomes a scalar load into an SGPR (the tables are written only by
+// the host).
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T* p, long i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
 // Index of (slot, col, theta) in the analysis work layout FA.
 __host__ __device__ inline long fa_index(int slot, int col, int theta, int n_cg, int nquad) {
   return (((long)slot * n_cg + (col >> 4)) * nquad + (theta >> 2)) * 64 + 16 * (theta & 3) +
@@ -83,20 +91,17 @@ struct FFTParams {
   int n_cg;         // analysis: 16-column groups, ceil(ncol / 16)
   int nquad;        // analysis: colatitude quads
   const double2* tw;  // tw[k] = exp(-2 pi i k / M), k < M
+  const double2* twa; // wavefront FFT: twa[k1 * 64 + j] = exp(-2 pi i j k1 / N), k1 < N / 64
   const double2* chirp;  // Bluestein: exp(-i pi k^2 / N), k < N
   const double2* hf;     // Bluestein: FFT_M of the forward filter / M
   const double2* hb;     // Bluestein: FFT_M of the backward filter / M
-  const double* wq;   // analysis weights w_t * 2 pi / N
+  const double* wq;   // analysis weights w_t * 2 pi / N (all 1 for the adjoint)
+  int adjoint;        // adjoint of the synthesis: real fields take the c2r
+                      // bin weights (2 for 0 < k < N/2, Re only at 0, N/2)
   double* F;
   const void* fin;    // analysis input field
   void* fout;         // synthesis output field
   long fstride;       // elements between fields
-  // Mixed-precision analysis: ns > 0 writes the weighted bins as ns bf16
-  // planes in the split layout of leg_anal_bf16 (xs) instead of FP64 FA.
-  int ns;
-  int nkc;            // 32-colatitude chunks
-  int ntile;          // 16-column tiles
-  __bf16* xs;
 };
 
 // bf16 index of (slot, col, theta, plane s) in the split analysis layout:
@@ -112,12 +117,19 @@ hipError_t launch_leg_synth_f64(const LegParams& p, hipStream_t s);
 hipError_t launch_leg_anal_f64(const LegParams& p, hipStream_t s);
 hipError_t launch_fft_synth(const FFTParams& p, hipStream_t s);
 hipError_t launch_fft_anal(const FFTParams& p, hipStream_t s);
+// Wavefront-per-ring FFTs (gsh_fft_wave.hip) for N in {512, 1024, 2048}.
+bool wave_fft_supported(int N);
+hipError_t launch_fft_synth_wave(const FFTParams& p, hipStream_t s);
+hipError_t launch_fft_anal_wave(const FFTParams& p, hipStream_t s);
 // Mixed precision kernels (gsh_mixed.hip); ns = 2 (bf16x3) or 3 (bf16x6).
 hipError_t launch_coef_split(const LegParams& p, int ns, long nblocks, hipStream_t s);
 hipError_t launch_leg_synth_bf16(const LegParams& p, int ns, hipStream_t s);
 hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s);
+// FA (FP64 analysis bins) -> split bf16 analysis operands (xs_index layout).
+hipError_t launch_bin_split(const LegParams& p, int ns, long nblocks, hipStream_t s);
 
-hipError_t launch_lmax0(int inverse, int real, int nfield, const double* in, long istride,
-                        double* out, long ostride, int accumulate, hipStream_t s);
+// lmax == 0: mode 1 synthesis, 0 analysis, 2 adjoint of the synthesis.
+hipError_t launch_lmax0(int mode, int real, int nfield, const double* in, long istride,
+                        double* out, long ostride, hipStream_t s);
 
 }  // namespace gsh

@@ -76,22 +76,23 @@ template <int T, int REAL>
 __global__ __launch_bounds__(256) void leg_synth_f64(LegParams p) {
   __shared__ double tr[4][4][66];
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  int blk = blockIdx.x;
-  const int tsb = blk % p.n_tsb;
-  blk /= p.n_tsb;
-  const int cg = blk % p.n_cg;
-  const int mw = blk / p.n_cg;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int tsb = blockIdx.x;
+  const int cg = blockIdx.y;
+  const int mw = blockIdx.z;
   const int m = order_of(mw, REAL);
   const int slot = m - p.m_first;
   const int l0 = max(abs(m), p.nabs);
   const int lmax = p.lmax;
   const int t = tsb * 256 + wave * 64 + lane;
   const bool tv = t < p.ntheta;
-  const double cs = tv ? p.cosv[t] : 0.0;
-  double e0 = tv ? p.seeds[(long)(m + p.L) * p.ntheta + t] : 0.0;
+  // Branch-free loads at a clamped colatitude keep the control flow uniform.
+  const int tc = min(t, p.ntheta - 1);
+  const double cs0 = p.cosv[tc], s0 = p.seeds[(long)(m + p.L) * p.ntheta + tc];
+  const double cs = tv ? cs0 : 0.0;
+  double e0 = tv ? s0 : 0.0;
   double e1 = 0.0;
-  rec_t* rc = rec_ptr(p.rec, p.roff[m + p.L] - l0);  // rc[l]: step to degree l
+  rec_t* rc = rec_ptr(p.rec, ld_uniform(p.roff, m + p.L) - l0);  // rc[l]: step to degree l
   const int krow = lane >> 4;
 
   // B-operand column of this lane per column tile.  Every load reads a valid
@@ -192,13 +193,13 @@ template <int MB>
 __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
   __shared__ double tr[4][16][66];  // also holds the cross-wave reduction
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int cg = blockIdx.x % p.n_cg;
-  const int mw = blockIdx.x / p.n_cg;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int cg = blockIdx.x;
+  const int mw = blockIdx.y;
   const int m = order_of(mw, p.real);
   const int l0 = max(abs(m), p.nabs);
   const int lmax = p.lmax;
-  rec_t* rc = rec_ptr(p.rec, p.roff[m + p.L] - l0);
+  rec_t* rc = rec_ptr(p.rec, ld_uniform(p.roff, m + p.L) - l0);
   const int slot = m - p.m_first;
   const int krow = lane >> 4;
   const int col = cg * 16 + (lane & 15);
@@ -231,8 +232,10 @@ __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
   const int tq = (ntailq - wave + 3) >> 2;  // tail quads of this wave
   const int tt = tail0 + 16 * (lane >> 2) + 4 * wave + (lane & 3);
   const bool ttv = tt < nt;
-  double cst = ttv ? p.cosv[tt] : 0.0;
-  double e0t = ttv ? p.seeds[(long)(m + p.L) * nt + tt] : 0.0;
+  const int ttc = min(tt, nt - 1);
+  const double cs0 = p.cosv[ttc], s0 = p.seeds[(long)(m + p.L) * nt + ttc];
+  double cst = ttv ? cs0 : 0.0;
+  double e0t = ttv ? s0 : 0.0;
   double e1t = 0.0;
 
   auto loadBlk = [&](int bi, double* Bq) {
@@ -325,8 +328,7 @@ __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
 
 template <int MB>
 hipError_t launch_anal(const LegParams& p, hipStream_t s) {
-  const long nblocks = (long)p.nslot * p.n_cg;
-  hipLaunchKernelGGL(leg_anal_f64<MB>, dim3(nblocks), dim3(256), 0, s, p);
+  hipLaunchKernelGGL(leg_anal_f64<MB>, dim3(p.n_cg, p.nslot), dim3(256), 0, s, p);
   return hipGetLastError();
 }
 
@@ -337,15 +339,15 @@ hipError_t launch_leg_synth_f64(const LegParams& p0, hipStream_t s) {
   p.n_tsb = (p.ntheta + 255) / 256;
   const int T = p.ncol > 16 ? 2 : 1;
   p.n_cg = (p.ncol + 16 * T - 1) / (16 * T);
-  const long nblocks = (long)p.nslot * p.n_cg * p.n_tsb;
+  const dim3 nblocks(p.n_tsb, p.n_cg, p.nslot);  // (theta block, column group, order)
   if (T == 2 && p.real)
-    hipLaunchKernelGGL((leg_synth_f64<2, 1>), dim3(nblocks), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((leg_synth_f64<2, 1>), nblocks, dim3(256), 0, s, p);
   else if (T == 2)
-    hipLaunchKernelGGL((leg_synth_f64<2, 0>), dim3(nblocks), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((leg_synth_f64<2, 0>), nblocks, dim3(256), 0, s, p);
   else if (p.real)
-    hipLaunchKernelGGL((leg_synth_f64<1, 1>), dim3(nblocks), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((leg_synth_f64<1, 1>), nblocks, dim3(256), 0, s, p);
   else
-    hipLaunchKernelGGL((leg_synth_f64<1, 0>), dim3(nblocks), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((leg_synth_f64<1, 0>), nblocks, dim3(256), 0, s, p);
   return hipGetLastError();
 }
 

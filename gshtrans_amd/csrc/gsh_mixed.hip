@@ -81,19 +81,29 @@ __device__ __forceinline__ f4 mfma(uint4 a, uint4 b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(a), as_bf8(b), c, 0, 0, 0);
 }
 
-// Split product of one A and one B operand (NS planes each) into the FP64
-// accumulator; cross products first, leading product last.
-template <int NS>
-__device__ __forceinline__ void split_mma(const uint4* A, const uint4* B, d4& acc) {
-  f4 c = {0.f, 0.f, 0.f, 0.f};
-  if (NS == 3) {
-    c = mfma(A[1], B[1], c);
-    c = mfma(A[0], B[2], c);
-    c = mfma(A[2], B[0], c);
-  }
-  c = mfma(A[0], B[1], c);
-  c = mfma(A[1], B[0], c);
-  c = mfma(A[0], B[0], c);
+// Split products of NA A operands (rows) with two B operands (columns), NS
+// planes each, as 2 NA independent MFMA chains advanced one product at a
+// time (so consecutive MFMAs never depend on each other); within each chain
+// the cross products go first and the leading product last.
+template <int NS, int NA>
+__device__ __forceinline__ void split_mma(const uint4 (&A)[NA][NS], const uint4 (&B)[2][NS],
+                                          f4 (&c)[NA][2]) {
+  constexpr int np = NS == 3 ? 6 : 3;
+  constexpr int pa[6] = {1, 0, 2, 0, 1, 0};  // plane of A in product q
+  constexpr int pb[6] = {1, 2, 0, 1, 0, 0};  // plane of B in product q
+#pragma unroll
+  for (int i = 0; i < NA; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) c[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int q = 6 - np; q < 6; q++)
+#pragma unroll
+    for (int i = 0; i < NA; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) c[i][j] = mfma(A[i][pa[q]], B[j][pb[q]], c[i][j]);
+}
+
+__device__ __forceinline__ void flush(d4& acc, const f4 c) {
   acc[0] += (double)c[0];
   acc[1] += (double)c[1];
   acc[2] += (double)c[2];
@@ -108,7 +118,7 @@ __device__ __forceinline__ void split_mma(const uint4* A, const uint4* B, d4& ac
 // ---------------------------------------------------------------------------
 template <int NS, int REAL>
 __global__ __launch_bounds__(256) void coef_split(LegParams p, long nblocks) {
-  const long b = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long b = (long)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (b >= nblocks) return;
   const int lane = threadIdx.x & 63;
   const long cb = b / p.ntile;
@@ -117,12 +127,12 @@ __global__ __launch_bounds__(256) void coef_split(LegParams p, long nblocks) {
   int lo = 0, hi = p.nslot - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (p.cso[mid] <= cb) lo = mid; else hi = mid - 1;
+    if (ld_uniform(p.cso, mid) <= cb) lo = mid; else hi = mid - 1;
   }
   const int slot = lo;
   const int m = slot + p.m_first;
   const int l0 = max(abs(m), p.nabs);
-  const int chunk = (int)(cb - p.cso[slot]);
+  const int chunk = (int)(cb - ld_uniform(p.cso, slot));
   const int kr = lane >> 4;
   const int col = tile * 16 + (lane & 15);
   const bool cv = col < p.ncol;
@@ -149,6 +159,48 @@ __global__ __launch_bounds__(256) void coef_split(LegParams p, long nblocks) {
 }
 
 // ---------------------------------------------------------------------------
+// Bin split: analysis B operands.  Block ((slot * nkc + kchunk) * ntile +
+// tile, s) holds, for lane = 16*q + c, the bf16 plane s of the weighted bins
+// X'(theta, col) for theta = 32*kchunk + 8*q + e (e = 0..7) and col =
+// 16*tile + c, read from the FP64 layout FA; colatitudes past ntheta and
+// columns past ncol are zero.  For a fixed e the 64 lanes read four whole
+// 128-byte rows of FA.
+// ---------------------------------------------------------------------------
+template <int NS>
+__global__ __launch_bounds__(256) void bin_split(LegParams p, long nblocks) {
+  const long b = (long)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (b >= nblocks) return;
+  const int lane = threadIdx.x & 63;
+  const int tile = (int)(b % p.ntile);
+  const long sk = b / p.ntile;
+  const int kc = (int)(sk % p.nkc);
+  const int slot = (int)(sk / p.nkc);
+  const int col = tile * 16 + (lane & 15);
+  const int t0 = 32 * kc + 8 * (lane >> 4);
+  const bool cv = col < p.ncol;
+  unsigned planes[NS][4];
+#pragma unroll
+  for (int e2 = 0; e2 < 4; e2++) {
+    double x[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int t = t0 + 2 * e2 + u;
+      const bool ok = cv && t < p.ntheta;
+      const double v = p.F[fa_index(slot, col, ok ? t : 0, p.n_cg, p.nquad)];
+      x[u] = ok ? v : 0.0;
+    }
+    unsigned o[NS];
+    split_pair<NS>(x[0], x[1], o);
+#pragma unroll
+    for (int s = 0; s < NS; s++) planes[s][e2] = o[s];
+  }
+#pragma unroll
+  for (int s = 0; s < NS; s++)
+    p.bs_out[(b * NS + s) * 64 + lane] =
+        make_uint4(planes[s][0], planes[s][1], planes[s][2], planes[s][3]);
+}
+
+// ---------------------------------------------------------------------------
 // Synthesis.  Workgroup = 8 waves: (256 colatitudes of super-block tsb) x
 // (128 columns of column super-group csg) for one order m.  Wave w < 4
 // produces the split D of colatitudes 64w..64w+63; wave w consumes
@@ -159,26 +211,30 @@ template <int NS, int REAL>
 __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
   __shared__ uint4 trb[2][NS][4][256];
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  int blk = blockIdx.x;
-  const int tsb = blk % p.n_tsb;
-  blk /= p.n_tsb;
-  const int csg = blk % p.n_cg;
-  const int mw = blk / p.n_cg;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int tsb = blockIdx.x;
+  const int csg = blockIdx.y;
+  const int mw = blockIdx.z;
   const int m = order_of(mw, REAL);
   const int slot = m - p.m_first;
   const int l0 = max(abs(m), p.nabs);
   const int lmax = p.lmax;
   const int nchunk = (lmax - l0 + 32) >> 5;
-  const bool prod = wave < 4;
+  // Producers whose 64 colatitudes all lie past ntheta have nothing to do;
+  // consumers skip row groups past ntheta (2 ngp of 8 remain).
+  const bool prod = wave < 4 && tsb * 256 + (wave & 3) * 64 < p.ntheta;
+  const int ngp = max(0, min(4, (p.ntheta - tsb * 256 - (wave >> 2) * 128 + 31) / 32));
 
   // Producer state: colatitude tp.
   const int tp = tsb * 256 + (wave & 3) * 64 + lane;
   const bool tpv = prod && tp < p.ntheta;
-  const double cs = tpv ? p.cosv[tp] : 0.0;
-  double e0 = tpv ? p.seeds[(long)(m + p.L) * p.ntheta + tp] : 0.0;
+  // Branch-free loads at a clamped colatitude keep the control flow uniform.
+  const int tpc = min(tp, p.ntheta - 1);
+  const double cs0 = p.cosv[tpc], s0 = p.seeds[(long)(m + p.L) * p.ntheta + tpc];
+  const double cs = tpv ? cs0 : 0.0;
+  double e0 = tpv ? s0 : 0.0;
   double e1 = 0.0;
-  rec_t* rc = rec_ptr(p.rec, p.roff[m + p.L] - l0);
+  rec_t* rc = rec_ptr(p.rec, ld_uniform(p.roff, m + p.L) - l0);
 
   auto produce = [&](int chunk, int buf) {
     const int lb = l0 + 32 * chunk;
@@ -207,7 +263,7 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
   const int h = wave >> 2;
   const int tile0 = csg * 8 + (wave & 3) * 2;
   const bool tv0 = tile0 < p.ntile, tv1 = tile0 + 1 < p.ntile;
-  const uint4* bbase = p.bs + (p.cso[slot] * (long)p.ntile) * NS * 64 + lane;
+  const uint4* bbase = p.bs + (ld_uniform(p.cso, slot) * (long)p.ntile) * NS * 64 + lane;
   auto loadB = [&](int chunk, uint4 (&B)[2][NS]) {
     const long c = (long)min(chunk, nchunk - 1) * p.ntile;
 #pragma unroll
@@ -235,13 +291,20 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
     uint4 Bn[2][NS];
     loadB(k + 1, Bn);
     const int buf = k & 1;
+    // One row group at a time: two independent MFMA chains (its two column
+    // tiles), flushed to FP64 after each group.
 #pragma unroll
     for (int g = 0; g < 8; g++) {
-      uint4 A[NS];
+      if (ngp == 4 || g < 2 * ngp) {
+        uint4 A[1][NS];
 #pragma unroll
-      for (int s = 0; s < NS; s++) A[s] = trb[buf][s][lane >> 4][h * 128 + g * 16 + (lane & 15)];
-      split_mma<NS>(A, B[0], acc[g][0]);
-      split_mma<NS>(A, B[1], acc[g][1]);
+        for (int s = 0; s < NS; s++)
+          A[0][s] = trb[buf][s][lane >> 4][h * 128 + g * 16 + (lane & 15)];
+        f4 c[1][2];
+        split_mma<NS, 1>(A, B, c);
+        flush(acc[g][0], c[0][0]);
+        flush(acc[g][1], c[0][1]);
+      }
     }
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -284,9 +347,9 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   __shared__ __bf16 Dl[2][NS][64][128];
   __shared__ double2 st[kAnalBf16MaxBlocks][64];
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int csg = blockIdx.x % p.n_cg;
-  const int mw = blockIdx.x / p.n_cg;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int csg = blockIdx.x;
+  const int mw = blockIdx.y;
   const int m = order_of(mw, REAL);
   const int slot = m - p.m_first;
   const int l0 = max(abs(m), p.nabs);
@@ -294,19 +357,21 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   const int nt = p.ntheta;
   const int nblk = (nt + 63) >> 6;
   const int ngrp = (nblk + 1) >> 1;
-  rec_t* rc = rec_ptr(p.rec, p.roff[m + p.L] - l0);
+  rec_t* rc = rec_ptr(p.rec, ld_uniform(p.roff, m + p.L) - l0);
   const double* seeds = p.seeds + (long)(m + p.L) * nt;
 
   // Initial recurrence state of every block.
   for (int b = wave; b < nblk; b += 8) {
     const int t = b * 64 + lane;
-    st[b][lane] = make_double2(t < nt ? seeds[t] : 0.0, 0.0);
+    const double s0 = seeds[min(t, nt - 1)];
+    st[b][lane] = make_double2(t < nt ? s0 : 0.0, 0.0);
   }
 
   // Produce the bf16 planes of block b (degrees lp..lp+63) into buffer half bb.
   auto produce = [&](int b, int lp, int buf, int bb) {
     const int t = b * 64 + lane;
-    const double cs = t < nt ? p.cosv[t] : 0.0;
+    const double cs0 = p.cosv[min(t, nt - 1)];
+    const double cs = t < nt ? cs0 : 0.0;
     double2 s2 = st[b][lane];
     double e0 = s2.x, e1 = s2.y;
 #pragma unroll 4
@@ -326,19 +391,12 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
     }
     st[b][lane] = make_double2(e0, e1);
   };
+  // Blocks past the last colatitude are never read (their K chunks are >= nkc).
   auto produce_group = [&](int g, int lp, int buf) {
 #pragma unroll
     for (int bb = 0; bb < 2; bb++) {
       const int b = 2 * g + bb;
-      if ((b & 7) == wave) {
-        if (b < nblk) {
-          produce(b, lp, buf, bb);
-        } else {
-#pragma unroll
-          for (int s = 0; s < NS; s++)
-            for (int k = 0; k < 64; k++) Dl[buf][s][k][bb * 64 + lane] = (__bf16)0.0f;
-        }
-      }
+      if ((b & 7) == wave && b < nblk) produce(b, lp, buf, bb);
     }
   };
 
@@ -370,22 +428,70 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
 
     produce_group(0, lp, 0);
     __syncthreads();
+    // B operands are loaded one K chunk ahead of their use.
+    uint4 B[2][NS];
+    loadB(0, B);
     for (int g = 0; g < ngrp; g++) {
       if (g + 1 < ngrp) produce_group(g + 1, lp, (g + 1) & 1);
       const int buf = g & 1;
+      auto readA = [&](int kq, uint4 (&A)[2][NS]) {
 #pragma unroll
-      for (int kq = 0; kq < 4; kq++) {
-        uint4 B[2][NS];
-        loadB(4 * g + kq, B);
-#pragma unroll
-        for (int lt = 0; lt < 2; lt++) {
-          uint4 A[NS];
+        for (int lt = 0; lt < 2; lt++)
 #pragma unroll
           for (int s = 0; s < NS; s++)
-            A[s] = *reinterpret_cast<const uint4*>(
+            A[lt][s] = *reinterpret_cast<const uint4*>(
                 &Dl[buf][s][(2 * lh + lt) * 16 + (lane & 15)][kq * 32 + 8 * (lane >> 4)]);
-          split_mma<NS>(A, B[0], acc[lt][0]);
-          split_mma<NS>(A, B[1], acc[lt][1]);
+      };
+      const int nkq = min(4, p.nkc - 4 * g);
+      if (nkq == 4) {
+        // The chains of chunk kq are flushed to FP64 while those of kq + 1 run.
+        f4 cp[2][2];
+#pragma unroll
+        for (int kq = 0; kq < 4; kq++) {
+          uint4 Bn[2][NS];
+          loadB(4 * g + kq + 1, Bn);
+          uint4 A[2][NS];
+          readA(kq, A);
+          f4 c[2][2];
+          split_mma<NS, 2>(A, B, c);
+          if (kq > 0) {
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+              for (int j = 0; j < 2; j++) flush(acc[i][j], cp[i][j]);
+          }
+#pragma unroll
+          for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) cp[i][j] = c[i][j];
+#pragma unroll
+          for (int s = 0; s < NS; s++) {
+            B[0][s] = Bn[0][s];
+            B[1][s] = Bn[1][s];
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) flush(acc[i][j], cp[i][j]);
+      } else {
+        // Last, partial group: only the K chunks below nkc.
+        for (int kq = 0; kq < nkq; kq++) {
+          uint4 Bn[2][NS];
+          loadB(4 * g + kq + 1, Bn);
+          uint4 A[2][NS];
+          readA(kq, A);
+          f4 c[2][2];
+          split_mma<NS, 2>(A, B, c);
+#pragma unroll
+          for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) flush(acc[i][j], c[i][j]);
+#pragma unroll
+          for (int s = 0; s < NS; s++) {
+            B[0][s] = Bn[0][s];
+            B[1][s] = Bn[1][s];
+          }
         }
       }
       __syncthreads();
@@ -428,11 +534,23 @@ hipError_t launch_coef_split(const LegParams& p, int ns, long nblocks, hipStream
   return hipGetLastError();
 }
 
+hipError_t launch_bin_split(const LegParams& p0, int ns, long nblocks, hipStream_t s) {
+  LegParams p = p0;
+  p.n_cg = (p.ncol + 15) / 16;
+  p.nquad = (p.ntheta + 3) / 4;
+  const dim3 grid((nblocks + 3) / 4);
+  if (ns == 3)
+    hipLaunchKernelGGL((bin_split<3>), grid, dim3(256), 0, s, p, nblocks);
+  else
+    hipLaunchKernelGGL((bin_split<2>), grid, dim3(256), 0, s, p, nblocks);
+  return hipGetLastError();
+}
+
 hipError_t launch_leg_synth_bf16(const LegParams& p0, int ns, hipStream_t s) {
   LegParams p = p0;
   p.n_tsb = (p.ntheta + 255) / 256;
   p.n_cg = (p.ntile + 7) / 8;  // column super-groups of 8 tiles
-  const dim3 grid((long)p.nslot * p.n_cg * p.n_tsb);
+  const dim3 grid(p.n_tsb, p.n_cg, p.nslot);  // (theta block, column group, order)
   if (ns == 3 && p.real)
     hipLaunchKernelGGL((leg_synth_bf16<3, 1>), grid, dim3(512), 0, s, p);
   else if (ns == 3)
@@ -448,7 +566,7 @@ template <int NS>
 hipError_t launch_anal_bf16_t(const LegParams& p0, hipStream_t s) {
   LegParams p = p0;
   p.n_cg = (p.ntile + 7) / 8;
-  const dim3 grid((long)p.nslot * p.n_cg);
+  const dim3 grid(p.n_cg, p.nslot);  // (column group, order)
   if (p.real)
     hipLaunchKernelGGL((leg_anal_bf16<NS, 1>), grid, dim3(512), 0, s, p);
   else

@@ -125,7 +125,8 @@ enum gsh_kernel {
   GSH_KERNEL_FFT_ANALYSIS = 2,
   GSH_KERNEL_LEGENDRE_ANALYSIS = 3,
   GSH_KERNEL_COEF_SPLIT = 4,      /* mixed precision: coefficient split   */
-  GSH_KERNEL_COUNT = 5
+  GSH_KERNEL_BIN_SPLIT = 5,       /* mixed precision: analysis bin split  */
+  GSH_KERNEL_COUNT = 6
 };
 /* enable != 0 records a HIP event pair around every kernel launch of this
  * grid (on the launch stream) and clears the accumulated totals. */

@@ -244,3 +244,28 @@ def test_L1024_batch_round_trip_and_linearity(torch):
         f2 = g.inverse(L, n, dc[:2] * 0.5 + dc[2:4], real=real)
         lin = 0.5 * f[:2] + f[2:4]
         assert rel_l2(host(f2), host(lin)) < 1e-13
+
+
+@pytest.mark.parametrize("L,lmax,n,real,batch", [
+    (256, 256, 0, False, 9), (256, 256, 1, True, 9), (256, 130, -2, False, 3),
+    (256, 70, 2, True, 2), (512, 512, 1, True, 3), (512, 400, 0, False, 17),
+    (1024, 1024, 0, True, 3)])
+def test_wavefront_ring_fft_batches(torch, L, lmax, n, real, batch):
+    """Ring lengths 512, 1024 and 2048 take the one-wavefront-per-ring FFT;
+    batches that leave a workgroup's field group (8 complex or 16 real
+    fields) partly empty, odd real batches (the last real pair has one
+    field), and truncation lmax < L, each field against the oracle."""
+    say(f"[wavefft] L={L} lmax={lmax} n={n} real={real} batch={batch}")
+    g = _grid(L, 2)
+    og = oracle_grid(L, 2)
+    cs = G.random_coefficients(lmax, n, real=real, seed=SEED0 + 40 + L + lmax, batch=batch,
+                               grid_lmax=L)
+    f = host(g.inverse(lmax, n, dev(torch, cs), real=real))
+    c2 = host(g.forward(lmax, n, dev(torch, f), real=real))
+    check = sorted({0, batch - 1} if L == 1024 else set(range(batch)))
+    for b in check:
+        f_ref = og.inverse(lmax, n, cs[b], real)
+        assert rel_l2(f[b], f_ref) <= 1e-12, b
+        c_ref = og.forward(lmax, n, f[b], real)
+        assert rel_l2(c2[b], c_ref) <= 1e-12, b
+    assert float(np.max(np.abs(c2 - cs))) < 5e4 * EPS
