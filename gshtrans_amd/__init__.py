@@ -36,7 +36,8 @@ EXPORTS = [
     "gsh_wigner_seed", "gsh_grid_create", "gsh_grid_destroy", "gsh_grid_info",
     "gsh_grid_colatitudes", "gsh_grid_prepare", "gsh_inverse", "gsh_forward",
     "gsh_adjoint_inverse", "gsh_inverse_host", "gsh_forward_host",
-    "gsh_timing_enable", "gsh_timing_read",
+    "gsh_timing_enable", "gsh_timing_read", "gsh_plan_create", "gsh_plan_destroy",
+    "gsh_plan_inverse", "gsh_plan_forward", "gsh_plan_info",
 ]
 KERNELS = ["legendre_synthesis", "fft_synthesis", "fft_analysis", "legendre_analysis",
            "coef_split", "bin_split"]
@@ -391,3 +392,83 @@ class GaussLegendreGrid:
         self.forward(lmax, n, inp.reshape(1, self._lmax + 1, 2 * self._lmax),
                      coeffs=out.reshape(1, -1), real=real, accumulate=True)
         return out
+
+
+class BatchPlan:
+    """Many fields of uneven truncation degree and upper index on one grid
+    (gsh_plan_*): field i has lmax[i] and n[i].  Coefficients are a (count,
+    cstride) complex tensor (complex128 or complex64 for storage="fp32"),
+    field i's first coefficient_size(lmax[i], n[i]) entries in the reference
+    layout; fields are (count, ntheta, nphi).  The first call in each
+    direction autotunes the tile shapes; graph=True captures the launch
+    sequence for the given tensors into a HIP graph on first use and replays
+    it afterwards."""
+
+    def __init__(self, grid, lmax, n, real=False, precision="fp64", storage="fp64"):
+        import numpy as _np
+        self.grid = grid
+        self.lmax = _np.ascontiguousarray(lmax, dtype=_np.int32)
+        self.n = _np.ascontiguousarray(n, dtype=_np.int32)
+        self.real, self.precision, self.storage = real, precision, storage
+        self.count = len(self.lmax)
+        self.sizes = [coefficient_size(int(l), int(m), real) for l, m in zip(self.lmax, self.n)]
+        self.cmax = max(self.sizes)
+        self._h = ctypes.c_void_p()
+        ip = ctypes.POINTER(ctypes.c_int)
+        _check(_lib().gsh_plan_create(grid.handle, self.count,
+                                      self.lmax.ctypes.data_as(ip), self.n.ctypes.data_as(ip),
+                                      REAL if real else COMPLEX, PRECISION[precision],
+                                      STORAGE[storage], ctypes.byref(self._h)),
+               "gsh_plan_create")
+
+    def close(self):
+        if self._h:
+            _lib().gsh_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def field_dtype(self):
+        import torch
+        if self.storage == "fp32":
+            return torch.float32 if self.real else torch.complex64
+        return torch.float64 if self.real else torch.complex128
+
+    def coeff_dtype(self):
+        import torch
+        return torch.complex64 if self.storage == "fp32" else torch.complex128
+
+    def inverse(self, coeffs, field=None, graph=False, stream=None):
+        import torch
+        g = self.grid
+        if field is None:
+            field = torch.empty((self.count, g.NumberOfCoLatitudes(), g.NumberOfLongitudes()),
+                                dtype=self.field_dtype(), device=coeffs.device)
+        _check(_lib().gsh_plan_inverse(self._h, _ptr(coeffs), coeffs.shape[1], _ptr(field),
+                                       g.FieldSize(), 1 if graph else 0, _stream(stream)),
+               "gsh_plan_inverse")
+        return field
+
+    def forward(self, field, coeffs=None, graph=False, stream=None):
+        import torch
+        if coeffs is None:
+            coeffs = torch.zeros((self.count, self.cmax), dtype=self.coeff_dtype(),
+                                 device=field.device)
+        _check(_lib().gsh_plan_forward(self._h, _ptr(field), self.grid.FieldSize(),
+                                       _ptr(coeffs), coeffs.shape[1], 1 if graph else 0,
+                                       _stream(stream)), "gsh_plan_forward")
+        return coeffs
+
+    def info(self):
+        """Launch groups with their tuned choices."""
+        cap = 64
+        ng = ctypes.c_int()
+        arrs = [(ctypes.c_int * cap)() for _ in range(6)]
+        _check(_lib().gsh_plan_info(self._h, ctypes.byref(ng), cap, *arrs), "gsh_plan_info")
+        keys = ["n", "lmax", "count", "synth_tiles", "fft_inverse", "fft_forward"]
+        return [{k: arrs[j][i] for j, k in enumerate(keys)} for i in range(ng.value)]
+

@@ -10,7 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgshtrans_hip.so")
 
-SOURCES = ["gsh_capi.cpp", "gsh_legendre.hip", "gsh_fft.hip", "gsh_fft_wave.hip", "gsh_mixed.hip"]
+SOURCES = ["gsh_capi.cpp", "gsh_legendre.hip", "gsh_fft.hip", "gsh_fft_wave.hip", "gsh_mixed.hip",
+           "gsh_batch.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -20,6 +20,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/gshtrans_amd.h"
@@ -636,36 +637,40 @@ int gsh_grid_prepare(gsh_grid_t g, int n) {
   return ensure_ndata(g, n, &d);
 }
 
-int gsh_inverse(gsh_grid_t g, int lmax, int n, int value_kind, int precision, int storage,
-                int batch, const void* coeffs, long coeff_stride, void* field, long field_stride,
-                void* stream) {
-  int st = check_call(g, lmax, n, value_kind, precision, storage, batch);
-  if (st) return st;
-  if (batch == 0) return GSH_OK;
-  if (!coeffs || !field) return GSH_ERR_INVALID_ARGUMENT;
-  if (precision == GSH_FP32 || storage != GSH_STORE_FP64) return GSH_ERR_UNSUPPORTED;
-  const int real = value_kind == GSH_REAL;
-  const long csize = gsh_coefficient_size(lmax, n, value_kind);
-  const long fsize = static_cast<long>(g->ntheta()) * g->nphi();
-  if (batch > 1 && (coeff_stride < csize || field_stride < fsize)) return GSH_ERR_INVALID_ARGUMENT;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  std::lock_guard<std::mutex> lock(g->mu);
-  GSH_HIP(hipSetDevice(g->device));
-  if (lmax == 0) {
-    GSH_HIP(gsh::launch_lmax0(1, real, batch, static_cast<const double*>(coeffs), coeff_stride,
-                              static_cast<double*>(field), field_stride, s));
-    return GSH_OK;
-  }
-  NData* d;
-  if ((st = ensure_ndata(g, n, &d))) return st;
+namespace {
+
+// Options of one launch group of a batch plan (defaults: a plain batched call).
+struct RunOpts {
+  const int* fmap = nullptr;     // device: column field -> caller's field index
+  const int* f_lmax = nullptr;   // device: truncation degree per column field
+  const int* cg_lmax = nullptr;  // device: max f_lmax per 16-column group
+  int f32 = 0;                   // fields stored in FP32
+  int synth_tiles = 0;           // FP64 synthesis: 16-column tiles per workgroup (0: auto)
+  int fft = 0;                   // 0: auto, 1: wavefront-per-ring kernel, 2: LDS kernel
+};
+
+bool use_wave_fft(const FFTParams& q, const RunOpts& o) {
+  return q.twa != nullptr && o.fft != 2;
+}
+
+// Synthesis of `batch` column fields with lmax >= 1 (no argument checks; the
+// caller holds the grid lock and has prepared n).
+int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precision, int batch,
+                const double* cin, long cstride, void* field, long fstride, hipStream_t s,
+                const RunOpts& o) {
+  int st;
   const int nslot = real ? lmax + 1 : 2 * lmax + 1;
   if ((st = ensure_work(g, sizeof(double) * g->ntheta() * nslot * 2 * (size_t)batch))) return st;
   LegParams p = base_params(g, d, lmax, n, real, batch);
-  p.cin = static_cast<const double*>(coeffs);
-  p.cstride = coeff_stride;
-  if (precision == GSH_FP64) {
-    GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_SYNTHESIS, s,
-                  [&] { return gsh::launch_leg_synth_f64(p, s); }));
+  p.cin = cin;
+  p.cstride = cstride;
+  p.fmap = o.fmap;
+  p.f_lmax = o.f_lmax;
+  p.cg_lmax = o.cg_lmax;
+  if (precision == GSH_FP64 || precision == GSH_FP32) {
+    GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_SYNTHESIS, s, [&] {
+      return gsh::launch_leg_synth_f64(p, s, o.synth_tiles, precision == GSH_FP32);
+    }));
   } else {
     const int ns = precision == GSH_BF16X6 ? 3 : 2;
     long* cso = nullptr;
@@ -684,62 +689,53 @@ int gsh_inverse(gsh_grid_t g, int lmax, int n, int value_kind, int precision, in
   }
   FFTParams q = fft_params(g, lmax, real, batch);
   q.fout = field;
-  q.fstride = field_stride;
+  q.fstride = fstride;
+  q.f32 = o.f32;
+  q.fmap = o.fmap;
   GSH_HIP(timed(g, GSH_KERNEL_FFT_SYNTHESIS, s, [&] {
-    return q.twa ? gsh::launch_fft_synth_wave(q, s) : gsh::launch_fft_synth(q, s);
+    return use_wave_fft(q, o) ? gsh::launch_fft_synth_wave(q, s) : gsh::launch_fft_synth(q, s);
   }));
   return GSH_OK;
 }
 
-// Analysis (adjoint = 0) or the adjoint of the synthesis (adjoint = 1): the
-// same ring FFT and Legendre contraction, the adjoint with unit colatitude
-// weights, no 2 pi / N, no (L, L) zeroing, and for real fields the c2r
-// weights (2 on bins 0 < k < N/2, real part only on bins 0 and N/2).
-static int forward_impl(gsh_grid_t g, int lmax, int n, int value_kind, int precision, int storage,
-                        int batch, const void* field, long field_stride, void* coeffs,
-                        long coeff_stride, int accumulate, void* stream, int adjoint) {
-  int st = check_call(g, lmax, n, value_kind, precision, storage, batch);
-  if (st) return st;
-  if (batch == 0) return GSH_OK;
-  if (!coeffs || !field) return GSH_ERR_INVALID_ARGUMENT;
-  if (precision == GSH_FP32 || storage != GSH_STORE_FP64) return GSH_ERR_UNSUPPORTED;
-  const int real = value_kind == GSH_REAL;
-  const long csize = gsh_coefficient_size(lmax, n, value_kind);
-  const long fsize = static_cast<long>(g->ntheta()) * g->nphi();
-  if (batch > 1 && (coeff_stride < csize || field_stride < fsize)) return GSH_ERR_INVALID_ARGUMENT;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  std::lock_guard<std::mutex> lock(g->mu);
-  GSH_HIP(hipSetDevice(g->device));
-  if (lmax == 0) {
-    GSH_HIP(gsh::launch_lmax0(adjoint ? 2 : 0, real, batch, static_cast<const double*>(field),
-                              field_stride, static_cast<double*>(coeffs), coeff_stride, s));
-    return GSH_OK;
-  }
-  NData* d;
-  if ((st = ensure_ndata(g, n, &d))) return st;
+// Analysis (adjoint = 0) or the adjoint of the synthesis (adjoint = 1) of
+// `batch` column fields with lmax >= 1: the same ring FFT and Legendre
+// contraction, the adjoint with unit colatitude weights, no 2 pi / N, no
+// (L, L) zeroing, and for real fields the c2r weights (2 on bins
+// 0 < k < N/2, real part only on bins 0 and N/2).
+int run_forward(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precision, int batch,
+                const void* field, long fstride, double* cout, long cstride, int accumulate,
+                int adjoint, hipStream_t s, const RunOpts& o) {
+  int st;
   const int nslot = real ? lmax + 1 : 2 * lmax + 1;
   const int ncol = 2 * batch;
   FFTParams q = fft_params(g, lmax, real, batch);
   q.fin = field;
-  q.fstride = field_stride;
+  q.fstride = fstride;
+  q.f32 = o.f32;
+  q.fmap = o.fmap;
   if (adjoint) {
     q.wq = g->d_one;
     q.adjoint = 1;
   }
   LegParams p = base_params(g, d, lmax, n, real, batch);
-  p.cout = static_cast<double*>(coeffs);
-  p.cstride = coeff_stride;
+  p.cout = cout;
+  p.cstride = cstride;
   p.accumulate = accumulate ? 1 : 0;
-  p.zero_last = (!adjoint && !real && lmax == g->L) ? 1 : 0;
+  p.zero_last = (!adjoint && !real) ? 1 : 0;
+  p.fmap = o.fmap;
+  p.f_lmax = o.f_lmax;
+  p.cg_lmax = o.cg_lmax;
   const size_t fa = (size_t)nslot * ((ncol + 15) / 16) * ((g->ntheta() + 3) / 4) * 64;
   if ((st = ensure_work(g, sizeof(double) * fa))) return st;
   q.F = p.F = g->work;
   GSH_HIP(timed(g, GSH_KERNEL_FFT_ANALYSIS, s, [&] {
-    return q.twa ? gsh::launch_fft_anal_wave(q, s) : gsh::launch_fft_anal(q, s);
+    return use_wave_fft(q, o) ? gsh::launch_fft_anal_wave(q, s) : gsh::launch_fft_anal(q, s);
   }));
-  if (precision == GSH_FP64) {
-    GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_ANALYSIS, s,
-                  [&] { return gsh::launch_leg_anal_f64(p, s); }));
+  if (precision == GSH_FP64 || precision == GSH_FP32) {
+    GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_ANALYSIS, s, [&] {
+      return gsh::launch_leg_anal_f64(p, s, precision == GSH_FP32);
+    }));
     return GSH_OK;
   }
   const int ns = precision == GSH_BF16X6 ? 3 : 2;
@@ -756,6 +752,61 @@ static int forward_impl(gsh_grid_t g, int lmax, int n, int value_kind, int preci
   GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_ANALYSIS, s,
                 [&] { return gsh::launch_leg_anal_bf16(p, ns, s); }));
   return GSH_OK;
+}
+
+}  // namespace
+
+int gsh_inverse(gsh_grid_t g, int lmax, int n, int value_kind, int precision, int storage,
+                int batch, const void* coeffs, long coeff_stride, void* field, long field_stride,
+                void* stream) {
+  int st = check_call(g, lmax, n, value_kind, precision, storage, batch);
+  if (st) return st;
+  if (batch == 0) return GSH_OK;
+  if (!coeffs || !field) return GSH_ERR_INVALID_ARGUMENT;
+  if (storage != GSH_STORE_FP64) return GSH_ERR_UNSUPPORTED;
+  const int real = value_kind == GSH_REAL;
+  const long csize = gsh_coefficient_size(lmax, n, value_kind);
+  const long fsize = static_cast<long>(g->ntheta()) * g->nphi();
+  if (batch > 1 && (coeff_stride < csize || field_stride < fsize)) return GSH_ERR_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> lock(g->mu);
+  GSH_HIP(hipSetDevice(g->device));
+  if (lmax == 0) {
+    GSH_HIP(gsh::launch_lmax0(1, real, batch, static_cast<const double*>(coeffs), coeff_stride,
+                              static_cast<double*>(field), field_stride, s));
+    return GSH_OK;
+  }
+  NData* d;
+  if ((st = ensure_ndata(g, n, &d))) return st;
+  return run_inverse(g, d, lmax, n, real, precision, batch, static_cast<const double*>(coeffs),
+                     coeff_stride, field, field_stride, s, RunOpts{});
+}
+
+static int forward_impl(gsh_grid_t g, int lmax, int n, int value_kind, int precision, int storage,
+                        int batch, const void* field, long field_stride, void* coeffs,
+                        long coeff_stride, int accumulate, void* stream, int adjoint) {
+  int st = check_call(g, lmax, n, value_kind, precision, storage, batch);
+  if (st) return st;
+  if (batch == 0) return GSH_OK;
+  if (!coeffs || !field) return GSH_ERR_INVALID_ARGUMENT;
+  if (storage != GSH_STORE_FP64) return GSH_ERR_UNSUPPORTED;
+  const int real = value_kind == GSH_REAL;
+  const long csize = gsh_coefficient_size(lmax, n, value_kind);
+  const long fsize = static_cast<long>(g->ntheta()) * g->nphi();
+  if (batch > 1 && (coeff_stride < csize || field_stride < fsize)) return GSH_ERR_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> lock(g->mu);
+  GSH_HIP(hipSetDevice(g->device));
+  if (lmax == 0) {
+    GSH_HIP(gsh::launch_lmax0(adjoint ? 2 : 0, real, batch, static_cast<const double*>(field),
+                              field_stride, static_cast<double*>(coeffs), coeff_stride, s));
+    return GSH_OK;
+  }
+  NData* d;
+  if ((st = ensure_ndata(g, n, &d))) return st;
+  return run_forward(g, d, lmax, n, real, precision, batch, field, field_stride,
+                     static_cast<double*>(coeffs), coeff_stride, accumulate, adjoint, s,
+                     RunOpts{});
 }
 
 int gsh_forward(gsh_grid_t g, int lmax, int n, int value_kind, int precision, int storage,
@@ -865,6 +916,339 @@ int gsh_forward_host(gsh_grid_t g, int lmax, int n, int value_kind, const double
                    accumulate, nullptr);
   if (st) return st;
   GSH_HIP(hipMemcpy(coeffs, dco, cbytes, hipMemcpyDeviceToHost));
+  return GSH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batch plans (gsh_plan_*): many fields of uneven truncation degree and upper
+// index.  Fields are grouped by upper index; within a group they are ordered
+// by descending degree, so each 16-column group of a launch has a degree
+// bound close to its fields' own (cg_lmax) and every column masks its own
+// degrees (f_lmax).  The first execution in each direction times the
+// candidate tile shapes and keeps the fastest; graph executions capture the
+// whole launch sequence once per pointer set and replay it.
+// ---------------------------------------------------------------------------
+struct PlanGroup {
+  int n = 0, lmax = 0, count = 0;
+  std::vector<int> fmap, flmax;
+  int* d_fmap = nullptr;
+  int* d_flmax = nullptr;
+  int* d_cglmax = nullptr;
+  NData* nd = nullptr;
+  int synth_tiles = 0;  // FP64 synthesis column tiles (0: not tuned)
+  int fft_inv = 0;      // 1: wavefront FFT, 2: LDS FFT (0: not tuned)
+  int fft_fwd = 0;
+};
+
+struct GraphKey {
+  int dir;
+  const void* in;
+  long istride;
+  const void* out;
+  long ostride;
+  bool operator<(const GraphKey& o) const {
+    return std::tie(dir, in, istride, out, ostride) <
+           std::tie(o.dir, o.in, o.istride, o.out, o.ostride);
+  }
+};
+
+struct gsh_plan_s {
+  gsh_grid_s* g = nullptr;
+  int count = 0, kind = 0, precision = 0, storage = 0;
+  std::vector<int> lmax, n;
+  long cmax = 0;  // largest coefficient count of a field
+  std::vector<PlanGroup> groups;
+  bool tuned[2] = {false, false};
+  double* stage = nullptr;  // FP64 coefficients of FP32-storage plans
+  size_t stage_bytes = 0;
+  long* d_csize = nullptr;  // coefficient count per field
+  hipStream_t cap = nullptr;
+  std::map<GraphKey, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+};
+
+namespace {
+
+int plan_stage(gsh_plan_s* P, long cstride) {
+  const size_t bytes = 16 * (size_t)P->count * cstride;
+  if (bytes <= P->stage_bytes) return GSH_OK;
+  if (P->stage) {
+    GSH_HIP(hipDeviceSynchronize());
+    GSH_HIP(hipFree(P->stage));
+  }
+  P->stage = nullptr;
+  P->stage_bytes = 0;
+  if (hipMalloc(&P->stage, bytes) != hipSuccess) return GSH_ERR_OUT_OF_MEMORY;
+  P->stage_bytes = bytes;
+  return GSH_OK;
+}
+
+RunOpts group_opts(const gsh_plan_s* P, const PlanGroup& G, int dir) {
+  RunOpts o;
+  o.fmap = G.d_fmap;
+  o.f_lmax = G.d_flmax;
+  o.cg_lmax = G.d_cglmax;
+  o.f32 = P->storage == GSH_STORE_FP32;
+  o.synth_tiles = G.synth_tiles;
+  o.fft = dir == 0 ? G.fft_inv : G.fft_fwd;
+  return o;
+}
+
+// One group in one direction (dir 0 synthesis, 1 analysis).
+int run_group(gsh_plan_s* P, const PlanGroup& G, int dir, const double* cin, double* cout,
+              long cstride, const void* fin, void* fout, long fstride, hipStream_t s,
+              const RunOpts& o) {
+  const int real = P->kind == GSH_REAL;
+  if (dir == 0)
+    return run_inverse(P->g, G.nd, G.lmax, G.n, real, P->precision, G.count, cin, cstride, fout,
+                       fstride, s, o);
+  return run_forward(P->g, G.nd, G.lmax, G.n, real, P->precision, G.count, fin, fstride, cout,
+                     cstride, 0, 0, s, o);
+}
+
+// The whole plan in one direction on stream s.
+int run_plan(gsh_plan_s* P, int dir, const void* in, long istride, void* out, long ostride,
+             hipStream_t s) {
+  int st;
+  const bool f32 = P->storage == GSH_STORE_FP32;
+  if (dir == 0) {
+    const double* cin = static_cast<const double*>(in);
+    if (f32) {
+      if ((st = plan_stage(P, istride))) return st;
+      GSH_HIP(gsh::launch_widen_c32(in, P->stage, (long)P->count * istride, s));
+      cin = P->stage;
+    }
+    for (const PlanGroup& G : P->groups)
+      if ((st = run_group(P, G, 0, cin, nullptr, istride, nullptr, out, ostride, s,
+                          group_opts(P, G, 0))))
+        return st;
+    return GSH_OK;
+  }
+  double* cout = static_cast<double*>(out);
+  if (f32) {
+    if ((st = plan_stage(P, ostride))) return st;
+    cout = P->stage;
+  }
+  for (const PlanGroup& G : P->groups)
+    if ((st = run_group(P, G, 1, nullptr, cout, ostride, in, nullptr, istride, s,
+                        group_opts(P, G, 1))))
+      return st;
+  if (f32)
+    GSH_HIP(gsh::launch_narrow_c64(P->stage, out, ostride, P->d_csize, P->cmax, P->count, s));
+  return GSH_OK;
+}
+
+// Autotuner: per group, run each candidate (FP64 synthesis tiles x FFT family)
+// once to warm up and once timed with events, and keep the fastest.
+int tune_plan(gsh_plan_s* P, int dir, const void* in, long istride, void* out, long ostride,
+              hipStream_t s) {
+  int st;
+  const bool f32 = P->storage == GSH_STORE_FP32;
+  const double* cin = static_cast<const double*>(in);
+  double* cout = static_cast<double*>(out);
+  if (f32) {
+    if ((st = plan_stage(P, dir == 0 ? istride : ostride))) return st;
+    if (dir == 0) GSH_HIP(gsh::launch_widen_c32(in, P->stage, (long)P->count * istride, s));
+    cin = P->stage;
+    cout = P->stage;
+  }
+  hipEvent_t e0, e1;
+  GSH_HIP(hipEventCreate(&e0));
+  GSH_HIP(hipEventCreate(&e1));
+  const bool was_timing = P->g->timing;
+  P->g->timing = false;
+  for (PlanGroup& G : P->groups) {
+    std::vector<std::pair<int, int>> cands;  // (tiles, fft)
+    const bool wave_ok = P->g->d_twa != nullptr;
+    const bool tiles2 =
+        dir == 0 && (P->precision == GSH_FP64 || P->precision == GSH_FP32) && G.count > 8;
+    for (int t : tiles2 ? std::vector<int>{1, 2} : std::vector<int>{0})
+      for (int f : wave_ok ? std::vector<int>{1, 2} : std::vector<int>{2}) cands.emplace_back(t, f);
+    float best = 1e30f;
+    std::pair<int, int> pick = cands[0];
+    for (const auto& c : cands) {
+      RunOpts o = group_opts(P, G, dir);
+      o.synth_tiles = c.first;
+      o.fft = c.second;
+      for (int rep = 0; rep < 2; rep++) {
+        if (rep == 1) GSH_HIP(hipEventRecord(e0, s));
+        if ((st = run_group(P, G, dir, cin, cout, dir == 0 ? istride : ostride, in, out,
+                            dir == 0 ? ostride : istride, s, o))) {
+          P->g->timing = was_timing;
+          return st;
+        }
+      }
+      GSH_HIP(hipEventRecord(e1, s));
+      GSH_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      GSH_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        pick = c;
+      }
+    }
+    if (dir == 0) {
+      G.synth_tiles = pick.first;
+      G.fft_inv = pick.second;
+    } else {
+      G.fft_fwd = pick.second;
+    }
+  }
+  P->g->timing = was_timing;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  P->tuned[dir] = true;
+  return GSH_OK;
+}
+
+int plan_exec(gsh_plan_s* P, int dir, const void* in, long istride, void* out, long ostride,
+              int graph, void* stream) {
+  if (!P || !in || !out) return GSH_ERR_INVALID_ARGUMENT;
+  gsh_grid_s* g = P->g;
+  const long fsize = (long)g->ntheta() * g->nphi();
+  const long cs = dir == 0 ? istride : ostride;
+  const long fs = dir == 0 ? ostride : istride;
+  if (cs < P->cmax || fs < fsize) return GSH_ERR_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> lock(g->mu);
+  GSH_HIP(hipSetDevice(g->device));
+  int st;
+  const GraphKey key{dir, in, istride, out, ostride};
+  if (graph) {
+    auto it = P->graphs.find(key);
+    if (it != P->graphs.end()) {
+      GSH_HIP(hipGraphLaunch(it->second.second, s));
+      return GSH_OK;
+    }
+  }
+  if (!P->tuned[dir] && (st = tune_plan(P, dir, in, istride, out, ostride, s))) return st;
+  // Eager run: the result, and every work buffer sized before any capture.
+  if ((st = run_plan(P, dir, in, istride, out, ostride, s))) return st;
+  if (!graph) return GSH_OK;
+  if (!P->cap) GSH_HIP(hipStreamCreateWithFlags(&P->cap, hipStreamNonBlocking));
+  GSH_HIP(hipStreamSynchronize(s));
+  const bool was_timing = g->timing;
+  g->timing = false;
+  hipGraph_t gr = nullptr;
+  GSH_HIP(hipStreamBeginCapture(P->cap, hipStreamCaptureModeThreadLocal));
+  st = run_plan(P, dir, in, istride, out, ostride, P->cap);
+  const hipError_t ec = hipStreamEndCapture(P->cap, &gr);
+  g->timing = was_timing;
+  if (st) return st;
+  if (ec != hipSuccess) return GSH_ERR_DEVICE;
+  hipGraphExec_t ex = nullptr;
+  if (hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(gr);
+    return GSH_ERR_DEVICE;
+  }
+  P->graphs.emplace(key, std::make_pair(gr, ex));
+  return GSH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gsh_plan_create(gsh_grid_t g, int count, const int* lmax, const int* n, int value_kind,
+                    int precision, int storage, gsh_plan_t* plan) {
+  if (!g || !plan || count <= 0 || !lmax || !n) return GSH_ERR_INVALID_ARGUMENT;
+  *plan = nullptr;
+  int st;
+  for (int i = 0; i < count; i++) {
+    if (lmax[i] < 1) return GSH_ERR_DEGREE;
+    if ((st = check_call(g, lmax[i], n[i], value_kind, precision, storage, 1))) return st;
+  }
+  auto P = std::make_unique<gsh_plan_s>();
+  P->g = g;
+  P->count = count;
+  P->kind = value_kind;
+  P->precision = precision;
+  P->storage = storage;
+  P->lmax.assign(lmax, lmax + count);
+  P->n.assign(n, n + count);
+  std::vector<long> csize(count);
+  for (int i = 0; i < count; i++) {
+    csize[i] = gsh_coefficient_size(lmax[i], n[i], value_kind);
+    P->cmax = std::max(P->cmax, csize[i]);
+  }
+  std::map<int, std::vector<int>> byn;
+  for (int i = 0; i < count; i++) byn[n[i]].push_back(i);
+  std::lock_guard<std::mutex> lock(g->mu);
+  GSH_HIP(hipSetDevice(g->device));
+  for (auto& kv : byn) {
+    PlanGroup G;
+    G.n = kv.first;
+    G.fmap = kv.second;
+    std::stable_sort(G.fmap.begin(), G.fmap.end(),
+                     [&](int a, int b) { return lmax[a] > lmax[b]; });
+    G.count = static_cast<int>(G.fmap.size());
+    for (int f : G.fmap) G.flmax.push_back(lmax[f]);
+    G.lmax = G.flmax[0];
+    const int ncg = (2 * G.count + 15) / 16;
+    std::vector<int> cg(ncg);
+    for (int c = 0; c < ncg; c++) cg[c] = G.flmax[8 * c];  // descending: first is largest
+    if ((st = ensure_ndata(g, G.n, &G.nd))) return st;
+    GSH_HIP(hipMalloc(&G.d_fmap, G.count * sizeof(int)));
+    GSH_HIP(hipMalloc(&G.d_flmax, G.count * sizeof(int)));
+    GSH_HIP(hipMalloc(&G.d_cglmax, ncg * sizeof(int)));
+    GSH_HIP(hipMemcpy(G.d_fmap, G.fmap.data(), G.count * sizeof(int), hipMemcpyHostToDevice));
+    GSH_HIP(hipMemcpy(G.d_flmax, G.flmax.data(), G.count * sizeof(int), hipMemcpyHostToDevice));
+    GSH_HIP(hipMemcpy(G.d_cglmax, cg.data(), ncg * sizeof(int), hipMemcpyHostToDevice));
+    P->groups.push_back(std::move(G));
+  }
+  GSH_HIP(hipMalloc(&P->d_csize, count * sizeof(long)));
+  GSH_HIP(hipMemcpy(P->d_csize, csize.data(), count * sizeof(long), hipMemcpyHostToDevice));
+  *plan = P.release();
+  return GSH_OK;
+}
+
+int gsh_plan_destroy(gsh_plan_t P) {
+  if (!P) return GSH_ERR_INVALID_ARGUMENT;
+  {
+    std::lock_guard<std::mutex> lock(P->g->mu);
+    (void)hipSetDevice(P->g->device);
+    (void)hipDeviceSynchronize();
+    for (auto& kv : P->graphs) {
+      (void)hipGraphExecDestroy(kv.second.second);
+      (void)hipGraphDestroy(kv.second.first);
+    }
+    for (PlanGroup& G : P->groups) {
+      (void)hipFree(G.d_fmap);
+      (void)hipFree(G.d_flmax);
+      (void)hipFree(G.d_cglmax);
+    }
+    if (P->stage) (void)hipFree(P->stage);
+    (void)hipFree(P->d_csize);
+    if (P->cap) (void)hipStreamDestroy(P->cap);
+  }
+  delete P;
+  return GSH_OK;
+}
+
+int gsh_plan_inverse(gsh_plan_t P, const void* coeffs, long coeff_stride, void* field,
+                     long field_stride, int graph, void* stream) {
+  return plan_exec(P, 0, coeffs, coeff_stride, field, field_stride, graph, stream);
+}
+
+int gsh_plan_forward(gsh_plan_t P, const void* field, long field_stride, void* coeffs,
+                     long coeff_stride, int graph, void* stream) {
+  return plan_exec(P, 1, field, field_stride, coeffs, coeff_stride, graph, stream);
+}
+
+int gsh_plan_info(gsh_plan_t P, int* ngroups, int cap, int* n, int* lmax, int* count,
+                  int* synth_tiles, int* fft_inverse, int* fft_forward) {
+  if (!P || !ngroups) return GSH_ERR_INVALID_ARGUMENT;
+  *ngroups = static_cast<int>(P->groups.size());
+  for (int i = 0; i < cap && i < *ngroups; i++) {
+    const PlanGroup& G = P->groups[i];
+    if (n) n[i] = G.n;
+    if (lmax) lmax[i] = G.lmax;
+    if (count) count[i] = G.count;
+    if (synth_tiles) synth_tiles[i] = G.synth_tiles;
+    if (fft_inverse) fft_inverse[i] = G.fft_inv;
+    if (fft_forward) fft_forward[i] = G.fft_fwd;
+  }
   return GSH_OK;
 }
 

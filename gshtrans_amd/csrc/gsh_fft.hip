@@ -202,11 +202,16 @@ __global__ __launch_bounds__(kFFTThreads) void fft_synth(FFTParams p) {
     const int j = idx - q * N;
     const int f = fg * R + q;
     if (f >= p.nfield) continue;
-    const long o = (long)f * p.fstride + (long)theta * N + j;
-    if (p.real)
-      static_cast<double*>(p.fout)[o] = buf[q * M + j].x;
+    const long o = (long)(p.fmap ? p.fmap[f] : f) * p.fstride + (long)theta * N + j;
+    const double2 x = buf[q * M + j];
+    if (p.real && p.f32)
+      static_cast<float*>(p.fout)[o] = (float)x.x;
+    else if (p.real)
+      static_cast<double*>(p.fout)[o] = x.x;
+    else if (p.f32)
+      static_cast<float2*>(p.fout)[o] = make_float2((float)x.x, (float)x.y);
     else
-      static_cast<double2*>(p.fout)[o] = buf[q * M + j];
+      static_cast<double2*>(p.fout)[o] = x;
   }
 }
 
@@ -222,11 +227,17 @@ __global__ __launch_bounds__(kFFTThreads) void fft_anal(FFTParams p) {
     const int f = fg * R + q;
     double2 x = make_double2(0.0, 0.0);
     if (f < p.nfield && j < N) {
-      const long o = (long)f * p.fstride + (long)theta * N + j;
-      if (p.real)
+      const long o = (long)(p.fmap ? p.fmap[f] : f) * p.fstride + (long)theta * N + j;
+      if (p.real && p.f32) {
+        x.x = static_cast<const float*>(p.fin)[o];
+      } else if (p.real) {
         x.x = static_cast<const double*>(p.fin)[o];
-      else
+      } else if (p.f32) {
+        const float2 y = static_cast<const float2*>(p.fin)[o];
+        x = make_double2(y.x, y.y);
+      } else {
         x = static_cast<const double2*>(p.fin)[o];
+      }
     }
     buf[idx] = x;
   }

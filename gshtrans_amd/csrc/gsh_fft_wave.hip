@@ -189,6 +189,35 @@ __device__ __forceinline__ void wave_fft(double2 (&v)[R], double2* lds,
     for (int d = 0; d < 8; d++) v[cq + G * d] = o[cq][bitrev(d, 3)];
 }
 
+// Field element access for FP64 or FP32 storage (index in elements).
+template <int F32>
+__device__ __forceinline__ double2 ld_cplx(const void* b, long i) {
+  if (F32) {
+    const float2 v = static_cast<const float2*>(b)[i];
+    return make_double2(v.x, v.y);
+  }
+  return static_cast<const double2*>(b)[i];
+}
+template <int F32>
+__device__ __forceinline__ double ld_real(const void* b, long i) {
+  if (F32) return static_cast<const float*>(b)[i];
+  return static_cast<const double*>(b)[i];
+}
+template <int F32>
+__device__ __forceinline__ void st_cplx(void* b, long i, double2 v) {
+  if (F32)
+    static_cast<float2*>(b)[i] = make_float2((float)v.x, (float)v.y);
+  else
+    static_cast<double2*>(b)[i] = v;
+}
+template <int F32>
+__device__ __forceinline__ void st_real(void* b, long i, double v) {
+  if (F32)
+    static_cast<float*>(b)[i] = (float)v;
+  else
+    static_cast<double*>(b)[i] = v;
+}
+
 __device__ __forceinline__ double2 ld2(const double* p) {
   return *reinterpret_cast<const double2*>(p);
 }
@@ -197,7 +226,7 @@ __device__ __forceinline__ void st2(double* p, double2 v) {
 }
 
 // Synthesis: order bins of F[theta][slot][col] -> backward DFT -> field ring.
-template <int R, int REAL>
+template <int R, int REAL, int F32>
 __global__ __launch_bounds__(512) void fft_synth_w(FFTParams p) {
   __shared__ double2 lds_all[kWaves * kLdsPerWave];
   constexpr int N = 64 * R;
@@ -237,18 +266,17 @@ __global__ __launch_bounds__(512) void fft_synth_w(FFTParams p) {
     v[n1] = X;
   }
   wave_fft<R, +1>(v, lds_all + wave * kLdsPerWave, p.twa, p.tw, lane);
+  const long oa = (long)field_index(p.fmap, f0) * p.fstride + (long)theta * N + lane;
   if (!REAL) {
-    double2* out = static_cast<double2*>(p.fout) + (long)f0 * p.fstride + (long)theta * N + lane;
 #pragma unroll
-    for (int t = 0; t < R; t++) out[64 * t] = v[t];
+    for (int t = 0; t < R; t++) st_cplx<F32>(p.fout, oa + 64 * t, v[t]);
   } else {
-    double* oa = static_cast<double*>(p.fout) + (long)f0 * p.fstride + (long)theta * N + lane;
 #pragma unroll
-    for (int t = 0; t < R; t++) oa[64 * t] = v[t].x;
+    for (int t = 0; t < R; t++) st_real<F32>(p.fout, oa + 64 * t, v[t].x);
     if (f1v) {
-      double* ob = oa + p.fstride;
+      const long ob = (long)field_index(p.fmap, f0 + 1) * p.fstride + (long)theta * N + lane;
 #pragma unroll
-      for (int t = 0; t < R; t++) ob[64 * t] = v[t].y;
+      for (int t = 0; t < R; t++) st_real<F32>(p.fout, ob + 64 * t, v[t].y);
     }
   }
 }
@@ -256,7 +284,7 @@ __global__ __launch_bounds__(512) void fft_synth_w(FFTParams p) {
 // Analysis: field ring -> forward DFT -> weighted bins into the analysis
 // layout FA (fa_index); the last ring also zeroes the padding colatitudes
 // of the last quad.
-template <int R, int REAL>
+template <int R, int REAL, int F32>
 __global__ __launch_bounds__(512) void fft_anal_w(FFTParams p) {
   __shared__ double2 lds_all[kWaves * kLdsPerWave];
   constexpr int N = 64 * R;
@@ -269,19 +297,17 @@ __global__ __launch_bounds__(512) void fft_anal_w(FFTParams p) {
   const bool f1v = REAL && f0 + 1 < p.nfield;
   const int lmax = p.lmax;
   double2 v[R];
+  const long ia = (long)field_index(p.fmap, f0) * p.fstride + (long)theta * N + lane;
   if (!REAL) {
-    const double2* in =
-        static_cast<const double2*>(p.fin) + (long)f0 * p.fstride + (long)theta * N + lane;
 #pragma unroll
-    for (int n1 = 0; n1 < R; n1++) v[n1] = in[64 * n1];
+    for (int n1 = 0; n1 < R; n1++) v[n1] = ld_cplx<F32>(p.fin, ia + 64 * n1);
   } else {
-    const double* ia =
-        static_cast<const double*>(p.fin) + (long)f0 * p.fstride + (long)theta * N + lane;
-    const double* ib = ia + (f1v ? p.fstride : 0);
+    const long ib =
+        f1v ? (long)field_index(p.fmap, f0 + 1) * p.fstride + (long)theta * N + lane : ia;
 #pragma unroll
     for (int n1 = 0; n1 < R; n1++) {
-      const double xa = ia[64 * n1];
-      const double xb = ib[64 * n1];
+      const double xa = ld_real<F32>(p.fin, ia + 64 * n1);
+      const double xb = ld_real<F32>(p.fin, ib + 64 * n1);
       v[n1] = make_double2(xa, f1v ? xb : 0.0);
     }
   }
@@ -340,20 +366,28 @@ __global__ __launch_bounds__(512) void fft_anal_w(FFTParams p) {
 template <int R>
 hipError_t launch_synth_r(const FFTParams& p, hipStream_t s) {
   const dim3 grid(p.nfg, p.ntheta), block(64 * kWaves);
-  if (p.real)
-    hipLaunchKernelGGL((fft_synth_w<R, 1>), grid, block, 0, s, p);
+  if (p.real && p.f32)
+    hipLaunchKernelGGL((fft_synth_w<R, 1, 1>), grid, block, 0, s, p);
+  else if (p.real)
+    hipLaunchKernelGGL((fft_synth_w<R, 1, 0>), grid, block, 0, s, p);
+  else if (p.f32)
+    hipLaunchKernelGGL((fft_synth_w<R, 0, 1>), grid, block, 0, s, p);
   else
-    hipLaunchKernelGGL((fft_synth_w<R, 0>), grid, block, 0, s, p);
+    hipLaunchKernelGGL((fft_synth_w<R, 0, 0>), grid, block, 0, s, p);
   return hipGetLastError();
 }
 
 template <int R>
 hipError_t launch_anal_r(const FFTParams& p, hipStream_t s) {
   const dim3 grid(p.nfg, p.ntheta), block(64 * kWaves);
-  if (p.real)
-    hipLaunchKernelGGL((fft_anal_w<R, 1>), grid, block, 0, s, p);
+  if (p.real && p.f32)
+    hipLaunchKernelGGL((fft_anal_w<R, 1, 1>), grid, block, 0, s, p);
+  else if (p.real)
+    hipLaunchKernelGGL((fft_anal_w<R, 1, 0>), grid, block, 0, s, p);
+  else if (p.f32)
+    hipLaunchKernelGGL((fft_anal_w<R, 0, 1>), grid, block, 0, s, p);
   else
-    hipLaunchKernelGGL((fft_anal_w<R, 0>), grid, block, 0, s, p);
+    hipLaunchKernelGGL((fft_anal_w<R, 0, 0>), grid, block, 0, s, p);
   return hipGetLastError();
 }
 

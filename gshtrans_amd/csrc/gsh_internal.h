@@ -50,7 +50,8 @@ struct LegParams {
   const double4* rec;
   const long* roff;
   int accumulate;       // analysis: coefficients += result
-  int zero_last;        // analysis: zero the (L, L) coefficient (complex, lmax == L)
+  int zero_last;        // analysis: zero the (L, L) coefficient of complex fields
+                        // whose truncation degree is L
   int n_tsb;            // synthesis: theta super-blocks of 256
   int n_cg;             // column groups
   int nquad;            // analysis: colatitude quads, ceil(ntheta / 4)
@@ -64,7 +65,20 @@ struct LegParams {
   const long* cso;      // per slot: first chunk of that order in bs
   int ntile;            // 16-column tiles, ceil(ncol / 16)
   int nkc;              // analysis: 32-colatitude chunks, ceil(ntheta / 32)
+  // Batches of uneven truncation (null pointers: every field has lmax and
+  // column field f is the caller's field f):
+  const int* fmap;      // column field f -> index of the field in the caller's arrays
+  const int* f_lmax;    // truncation degree of column field f (<= lmax)
+  const int* cg_lmax;   // max of f_lmax over each 16-column group
 };
+
+// Truncation degree and caller index of column field f.
+__device__ __forceinline__ int field_lmax(const LegParams& p, int f) {
+  return p.f_lmax ? p.f_lmax[f] : p.lmax;
+}
+__device__ __forceinline__ int field_index(const int* fmap, int f) {
+  return fmap ? fmap[f] : f;
+}
 
 // Load of a wave-uniform table entry through the constant address space, so
 // that it becomes a scalar load into an SGPR (the tables are written only by
@@ -102,6 +116,8 @@ struct FFTParams {
   const void* fin;    // analysis input field
   void* fout;         // synthesis output field
   long fstride;       // elements between fields
+  int f32;            // fields stored in FP32 (float / complex float)
+  const int* fmap;    // column field f -> index of the field in the caller's arrays
 };
 
 // bf16 index of (slot, col, theta, plane s) in the split analysis layout:
@@ -113,8 +129,12 @@ __host__ __device__ inline long xs_index(int slot, int col, int theta, int s, in
   return (blk * 64 + 16 * ((theta & 31) >> 3) + (col & 15)) * 8 + (theta & 7);
 }
 
-hipError_t launch_leg_synth_f64(const LegParams& p, hipStream_t s);
-hipError_t launch_leg_anal_f64(const LegParams& p, hipStream_t s);
+// tiles: 16-column tiles per workgroup (1 or 2; 0 picks 2 when ncol > 16);
+// f32m: FP32 MFMA on FP32-rounded operands with FP32 accumulation (the
+// recurrence and weights stay FP64).
+hipError_t launch_leg_synth_f64(const LegParams& p, hipStream_t s, int tiles = 0,
+                                bool f32m = false);
+hipError_t launch_leg_anal_f64(const LegParams& p, hipStream_t s, bool f32m = false);
 hipError_t launch_fft_synth(const FFTParams& p, hipStream_t s);
 hipError_t launch_fft_anal(const FFTParams& p, hipStream_t s);
 // Wavefront-per-ring FFTs (gsh_fft_wave.hip) for N in {512, 1024, 2048}.
@@ -127,6 +147,12 @@ hipError_t launch_leg_synth_bf16(const LegParams& p, int ns, hipStream_t s);
 hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s);
 // FA (FP64 analysis bins) -> split bf16 analysis operands (xs_index layout).
 hipError_t launch_bin_split(const LegParams& p, int ns, long nblocks, hipStream_t s);
+
+// FP32 <-> FP64 coefficient staging of batch plans (gsh_batch.hip); strides
+// and sizes in complex elements.
+hipError_t launch_widen_c32(const void* in, double* out, long total, hipStream_t s);
+hipError_t launch_narrow_c64(const double* in, void* out, long stride, const long* csize,
+                             long cmax, int count, hipStream_t s);
 
 // lmax == 0: mode 1 synthesis, 0 analysis, 2 adjoint of the synthesis.
 hipError_t launch_lmax0(int mode, int real, int nfield, const double* in, long istride,

@@ -32,6 +32,32 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // (s_load) loads even when the kernel stores to global memory in the loop.
 typedef const __attribute__((address_space(4))) d4 rec_t;
 
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// The contraction's MFMA and accumulator: FP64 (v_mfma_f64_16x16x4f64; the
+// lane holding column lane%16 holds C rows lane/16 + 4r) or, for the FP32
+// precision mode, v_mfma_f32_16x16x4f32 on FP32-rounded operands with FP32
+// accumulation (C rows 4 (lane/16) + r).  A operand: row lane%16, k
+// lane/16; B operand: k lane/16, column lane%16 (both instructions).
+template <bool F32M>
+struct Mma {
+  typedef d4 acc_t;
+  static __device__ __forceinline__ acc_t zero() { return d4{0.0, 0.0, 0.0, 0.0}; }
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int q, int r) { return q + 4 * r; }
+};
+template <>
+struct Mma<true> {
+  typedef f4 acc_t;
+  static __device__ __forceinline__ acc_t zero() { return f4{0.f, 0.f, 0.f, 0.f}; }
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32((float)a, (float)b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int q, int r) { return 4 * q + r; }
+};
+
 __device__ __forceinline__ rec_t* rec_ptr(const double4* rec, long off) {
   return (rec_t*)(rec) + off;
 }
@@ -72,8 +98,9 @@ __device__ __forceinline__ void rec_step(const d4 q, double cs, double& e0, doub
 // operands, and 4*T MFMAs accumulate F[theta][col].  Coefficients of the
 // next 4 degrees are loaded one chunk ahead.
 // ---------------------------------------------------------------------------
-template <int T, int REAL>
+template <int T, int REAL, bool F32M>
 __global__ __launch_bounds__(256) void leg_synth_f64(LegParams p) {
+  typedef Mma<F32M> M;
   __shared__ double tr[4][4][66];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
@@ -92,37 +119,59 @@ __global__ __launch_bounds__(256) void leg_synth_f64(LegParams p) {
   const double cs = tv ? cs0 : 0.0;
   double e0 = tv ? s0 : 0.0;
   double e1 = 0.0;
-  rec_t* rc = rec_ptr(p.rec, ld_uniform(p.roff, m + p.L) - l0);  // rc[l]: step to degree l
   const int krow = lane >> 4;
+  // Recurrence coefficients of one 8-degree iteration (entries l+1 .. l+8,
+  // 256 bytes) are staged through a per-wave LDS slot: lanes 0..15 fetch the
+  // next iteration's 16 bytes each one iteration ahead, so the steps never
+  // wait on scalar loads.  Entries up to L + kRecPad exist.
+  __shared__ uint4 rsw[4][16];
+  const uint4* recq = reinterpret_cast<const uint4*>(p.rec + (ld_uniform(p.roff, m + p.L) - l0));
+  auto fetch = [&](int l) { return lane < 16 ? recq[2 * (l + 1) + lane] : make_uint4(0, 0, 0, 0); };
+  const d4* rl = reinterpret_cast<const d4*>(&rsw[wave][0]);  // rl[j]: entry l + 1 + j
 
   // B-operand column of this lane per column tile.  Every load reads a valid
   // address (column and degree clamped); out-of-range values are zeroed by a
   // select when the operand is used, so loads stay in flight across a chunk.
+  // Each column's own truncation degree lf masks its degrees; a column with
+  // lf < l0 (no degree of this order) reads element 0 and is masked out.
   const double* cptr[T];
   bool cv[T];
+  int lf[T];
 #pragma unroll
   for (int j = 0; j < T; j++) {
     const int col = cg * 16 * T + 16 * j + (lane & 15);
     cv[j] = col < p.ncol;
     const int cc = cv[j] ? col : 0;
-    cptr[j] = p.cin + (long)(cc >> 1) * p.cstride * 2 + (cc & 1);
+    lf[j] = field_lmax(p, cc >> 1);
+    cptr[j] = p.cin + (long)field_index(p.fmap, cc >> 1) * p.cstride * 2 + (cc & 1);
+  }
+  // Degree loop bound of the workgroup: the largest lf of its column groups.
+  int lwg = lmax;
+  if (p.cg_lmax) {
+    lwg = 0;
+#pragma unroll
+    for (int j = 0; j < T; j++)
+      if ((cg * T + j) * 16 < p.ncol) lwg = max(lwg, ld_uniform(p.cg_lmax, cg * T + j));
   }
   auto loadB = [&](int l, double* out) {
-    const int lk = min(l + krow, lmax);
-    const long ci = 2 * coef_index(lk, m, p.nabs, REAL);
 #pragma unroll
-    for (int j = 0; j < T; j++) out[j] = cptr[j][ci];
+    for (int j = 0; j < T; j++) {
+      const int lk = min(l + krow, lf[j]);
+      const long ci = lf[j] >= l0 ? 2 * coef_index(lk, m, p.nabs, REAL) : 0;
+      out[j] = cptr[j][ci];
+    }
   };
 
-  d4 acc[4][T];
+  typename M::acc_t acc[4][T];
 #pragma unroll
   for (int g = 0; g < 4; g++)
 #pragma unroll
-    for (int j = 0; j < T; j++) acc[g][j] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < T; j++) acc[g][j] = M::zero();
 
-  // One 4-degree chunk: recurrence, LDS transpose, B mask, 4*T MFMAs.
-  auto chunk = [&](int l, double* B) {
-    const d4 q1 = rc[l + 1], q2 = rc[l + 2], q3 = rc[l + 3], q4 = rc[l + 4];
+  // One 4-degree chunk: recurrence, LDS transpose, B mask, 4*T MFMAs; j0 is
+  // the chunk's first entry in the staged coefficients.
+  auto chunk = [&](int l, int j0, double* B) {
+    const d4 q1 = rl[j0], q2 = rl[j0 + 1], q3 = rl[j0 + 2], q4 = rl[j0 + 3];
     tr[wave][0][lane] = e0;
     rec_step(q1, cs, e0, e1);
     tr[wave][1][lane] = e0;
@@ -136,13 +185,14 @@ __global__ __launch_bounds__(256) void leg_synth_f64(LegParams p) {
 #pragma unroll
     for (int g = 0; g < 4; g++) A[g] = tr[wave][krow][g * 16 + (lane & 15)];
     wave_sync();
-    const bool lv = l + krow <= lmax;
+    bool lv[T];
+#pragma unroll
+    for (int j = 0; j < T; j++) lv[j] = cv[j] && l + krow <= lf[j];
 #pragma unroll
     for (int g = 0; g < 4; g++)
 #pragma unroll
       for (int j = 0; j < T; j++)
-        acc[g][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[g], (lv && cv[j]) ? B[j] : 0.0,
-                                                         acc[g][j], 0, 0, 0);
+        acc[g][j] = M::mma(A[g], lv[j] ? B[j] : 0.0, acc[g][j]);
   };
 
   // Chunks are processed in pairs with ping-pong operand buffers, so each
@@ -150,14 +200,18 @@ __global__ __launch_bounds__(256) void leg_synth_f64(LegParams p) {
   // paired with an all-zero one.
   double B0[T], B1[T];
   loadB(l0, B0);
-  for (int l = l0; l <= lmax; l += 8) {
+  uint4 rq = fetch(l0);
+  for (int l = l0; l <= lwg; l += 8) {
+    if (lane < 16) rsw[wave][lane] = rq;
+    wave_sync();
+    rq = fetch(l + 8);
     loadB(l + 4, B1);
-    chunk(l, B0);
+    chunk(l, 0, B0);
     loadB(l + 8, B0);
-    chunk(l + 4, B1);
+    chunk(l + 4, 4, B1);
   }
 
-  // C layout of the 16x16x4 f64 MFMA: lane holds rows lane/16 + 4r, column lane%16.
+  // C layout: the lane holds column lane%16, rows M::row(lane/16, r).
 #pragma unroll
   for (int g = 0; g < 4; g++)
 #pragma unroll
@@ -166,7 +220,7 @@ __global__ __launch_bounds__(256) void leg_synth_f64(LegParams p) {
       if (col >= p.ncol) continue;
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int row = tsb * 256 + wave * 64 + g * 16 + krow + 4 * r;
+        const int row = tsb * 256 + wave * 64 + g * 16 + M::row(krow, r);
         if (row < p.ntheta) p.F[((long)row * p.nslot + slot) * p.ncol + col] = acc[g][j][r];
       }
     }
@@ -189,8 +243,9 @@ __global__ __launch_bounds__(256) void leg_synth_f64(LegParams p) {
 // ---------------------------------------------------------------------------
 constexpr int kAnalLT = 4;  // 16-degree tiles per chunk
 
-template <int MB>
+template <int MB, bool F32M>
 __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
+  typedef Mma<F32M> M;
   __shared__ double tr[4][16][66];  // also holds the cross-wave reduction
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
@@ -199,11 +254,23 @@ __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
   const int m = order_of(mw, p.real);
   const int l0 = max(abs(m), p.nabs);
   const int lmax = p.lmax;
-  rec_t* rc = rec_ptr(p.rec, ld_uniform(p.roff, m + p.L) - l0);
   const int slot = m - p.m_first;
   const int krow = lane >> 4;
   const int col = cg * 16 + (lane & 15);
+  // Recurrence coefficients of a 64-degree pass (entries lc+1 .. lc+64,
+  // 2 KiB) are staged in LDS, double-buffered by pass parity: threads
+  // 0..127 fetch the next pass's 16 bytes each at the start of a pass and
+  // store them before the pass's last barrier.  Entries up to L + kRecPad
+  // exist.
+  __shared__ uint4 rsa[2][128];
+  const uint4* recq = reinterpret_cast<const uint4*>(p.rec + (ld_uniform(p.roff, m + p.L) - l0));
+  const int tid = threadIdx.x;
+  if (tid < 128) rsa[0][tid] = recq[2 * (l0 + 1) + tid];
+  __syncthreads();
   const bool cv = col < p.ncol;
+  const int lf = field_lmax(p, cv ? col >> 1 : 0);  // this column's truncation degree
+  const long cbase = (long)field_index(p.fmap, cv ? col >> 1 : 0) * p.cstride;
+  const int lwg = p.cg_lmax ? ld_uniform(p.cg_lmax, cg) : lmax;  // degree loop bound
   const int nt = p.ntheta;
   // B operands of this (order, column group): quad-major, 64 lanes per quad,
   // read with raw buffer loads (one lane offset register, scalar quad
@@ -242,9 +309,10 @@ __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
 #pragma unroll
     for (int q = 0; q < 16; q++) Bq[q] = ldB(16 * (wave + 4 * bi) + q);
   };
-  // 16 recurrence steps of one block into the transpose buffer.
-  auto steps16 = [&](int lb, double cs_, double& e0_, double& e1_) {
-    rec_t* r = rc + (lb + 1);
+  // 16 recurrence steps of one block into the transpose buffer: tile lt of
+  // the pass, staged coefficients rl (entry lc + 1 + j at rl[j]).
+  auto steps16 = [&](const d4* rl, int lt, double cs_, double& e0_, double& e1_) {
+    const d4* r = rl + 16 * lt;
 #pragma unroll
     for (int k = 0; k < 16; k++) {
       tr[wave][k][lane] = e0_;
@@ -252,10 +320,14 @@ __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
     }
   };
 
-  for (int lc = l0; lc <= lmax; lc += 16 * kAnalLT) {
-    d4 acc[kAnalLT];
+  for (int lc = l0; lc <= lwg; lc += 16 * kAnalLT) {
+    const d4* rl = reinterpret_cast<const d4*>(&rsa[((lc - l0) >> 6) & 1][0]);
+    const bool more = lc + 64 <= lwg;
+    uint4 rqn = make_uint4(0, 0, 0, 0);
+    if (tid < 128 && more) rqn = recq[2 * (lc + 65) + tid];
+    typename M::acc_t acc[kAnalLT];
 #pragma unroll
-    for (int lt = 0; lt < kAnalLT; lt++) acc[lt] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int lt = 0; lt < kAnalLT; lt++) acc[lt] = M::zero();
 
     double Ba[16], Bb[16];
     if (MB > 0) loadBlk(0, Ba);
@@ -266,30 +338,30 @@ __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
       if (bi + 1 < MB) loadBlk(bi + 1, Bnext);
 #pragma unroll
       for (int lt = 0; lt < kAnalLT; lt++) {
-        steps16(lc + 16 * lt, cs[bi], e0[bi], e1[bi]);
+        steps16(rl, lt, cs[bi], e0[bi], e1[bi]);
         wave_sync();
 #pragma unroll
         for (int q = 0; q < 16; q++) {
           const double A = tr[wave][lane & 15][4 * q + krow];
-          acc[lt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A, Bcur[q], acc[lt], 0, 0, 0);
+          acc[lt] = M::mma(A, Bcur[q], acc[lt]);
         }
         wave_sync();
       }
     }
 
     if (tq > 0) {
-      d4 acct[kAnalLT];
+      typename M::acc_t acct[kAnalLT];
 #pragma unroll
-      for (int lt = 0; lt < kAnalLT; lt++) acct[lt] = d4{0.0, 0.0, 0.0, 0.0};
+      for (int lt = 0; lt < kAnalLT; lt++) acct[lt] = M::zero();
 #pragma unroll
       for (int lt = 0; lt < kAnalLT; lt++) {
-        steps16(lc + 16 * lt, cst, e0t, e1t);
+        steps16(rl, lt, cst, e0t, e1t);
         wave_sync();
         for (int i = 0; i < tq; i++) {
           const int q = (tail0 >> 2) + 4 * i + wave;  // global quad index
           const double Bv = ldB(q);
           const double A = tr[wave][lane & 15][4 * i + krow];
-          acct[lt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A, Bv, acct[lt], 0, 0, 0);
+          acct[lt] = M::mma(A, Bv, acct[lt]);
         }
         wave_sync();
       }
@@ -312,61 +384,69 @@ __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
       s += red[((1 * kAnalLT + lt) * 4 + r) * 64 + lane];
       s += red[((2 * kAnalLT + lt) * 4 + r) * 64 + lane];
       s += red[((3 * kAnalLT + lt) * 4 + r) * 64 + lane];
-      const int l = lc + 16 * lt + krow + 4 * r;
-      if (cv && l <= lmax) {
-        const long ci = ((long)(col >> 1) * p.cstride + coef_index(l, m, p.nabs, p.real)) * 2 +
-                        (col & 1);
+      const int l = lc + 16 * lt + M::row(krow, r);
+      if (cv && l <= lf) {
+        const long ci = (cbase + coef_index(l, m, p.nabs, p.real)) * 2 + (col & 1);
         double v = s;
         if (p.accumulate) v += p.cout[ci];
-        if (p.zero_last && l == lmax && m == lmax) v = 0.0;
+        if (p.zero_last && lf == p.L && l == lf && m == lf) v = 0.0;
         p.cout[ci] = v;
       }
     }
+    if (tid < 128 && more) rsa[(((lc - l0) >> 6) + 1) & 1][tid] = rqn;
     __syncthreads();
   }
 }
 
 template <int MB>
-hipError_t launch_anal(const LegParams& p, hipStream_t s) {
-  hipLaunchKernelGGL(leg_anal_f64<MB>, dim3(p.n_cg, p.nslot), dim3(256), 0, s, p);
+hipError_t launch_anal(const LegParams& p, hipStream_t s, bool f32m) {
+  if (f32m)
+    hipLaunchKernelGGL((leg_anal_f64<MB, true>), dim3(p.n_cg, p.nslot), dim3(256), 0, s, p);
+  else
+    hipLaunchKernelGGL((leg_anal_f64<MB, false>), dim3(p.n_cg, p.nslot), dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+
+template <bool F32M>
+hipError_t launch_synth(const LegParams& p, int T, hipStream_t s) {
+  const dim3 nblocks(p.n_tsb, p.n_cg, p.nslot);  // (theta block, column group, order)
+  if (T == 2 && p.real)
+    hipLaunchKernelGGL((leg_synth_f64<2, 1, F32M>), nblocks, dim3(256), 0, s, p);
+  else if (T == 2)
+    hipLaunchKernelGGL((leg_synth_f64<2, 0, F32M>), nblocks, dim3(256), 0, s, p);
+  else if (p.real)
+    hipLaunchKernelGGL((leg_synth_f64<1, 1, F32M>), nblocks, dim3(256), 0, s, p);
+  else
+    hipLaunchKernelGGL((leg_synth_f64<1, 0, F32M>), nblocks, dim3(256), 0, s, p);
   return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_leg_synth_f64(const LegParams& p0, hipStream_t s) {
+hipError_t launch_leg_synth_f64(const LegParams& p0, hipStream_t s, int tiles, bool f32m) {
   LegParams p = p0;
   p.n_tsb = (p.ntheta + 255) / 256;
-  const int T = p.ncol > 16 ? 2 : 1;
+  const int T = tiles == 1 || tiles == 2 ? tiles : (p.ncol > 16 ? 2 : 1);
   p.n_cg = (p.ncol + 16 * T - 1) / (16 * T);
-  const dim3 nblocks(p.n_tsb, p.n_cg, p.nslot);  // (theta block, column group, order)
-  if (T == 2 && p.real)
-    hipLaunchKernelGGL((leg_synth_f64<2, 1>), nblocks, dim3(256), 0, s, p);
-  else if (T == 2)
-    hipLaunchKernelGGL((leg_synth_f64<2, 0>), nblocks, dim3(256), 0, s, p);
-  else if (p.real)
-    hipLaunchKernelGGL((leg_synth_f64<1, 1>), nblocks, dim3(256), 0, s, p);
-  else
-    hipLaunchKernelGGL((leg_synth_f64<1, 0>), nblocks, dim3(256), 0, s, p);
-  return hipGetLastError();
+  return f32m ? launch_synth<true>(p, T, s) : launch_synth<false>(p, T, s);
 }
 
-hipError_t launch_leg_anal_f64(const LegParams& p0, hipStream_t s) {
+hipError_t launch_leg_anal_f64(const LegParams& p0, hipStream_t s, bool f32m) {
   LegParams p = p0;
   p.n_cg = (p.ncol + 15) / 16;
   p.nquad = (p.ntheta + 3) / 4;
   const int mb = p.nquad / 64;  // full 16-quad blocks per wave
   switch (mb) {
-    case 0: return launch_anal<0>(p, s);
-    case 1: return launch_anal<1>(p, s);
-    case 2: return launch_anal<2>(p, s);
-    case 3: return launch_anal<3>(p, s);
-    case 4: return launch_anal<4>(p, s);
-    case 5: return launch_anal<5>(p, s);
-    case 6: return launch_anal<6>(p, s);
-    case 7: return launch_anal<7>(p, s);
-    case 8: return launch_anal<8>(p, s);
-    case 9: return launch_anal<9>(p, s);
+    case 0: return launch_anal<0>(p, s, f32m);
+    case 1: return launch_anal<1>(p, s, f32m);
+    case 2: return launch_anal<2>(p, s, f32m);
+    case 3: return launch_anal<3>(p, s, f32m);
+    case 4: return launch_anal<4>(p, s, f32m);
+    case 5: return launch_anal<5>(p, s, f32m);
+    case 6: return launch_anal<6>(p, s, f32m);
+    case 7: return launch_anal<7>(p, s, f32m);
+    case 8: return launch_anal<8>(p, s, f32m);
+    case 9: return launch_anal<9>(p, s, f32m);
     default: return hipErrorInvalidValue;
   }
 }

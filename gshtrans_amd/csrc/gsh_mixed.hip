@@ -43,6 +43,12 @@ __device__ __forceinline__ void rec_step(const d4 q, double cs, double& e0, doub
   e0 = en;
 }
 
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ int order_of(int mw, int real) {
   return real ? mw : ((mw & 1) ? -((mw + 1) >> 1) : (mw >> 1));
 }
@@ -136,7 +142,9 @@ __global__ __launch_bounds__(256) void coef_split(LegParams p, long nblocks) {
   const int kr = lane >> 4;
   const int col = tile * 16 + (lane & 15);
   const bool cv = col < p.ncol;
-  const double* cp = p.cin + (long)(cv ? col >> 1 : 0) * p.cstride * 2 + (col & 1);
+  const int f = cv ? col >> 1 : 0;
+  const int lf = field_lmax(p, f);  // this column's truncation degree
+  const double* cp = p.cin + (long)field_index(p.fmap, f) * p.cstride * 2 + (col & 1);
   unsigned planes[NS][4];
 #pragma unroll
   for (int e2 = 0; e2 < 4; e2++) {
@@ -144,8 +152,8 @@ __global__ __launch_bounds__(256) void coef_split(LegParams p, long nblocks) {
 #pragma unroll
     for (int u = 0; u < 2; u++) {
       const int l = l0 + 32 * chunk + 8 * kr + 2 * e2 + u;
-      const double v = cp[2 * coef_index(min(l, p.lmax), m, p.nabs, REAL)];
-      x[u] = (cv && l <= p.lmax) ? v : 0.0;
+      const double v = cp[lf >= l0 ? 2 * coef_index(min(l, lf), m, p.nabs, REAL) : 0];
+      x[u] = (cv && l <= lf) ? v : 0.0;
     }
     unsigned o[NS];
     split_pair<NS>(x[0], x[1], o);
@@ -234,20 +242,31 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
   const double cs = tpv ? cs0 : 0.0;
   double e0 = tpv ? s0 : 0.0;
   double e1 = 0.0;
-  rec_t* rc = rec_ptr(p.rec, ld_uniform(p.roff, m + p.L) - l0);
 
+  // Recurrence coefficients of a chunk (entries lb+1 .. lb+32, 1 KiB) are
+  // staged through a per-wave LDS slot: lane j fetches 16 bytes of the next
+  // chunk one chunk ahead, so the steps read them from LDS instead of
+  // waiting on scalar loads.  Entries up to L + kRecPad exist, so fetching
+  // chunk nchunk is in bounds.
+  __shared__ uint4 rst[4][64];
+  const uint4* recq = reinterpret_cast<const uint4*>(p.rec + (ld_uniform(p.roff, m + p.L) - l0));
+  auto fetch = [&](int chunk) { return recq[2 * (l0 + 32 * chunk + 1) + lane]; };
+  uint4 rq = prod ? fetch(0) : make_uint4(0, 0, 0, 0);
   auto produce = [&](int chunk, int buf) {
-    const int lb = l0 + 32 * chunk;
+    rst[wave & 3][lane] = rq;
+    wave_sync();
+    rq = fetch(chunk + 1);
+    const d4* rl = reinterpret_cast<const d4*>(&rst[wave & 3][0]);  // rl[j]: entry lb + 1 + j
 #pragma unroll
     for (int kr = 0; kr < 4; kr++) {
       unsigned pl[NS][4];
 #pragma unroll
       for (int e2 = 0; e2 < 4; e2++) {
-        const int l = lb + 8 * kr + 2 * e2;
+        const int j = 8 * kr + 2 * e2;
         const double v0 = e0;
-        rec_step(rc[l + 1], cs, e0, e1);
+        rec_step(rl[j], cs, e0, e1);
         const double v1 = e0;
-        rec_step(rc[l + 2], cs, e0, e1);
+        rec_step(rl[j + 1], cs, e0, e1);
         unsigned o[NS];
         split_pair<NS>(v0, v1, o);
 #pragma unroll
@@ -257,6 +276,7 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
       for (int s = 0; s < NS; s++)
         trb[buf][s][kr][(wave & 3) * 64 + lane] = make_uint4(pl[s][0], pl[s][1], pl[s][2], pl[s][3]);
     }
+    wave_sync();
   };
 
   // Consumer role.
@@ -357,8 +377,15 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   const int nt = p.ntheta;
   const int nblk = (nt + 63) >> 6;
   const int ngrp = (nblk + 1) >> 1;
-  rec_t* rc = rec_ptr(p.rec, ld_uniform(p.roff, m + p.L) - l0);
   const double* seeds = p.seeds + (long)(m + p.L) * nt;
+  // Recurrence coefficients of a pass (entries lp+1 .. lp+64, 2 KiB) are
+  // staged in LDS, double-buffered by pass parity: threads 0..127 fetch the
+  // next pass's 16 bytes each at the start of a pass and store them before
+  // the pass's last barrier.  Entries up to L + kRecPad exist.
+  __shared__ uint4 rsa[2][128];
+  const uint4* recq = reinterpret_cast<const uint4*>(p.rec + (ld_uniform(p.roff, m + p.L) - l0));
+  const int tid = threadIdx.x;
+  if (tid < 128) rsa[0][tid] = recq[2 * (l0 + 1) + tid];
 
   // Initial recurrence state of every block.
   for (int b = wave; b < nblk; b += 8) {
@@ -369,6 +396,7 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
 
   // Produce the bf16 planes of block b (degrees lp..lp+63) into buffer half bb.
   auto produce = [&](int b, int lp, int buf, int bb) {
+    const d4* rl = reinterpret_cast<const d4*>(&rsa[((lp - l0) >> 6) & 1][0]);  // entry lp+1+j
     const int t = b * 64 + lane;
     const double cs0 = p.cosv[min(t, nt - 1)];
     const double cs = t < nt ? cs0 : 0.0;
@@ -377,9 +405,9 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
 #pragma unroll 4
     for (int k = 0; k < 64; k += 2) {
       const double v0 = e0;
-      rec_step(rc[lp + k + 1], cs, e0, e1);
+      rec_step(rl[k], cs, e0, e1);
       const double v1 = e0;
-      rec_step(rc[lp + k + 2], cs, e0, e1);
+      rec_step(rl[k + 1], cs, e0, e1);
       unsigned o[NS];
       split_pair<NS>(v0, v1, o);
 #pragma unroll
@@ -431,7 +459,11 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
     // B operands are loaded one K chunk ahead of their use.
     uint4 B[2][NS];
     loadB(0, B);
+    const bool more = lp + 64 <= lmax;
+    uint4 rqn = make_uint4(0, 0, 0, 0);
+    if (tid < 128 && more) rqn = recq[2 * (lp + 65) + tid];
     for (int g = 0; g < ngrp; g++) {
+      if (g == ngrp - 1 && tid < 128 && more) rsa[(((lp - l0) >> 6) + 1) & 1][tid] = rqn;
       if (g + 1 < ngrp) produce_group(g + 1, lp, (g + 1) & 1);
       const int buf = g & 1;
       auto readA = [&](int kq, uint4 (&A)[2][NS]) {
@@ -504,15 +536,16 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
       for (int ct = 0; ct < 2; ct++) {
         const int col = (tile0 + ct) * 16 + (lane & 15);
         if (col >= p.ncol) continue;
+        const int lf = field_lmax(p, col >> 1);
+        const long cbase = (long)field_index(p.fmap, col >> 1) * p.cstride;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           const int l = lp + 16 * (2 * lh + lt) + 4 * (lane >> 4) + r;
-          if (l > lmax) continue;
-          const long ci = ((long)(col >> 1) * p.cstride + coef_index(l, m, p.nabs, REAL)) * 2 +
-                          (col & 1);
+          if (l > lf) continue;
+          const long ci = (cbase + coef_index(l, m, p.nabs, REAL)) * 2 + (col & 1);
           double v = acc[lt][ct][r];
           if (p.accumulate) v += p.cout[ci];
-          if (p.zero_last && l == lmax && m == lmax) v = 0.0;
+          if (p.zero_last && lf == p.L && l == lf && m == lf) v = 0.0;
           p.cout[ci] = v;
         }
       }

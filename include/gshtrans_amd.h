@@ -26,6 +26,9 @@
  *   gsh_wigner_seed        WignerDetails::WignerMinOrder, Wigner.h:52-80
  *   gsh_adjoint_inverse    no reference counterpart: the transpose of the
  *                          synthesis, used for least-squares gradients
+ *   gsh_plan_*             no reference counterpart: batches of fields with
+ *                          uneven lMax and upper index in one plan (graphs,
+ *                          autotuned tiles)
  *   gsh_timing_enable /    no reference counterpart (the reference has no
  *   gsh_timing_read        timing code): HIP events around each kernel
  *                          launch, on the launch stream, for benchmarks
@@ -117,6 +120,39 @@ int gsh_adjoint_inverse(gsh_grid_t grid, int lmax, int n, int value_kind,
                         int precision, int storage, int batch, const void* field,
                         long field_stride, void* coeffs, long coeff_stride,
                         void* stream);
+
+/* ---- batch plans: many fields of uneven truncation and upper index ---- */
+/* No reference counterpart (the reference transforms one field per call,
+ * GaussLegendreGrid.h:108-321).  A plan fixes `count` fields on one grid:
+ * field i has truncation degree lmax[i] (1 <= lmax[i] <= grid lMax) and
+ * upper index n[i]; all share one value kind, precision and storage.  The
+ * plan groups the fields by upper index, orders each group by degree so that
+ * every 16-column group of a launch carries its own degree bound, and on its
+ * first execution in each direction times the candidate tile shapes
+ * (synthesis column tiles, ring-FFT kernel family) and keeps the fastest.
+ *
+ * Arrays: coefficients of field i at coeffs + i*coeff_stride (complex
+ * elements of the storage type; GSHIndices(lmax[i], lmax[i], n[i]) layout,
+ * coeff_stride >= the largest coefficient count), field i at
+ * field + i*field_stride (elements of the storage type).
+ *
+ * graph != 0 captures the whole launch sequence for these pointers into a
+ * HIP graph on first use (after one eager run) and replays that graph on
+ * later calls with the same pointers and strides. */
+typedef struct gsh_plan_s* gsh_plan_t;
+int gsh_plan_create(gsh_grid_t grid, int count, const int* lmax, const int* n,
+                    int value_kind, int precision, int storage, gsh_plan_t* plan);
+int gsh_plan_destroy(gsh_plan_t plan);
+int gsh_plan_inverse(gsh_plan_t plan, const void* coeffs, long coeff_stride, void* field,
+                     long field_stride, int graph, void* stream);
+/* Analysis; overwrites the coefficients (no accumulation). */
+int gsh_plan_forward(gsh_plan_t plan, const void* field, long field_stride, void* coeffs,
+                     long coeff_stride, int graph, void* stream);
+/* Launch groups and the tuned choices: per group g < *ngroups (up to cap),
+ * n, lmax, field count, synthesis column tiles (0 = not tuned yet), and the
+ * FFT family of synthesis / analysis (1 wavefront-per-ring, 2 LDS). */
+int gsh_plan_info(gsh_plan_t plan, int* ngroups, int cap, int* n, int* lmax, int* count,
+                  int* synth_tiles, int* fft_inverse, int* fft_forward);
 
 /* ---- per-kernel timing ------------------------------------------------- */
 enum gsh_kernel {

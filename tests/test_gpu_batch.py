@@ -1,0 +1,141 @@
+"""Batch plans (gsh_plan_*): many fields of uneven truncation degree and
+upper index through grouped, autotuned launches and HIP-graph replays.
+
+Every field of a plan must equal the plain single-field call (gsh_inverse /
+gsh_forward) on the same input, which the FP64 parity tests pin to the
+oracle; a few fields are also checked against the oracle directly.  FP32
+storage is checked against the FP64 plan within FP32 rounding.  Graph
+replays are interleaved with eager work and each replay is checked against
+the eager result.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gshtrans_amd as G
+import oracle
+
+pytestmark = pytest.mark.gpu
+SEED0 = 20261202
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    assert _t.cuda.is_available(), "GPU tests need a device"
+    return _t
+
+
+def rel(a, b):
+    import torch
+    return float(torch.linalg.vector_norm((a - b).reshape(-1)) /
+                 max(float(torch.linalg.vector_norm(b.reshape(-1))), 1e-300))
+
+
+def make_coeffs(torch, lmax, n, real, cmax, seed, dtype=None):
+    out = np.zeros((len(lmax), cmax), dtype=np.complex128)
+    for i, (l, m) in enumerate(zip(lmax, n)):
+        c = G.random_coefficients(int(l), int(m), real=real, seed=seed + i)
+        out[i, :c.size] = c
+    t = torch.from_numpy(out).cuda()
+    return t if dtype is None else t.to(dtype)
+
+
+@pytest.mark.parametrize("L,real", [(128, False), (128, True), (48, False), (256, True)])
+def test_plan_matches_single_calls(torch, L, real):
+    rng = np.random.default_rng(L + real)
+    count = 21
+    lmax = np.clip(np.round(np.exp(rng.uniform(np.log(4), np.log(L), count))), 2, L).astype(int)
+    lmax[0] = L
+    n = rng.integers(-2, 3, count)
+    if real:
+        n = np.abs(n)
+    n = np.where(np.abs(n) > lmax, 0, n)
+    g = G.GaussLegendreGrid(L, 2)
+    plan = G.BatchPlan(g, lmax, n, real=real)
+    say(f"[plan] L={L} real={real} groups={plan.info()}")
+    c = make_coeffs(torch, lmax, n, real, plan.cmax, SEED0 + L)
+    f = plan.inverse(c)
+    c2 = plan.forward(f)
+    say(f"[plan] tuned {plan.info()}")
+    for i in range(count):
+        C = plan.sizes[i]
+        fs = g.inverse(int(lmax[i]), int(n[i]), c[i, :C].contiguous(), real=real)
+        assert rel(f[i], fs) <= 1e-13, i
+        cs = g.forward(int(lmax[i]), int(n[i]), fs, real=real)
+        assert rel(c2[i, :C], cs) <= 1e-13, i
+        assert float((c2[i, C:]).abs().max()) == 0.0 if C < plan.cmax else True
+    og = oracle.OracleGrid(L, 2, table=False)
+    for i in (0, count - 1):
+        C = plan.sizes[i]
+        ref = og.inverse(int(lmax[i]), int(n[i]), c[i, :C].cpu().numpy(), real)
+        assert rel(f[i].cpu(), torch.from_numpy(ref)) <= 1e-12
+    og.close()
+
+
+def test_plan_fp32_storage_and_graph_replays(torch):
+    L = 128
+    rng = np.random.default_rng(3)
+    count = 12
+    lmax = rng.integers(8, L + 1, count)
+    n = rng.integers(-2, 3, count)
+    g = G.GaussLegendreGrid(L, 2)
+    p64 = G.BatchPlan(g, lmax, n)
+    p32 = G.BatchPlan(g, lmax, n, storage="fp32")
+    c = make_coeffs(torch, lmax, n, False, p64.cmax, SEED0 + 9)
+    f64 = p64.inverse(c)
+    c32 = c.to(torch.complex64)
+    f32 = p32.inverse(c32)
+    assert f32.dtype == torch.complex64
+    assert rel(f32.to(torch.complex128), f64) < 1e-6
+    a32 = p32.forward(f32)
+    assert a32.dtype == torch.complex64
+    assert rel(a32.to(torch.complex128), c) < 1e-5
+    # Graph: capture on first use, replay with eager work and syncs between.
+    fg = torch.empty_like(f32)
+    eager = p32.inverse(c32).clone()
+    ag = torch.zeros_like(c32)
+    eager_a = p32.forward(eager).clone()
+    for rep in range(4):
+        fg.zero_()
+        p32.inverse(c32, field=fg, graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(fg, eager), rep
+        ag.zero_()
+        p32.forward(fg, coeffs=ag, graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(ag, eager_a), rep
+        _ = g.inverse(int(lmax[0]), int(n[0]), c[0, :p64.sizes[0]].contiguous())  # eager work
+        torch.cuda.synchronize()
+
+
+def test_config4_uneven_batch_round_trip(torch):
+    """256 fields, lMax log-uniform in [32, 1024], n uniform in {-2..2},
+    complex FP32 storage, coefficients (l+1)^-1: synthesis then analysis
+    through one plan (graph replay), round trip within FP32 rounding, and
+    five fields against the plain FP64 single-field calls."""
+    L, count = 1024, 256
+    rng = np.random.default_rng(SEED0 + 4)
+    lmax = np.round(np.exp(rng.uniform(np.log(32), np.log(1024), count))).astype(int)
+    n = rng.integers(-2, 3, count)
+    g = G.GaussLegendreGrid(L, 2)
+    plan = G.BatchPlan(g, lmax, n, storage="fp32")
+    c = make_coeffs(torch, lmax, n, False, plan.cmax, SEED0 + 400, dtype=torch.complex64)
+    f = plan.inverse(c, graph=True)
+    f = plan.inverse(c, field=f, graph=True)
+    c2 = plan.forward(f, graph=True)
+    c2 = plan.forward(f, coeffs=c2, graph=True)
+    torch.cuda.synchronize()
+    say(f"[config4] groups {plan.info()}")
+    err = max(rel(c2[i, :plan.sizes[i]], c[i, :plan.sizes[i]]) for i in range(count))
+    say(f"[config4] worst round-trip rel L2 {err:.2e}")
+    assert err < 2e-6
+    for i in rng.choice(count, 5, replace=False):
+        C = plan.sizes[i]
+        ref = g.inverse(int(lmax[i]), int(n[i]), c[i, :C].to(torch.complex128).contiguous())
+        assert rel(f[i].to(torch.complex128), ref) < 1e-6, i

@@ -2,7 +2,9 @@
 seeds, weights and FFT): relative L2 error of each field against the FP64
 GPU transform, which is itself pinned to the oracle at <= 1e-12 by
 test_gpu_parity.py.  Bars: bf16x6 <= 1e-7 (the precision target), bf16x3
-<= 1e-4 (two-way split, ~16 significant bits per operand)."""
+<= 1e-4 (two-way split, ~16 significant bits per operand), fp32 <= 3e-5
+(FP32 MFMA on FP32-rounded operands with FP32 accumulation over up to
+L + 1 terms; the mode is reported, not held to the 1e-7 target)."""
 import sys
 
 import numpy as np
@@ -12,7 +14,7 @@ import gshtrans_amd as G
 
 pytestmark = pytest.mark.gpu
 SEED0 = 20261202
-BARS = {"bf16x6": 1e-7, "bf16x3": 1e-4}
+BARS = {"bf16x6": 1e-7, "bf16x3": 1e-4, "fp32": 3e-5}
 
 
 def say(*a):
@@ -41,7 +43,7 @@ def per_field_rel(torch, a, b):
             torch.linalg.vector_norm(b.reshape(b.shape[0], -1), dim=1)).cpu().numpy()
 
 
-@pytest.mark.parametrize("prec", ["bf16x6", "bf16x3"])
+@pytest.mark.parametrize("prec", ["bf16x6", "bf16x3", "fp32"])
 @pytest.mark.parametrize("L,n,real,batch", [(64, 0, False, 3), (64, 2, True, 5),
                                             (512, 2, False, 8), (300, -1, False, 2),
                                             (1024, 0, True, 16), (1024, 2, False, 16),
@@ -59,7 +61,7 @@ def test_mixed_synthesis_error(torch, prec, L, n, real, batch):
     assert e.min() > 0 or prec == "fp64"
 
 
-@pytest.mark.parametrize("prec", ["bf16x6", "bf16x3"])
+@pytest.mark.parametrize("prec", ["bf16x6", "bf16x3", "fp32"])
 @pytest.mark.parametrize("L,n,real,batch", [(64, 0, False, 3), (64, 2, True, 5),
                                             (512, 2, False, 8), (300, -1, False, 2),
                                             (1024, 0, True, 16), (1024, 2, False, 16),
