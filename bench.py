@@ -86,7 +86,8 @@ def run_gpu(args):
     # FP64, the headline mixed mode, then the other reduced-precision modes
     # (each reported with its own throughput and errors).
     headline = args.precision or HEADLINE
-    modes = ["fp64"] + [m for m in [headline] + REPORTED if m != "fp64"]
+    extra = REPORTED if args.modes is None else [m for m in args.modes.split(",") if m]
+    modes = ["fp64"] + [m for m in [headline] + extra if m != "fp64"]
     modes = list(dict.fromkeys(modes))
     for m in modes:
         outs[m] = buffers(m)
@@ -206,6 +207,18 @@ def cpu_baseline(args):
             "setup_s": setup, "setup_threads": oracle.lib().gsho_num_threads()}
 
 
+def copy_probe():
+    """A plain copy kernel of known size: 1 GiB read + 1 GiB written per
+    dispatch, the calibration for FETCH_SIZE / WRITE_SIZE counter runs."""
+    import torch
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_per_dispatch": a.numel() * 8}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,8 +226,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default=None,
                     help="headline mode (default bf16x6)")
+    ap.add_argument("--modes", default=None,
+                    help="comma-separated reduced-precision modes reported beside the "
+                         "headline (default bf16x3,fp32)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy 5 times (the "
+                         "known-size reference for HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
               "(the transform batch shards over GPUs without exchange; see DESIGN.md)",

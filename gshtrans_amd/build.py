@@ -49,5 +49,26 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+ROOT = os.path.dirname(HERE)
+FACADE_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_facade.cpp")
+FACADE_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_facade")
+
+
+def build_facade_test(verbose: bool = False) -> str:
+    """Compile the C++ facade test (include/GSHTrans) against the library;
+    the binary finds the library through an $ORIGIN-relative rpath."""
+    cmd = ["g++", "-std=c++20", "-O1", "-Wall", "-I", os.path.join(ROOT, "include"),
+           FACADE_TEST_SRC, "-o", FACADE_TEST_BIN + ".tmp", "-L", HERE, "-lgshtrans_hip",
+           "-Wl,-rpath,$ORIGIN/../../gshtrans_amd", "-Wl,--allow-shlib-undefined"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + r.stderr)
+    os.replace(FACADE_TEST_BIN + ".tmp", FACADE_TEST_BIN)
+    return FACADE_TEST_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_facade_test(verbose=True))

@@ -660,7 +660,8 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
                 const RunOpts& o) {
   int st;
   const int nslot = real ? lmax + 1 : 2 * lmax + 1;
-  if ((st = ensure_work(g, sizeof(double) * g->ntheta() * nslot * 2 * (size_t)batch))) return st;
+  const size_t ncg16 = (2 * (size_t)batch + 15) / 16;
+  if ((st = ensure_work(g, sizeof(double) * g->ntheta() * nslot * 16 * ncg16))) return st;
   LegParams p = base_params(g, d, lmax, n, real, batch);
   p.cin = cin;
   p.cstride = cstride;

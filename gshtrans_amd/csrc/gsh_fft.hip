@@ -164,7 +164,7 @@ __global__ __launch_bounds__(kFFTThreads) void fft_synth(FFTParams p) {
   const int theta = blockIdx.x / p.nfg;
   const int fg = blockIdx.x % p.nfg;
   const int N = p.N, M = p.M, R = p.rings;
-  const long frow = (long)theta * p.nslot * p.ncol;
+  const int ncg16 = (p.ncol + 15) >> 4;
   // Fill: consecutive threads read consecutive fields of one bin.
   for (int idx = threadIdx.x; idx < R * M; idx += kFFTThreads) {
     const int q = idx % R;
@@ -172,22 +172,23 @@ __global__ __launch_bounds__(kFFTThreads) void fft_synth(FFTParams p) {
     const int f = fg * R + q;
     double2 X = make_double2(0.0, 0.0);
     if (f < p.nfield && k < N) {
-      const double* Fp = p.F + frow + 2 * f;
+      // slot s of this field at Fp[16 s] (F[theta][col/16][slot][16])
+      const double* Fp = p.F + f_index(theta, 0, 2 * f, p.nslot, ncg16);
       if (!p.real) {
         if (k <= p.lmax) {
           const int s = k - p.m_first;
-          X = make_double2(Fp[(long)s * p.ncol], Fp[(long)s * p.ncol + 1]);
+          X = make_double2(Fp[16 * (long)s], Fp[16 * (long)s + 1]);
         }
         if (k >= N - p.lmax) {
           const int s = (k - N) - p.m_first;
-          X = cadd(X, make_double2(Fp[(long)s * p.ncol], Fp[(long)s * p.ncol + 1]));
+          X = cadd(X, make_double2(Fp[16 * (long)s], Fp[16 * (long)s + 1]));
         }
       } else {
         // Hermitian spectrum from bins 0..N/2 (c2r semantics).
         const int kk = k <= N / 2 ? k : N - k;
         if (kk <= p.lmax) {
           const int s = kk - p.m_first;
-          X = make_double2(Fp[(long)s * p.ncol], Fp[(long)s * p.ncol + 1]);
+          X = make_double2(Fp[16 * (long)s], Fp[16 * (long)s + 1]);
           if (kk == 0 || kk == N / 2) X.y = 0.0;
           else if (k > N / 2) X.y = -X.y;
         }

@@ -238,20 +238,21 @@ __global__ __launch_bounds__(512) void fft_synth_w(FFTParams p) {
   if (f0 >= p.nfield) return;
   const bool f1v = REAL && f0 + 1 < p.nfield;
   const int lmax = p.lmax;
-  const double* Frow = p.F + (long)theta * p.nslot * p.ncol + 2 * f0;
+  // Order bins of this field: slot s at Frow[16 s] (F[theta][col/16][slot][16]).
+  const double* Frow = p.F + f_index(theta, 0, 2 * f0, p.nslot, (p.ncol + 15) >> 4);
   double2 v[R];
 #pragma unroll
   for (int n1 = 0; n1 < R; n1++) {
     const int k = 64 * n1 + lane;
     double2 X = make_double2(0.0, 0.0);
     if (!REAL) {
-      if (k <= lmax) X = ld2(Frow + (long)(k - p.m_first) * p.ncol);
-      if (k >= N - lmax) X = cadd(X, ld2(Frow + (long)(k - N - p.m_first) * p.ncol));
+      if (k <= lmax) X = ld2(Frow + 16 * (long)(k - p.m_first));
+      if (k >= N - lmax) X = cadd(X, ld2(Frow + 16 * (long)(k - N - p.m_first)));
     } else {
       const int kk = k <= N / 2 ? k : N - k;
       if (kk <= lmax) {
-        double2 xa = ld2(Frow + (long)kk * p.ncol);
-        double2 xb = f1v ? ld2(Frow + (long)kk * p.ncol + 2) : make_double2(0.0, 0.0);
+        double2 xa = ld2(Frow + 16 * (long)kk);
+        double2 xb = f1v ? ld2(Frow + 16 * (long)kk + 2) : make_double2(0.0, 0.0);
         // c2r reads bins 0..N/2 only and ignores Im of bins 0 and N/2.
         if (kk == 0 || kk == N / 2) {
           xa.y = 0.0;

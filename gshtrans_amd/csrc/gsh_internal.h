@@ -18,8 +18,10 @@ constexpr int kRecPad = 64;
 // Work layouts in HBM.  slot indexes the order m (slot = m - m_first;
 // m_first = -lmax for complex fields, 0 for real fields) and
 // col = 2*field + part (part 0 = real, 1 = imaginary).
-//   synthesis  F[theta][slot][col]: the Legendre kernel writes F_m(theta),
-//              the ring FFT reads all orders of one ring
+//   synthesis  F[theta][col/16][slot][16]: the Legendre kernel writes
+//              F_m(theta) in 128-byte rows of 16 columns, and the ring FFT of
+//              one colatitude and 16-column group reads one contiguous block
+//              of every order; see f_index()
 //   analysis   FA[slot][cg][quad][64]: the ring FFT writes the weighted bins
 //              X'_m(theta) in the lane order of the 4x16 MFMA B operand,
 //              quad = theta/4, cg = col/16, lane = 16*(theta%4) + col%16,
@@ -78,6 +80,12 @@ __device__ __forceinline__ int field_lmax(const LegParams& p, int f) {
 }
 __device__ __forceinline__ int field_index(const int* fmap, int f) {
   return fmap ? fmap[f] : f;
+}
+
+// Index of (theta, slot, col) in the synthesis work layout F (ncg16 =
+// ceil(ncol / 16) column groups).
+__host__ __device__ inline long f_index(int theta, int slot, int col, int nslot, int ncg16) {
+  return (((long)theta * ncg16 + (col >> 4)) * nslot + slot) * 16 + (col & 15);
 }
 
 // Load of a wave-uniform table entry through the constant address space, so

@@ -221,7 +221,8 @@ __global__ __launch_bounds__(256) void leg_synth_f64(LegParams p) {
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int row = tsb * 256 + wave * 64 + g * 16 + M::row(krow, r);
-        if (row < p.ntheta) p.F[((long)row * p.nslot + slot) * p.ncol + col] = acc[g][j][r];
+        if (row < p.ntheta)
+          p.F[f_index(row, slot, col, p.nslot, (p.ncol + 15) >> 4)] = acc[g][j][r];
       }
     }
 }

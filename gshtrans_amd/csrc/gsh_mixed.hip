@@ -344,7 +344,8 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int row = tsb * 256 + h * 128 + g * 16 + 4 * (lane >> 4) + r;
-        if (row < p.ntheta) p.F[((long)row * p.nslot + slot) * p.ncol + col] = acc[g][t][r];
+        if (row < p.ntheta)
+          p.F[f_index(row, slot, col, p.nslot, (p.ncol + 15) >> 4)] = acc[g][t][r];
       }
     }
 }

@@ -1,0 +1,120 @@
+// C++ facade test: the reference's own checks written against
+// include/GSHTrans (GaussLegendreGrid, CanonicalComponentField,
+// CanonicalComponentExpansion) running on the GPU through the C ABI.
+//   * accessors of the grid (GaussLegendreGrid.h:86-103, GridBase.h:17-113);
+//   * the coefficient round trip of tests/CheckCoeff2Coeff.h:74-83 for every
+//     upper index, complex and real fields, bar 5e4 eps;
+//   * the typed containers feeding the transforms through Data().
+// Prints one line per check and "OK" at the end; exits non-zero on failure.
+#include <GSHTrans/Core>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <limits>
+#include <vector>
+
+using namespace GSHTrans;
+using Grid = GaussLegendreGrid<double, All, All>;
+using RealGrid = GaussLegendreGrid<double, NonNegative, All>;
+
+static int failures = 0;
+static void expect(bool ok, const char* what) {
+  std::printf("%s %s\n", ok ? "pass" : "FAIL", what);
+  if (!ok) failures++;
+}
+
+template <typename G, typename Scalar>
+static double round_trip(const G& grid, int lMax, int n, bool real, unsigned seed) {
+  const auto size = real ? grid.RealCoefficientSize(lMax, n) : grid.ComplexCoefficientSize(lMax, n);
+  std::vector<std::complex<double>> c(size), c2(size, 0.0);
+  if (real)
+    grid.RandomRealCoefficient(lMax, n, c, seed);
+  else
+    grid.RandomComplexCoefficient(lMax, n, c, seed);
+  std::vector<Scalar> f(grid.FieldSize());
+  grid.InverseTransformation(lMax, n, c, f);
+  grid.ForwardTransformation(lMax, n, f, c2);
+  double err = 0;
+  for (std::size_t i = 0; i < c.size(); i++) err = std::max(err, std::abs(c2[i] - c[i]));
+  return err;
+}
+
+int main() {
+  const double eps = std::numeric_limits<double>::epsilon();
+  const int L = 32;
+  Grid grid(L, 2);
+  expect(grid.MaxDegree() == L && grid.MaxUpperIndex() == 2, "MaxDegree / MaxUpperIndex");
+  expect(grid.NumberOfCoLatitudes() == L + 1 && grid.NumberOfLongitudes() == 2 * L,
+         "grid shape (L+1) x 2L");
+  expect(grid.FieldSize() == (L + 1) * 2 * L, "FieldSize");
+  expect(grid.ComplexCoefficientSize(L, 2) == (L + 1) * (L + 1) - 4, "ComplexCoefficientSize");
+  expect(grid.RealCoefficientSize(L, 2) == (L + 1) * (L + 2) / 2 - 3, "RealCoefficientSize");
+  double wsum = 0;
+  for (double w : grid.CoLatitudeWeights()) wsum += w;
+  expect(std::abs(wsum - 2.0) < 1e-14, "CoLatitudeWeights sum to 2");
+  bool asc = true;
+  double prev = -1;
+  for (double t : grid.CoLatitudes()) {
+    asc = asc && t > prev && t > 0 && t < M_PI;
+    prev = t;
+  }
+  expect(asc, "CoLatitudes ascending in (0, pi)");
+  double lsum = 0;
+  for (double w : grid.LongitudeWeights()) lsum += w;
+  expect(std::abs(lsum - 2 * M_PI) < 1e-13, "LongitudeWeights sum to 2 pi");
+  int nidx = 0;
+  for (auto n : grid.UpperIndices()) nidx += static_cast<int>(n) + 10;
+  expect(nidx == 50, "UpperIndices -2..2");
+
+  for (int n = -2; n <= 2; n++) {
+    const double e = round_trip<Grid, std::complex<double>>(grid, L, n, false, 10 + n);
+    std::printf("  complex n=%d round trip %.2e\n", n, e);
+    expect(e < 5e4 * eps, "complex coefficient round trip");
+    const double e2 = round_trip<Grid, std::complex<double>>(grid, L - 7, n, false, 20 + n);
+    expect(e2 < 5e4 * eps, "complex round trip, lMax < L");
+  }
+  RealGrid rgrid(L, 2);
+  for (int n = -2; n <= 2; n++) {
+    const double e = round_trip<RealGrid, double>(rgrid, L, n, true, 30 + n);
+    std::printf("  real n=%d round trip %.2e\n", n, e);
+    expect(e < 5e4 * eps, "real coefficient round trip");
+  }
+
+  // Typed containers.
+  ComplexCanonicalComponentExpansion<1, Grid> u(grid);
+  grid.RandomComplexCoefficient(L, 1, u.Data(), 7);
+  ComplexCanonicalComponentField<1, Grid> f(grid);
+  grid.InverseTransformation(L, 1, u.Data(), f.Data());
+  ComplexCanonicalComponentExpansion<1, Grid> v(grid);
+  grid.ForwardTransformation(L, 1, f.Data(), v.Data());
+  double e = 0;
+  for (int l = 1; l <= L; l++)
+    for (int m = -l; m <= l; m++) e = std::max(e, std::abs(v(l, m) - u(l, m)));
+  expect(e < 5e4 * eps, "CanonicalComponentExpansion / Field round trip");
+  // A single mode: Y_{3,2} (n = 0) synthesised then read back.
+  ComplexCanonicalComponentExpansion<0, Grid> y(grid);
+  y(3, 2) = 1.0;
+  ComplexCanonicalComponentField<0, Grid> fy(grid);
+  grid.InverseTransformation(L, 0, y.Data(), fy.Data());
+  ComplexCanonicalComponentExpansion<0, Grid> y2(grid);
+  grid.ForwardTransformation(L, 0, fy.Data(), y2.Data());
+  expect(std::abs(y2(3, 2) - 1.0) < 1e-13 && std::abs(y2(3, 1)) < 1e-13, "single mode");
+  // The library's errors surface as exceptions (the reference asserts).
+  bool threw = false;
+  try {
+    std::vector<std::complex<double>> c(grid.ComplexCoefficientSize(L, 0));
+    std::vector<std::complex<double>> ff(grid.FieldSize());
+    grid.InverseTransformation(L, 3, c, ff);
+  } catch (const Error& err) {
+    threw = err.status == GSH_ERR_UPPER_INDEX;
+  }
+  expect(threw, "upper index outside the grid raises");
+  if (failures) {
+    std::printf("%d FAILED\n", failures);
+    return 1;
+  }
+  std::printf("OK\n");
+  return 0;
+}

@@ -104,3 +104,19 @@ def test_random_coefficients_masks_and_reproducible():
     # roughly standard normal before the (l+1)^-decay scaling
     big = G.random_coefficients(200, 0, seed=9, decay=0.0)
     assert abs(np.std(big.real) - 1) < 0.02 and abs(np.mean(big.imag)) < 0.02
+
+
+def test_cpp_facade_compiles_and_reports_errors():
+    """include/GSHTrans (the C++ facade mirroring the reference's
+    GaussLegendreGrid / CanonicalComponentField API) compiles as C++20; without
+    a device its grid constructor raises the library's NO_DEVICE status."""
+    import subprocess
+    src = os.path.join(ROOT, "tests", "cpp", "test_facade.cpp")
+    r = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-Wall", "-I",
+                        os.path.join(ROOT, "include"), src], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    binary = os.path.join(ROOT, "tests", "cpp", "test_facade")
+    if gpu_available() or not os.path.exists(binary):
+        return
+    r = subprocess.run([binary], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "no gfx950 device" in r.stderr
