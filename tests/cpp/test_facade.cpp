@@ -22,6 +22,7 @@ using RealGrid = GaussLegendreGrid<double, NonNegative, All>;
 static int failures = 0;
 static void expect(bool ok, const char* what) {
   std::printf("%s %s\n", ok ? "pass" : "FAIL", what);
+  std::fflush(stdout);
   if (!ok) failures++;
 }
 
@@ -104,7 +105,7 @@ int main() {
   // The library's errors surface as exceptions (the reference asserts).
   bool threw = false;
   try {
-    std::vector<std::complex<double>> c(grid.ComplexCoefficientSize(L, 0));
+    std::vector<std::complex<double>> c(grid.ComplexCoefficientSize(L, 3));
     std::vector<std::complex<double>> ff(grid.FieldSize());
     grid.InverseTransformation(L, 3, c, ff);
   } catch (const Error& err) {
