@@ -365,7 +365,11 @@ constexpr int kAnalBf16MaxBlocks = 40;  // colatitude blocks: L <= 2559
 
 template <int NS, int REAL>
 __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
-  __shared__ __bf16 Dl[2][NS][64][128];
+  // Rows of 128 colatitudes padded to 136 (272 bytes): the 16 lanes of a
+  // 16-byte A-operand read hit 16 consecutive rows, which then start 4 banks
+  // apart and cover all 64 banks once (256-byte rows would all map to the
+  // same 4 banks).
+  __shared__ __bf16 Dl[2][NS][64][136];
   __shared__ double2 st[kAnalBf16MaxBlocks][64];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
