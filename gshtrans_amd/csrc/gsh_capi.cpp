@@ -21,6 +21,7 @@
 #include <string>
 #include <thread>
 #include <tuple>
+#include <utility>
 #include <vector>
 
 #include "../../include/gshtrans_amd.h"
@@ -145,6 +146,16 @@ struct NData {
   double* seeds = nullptr;  // [(m+L)*ntheta + t]
   double4* rec = nullptr;  // {a, b, c, 0} per (m, l), see gsh_internal.h
   long* roff = nullptr;
+  // Recurrence checkpoints at the start of every analysis pass k >= 1
+  // (built on the first analysis with this upper index).
+  double2* ck = nullptr;
+  long* ckoff = nullptr;
+};
+
+// Analysis work list on the device (see gsh::work_item).
+struct WorkList {
+  uint2* d = nullptr;
+  int per = 0;
 };
 
 #define GSH_HIP(call)                                  \
@@ -175,6 +186,8 @@ struct gsh_grid_s {
   // Per (lmax, n, value kind): first 32-degree chunk of each order in the
   // split coefficient buffer (device), and the total chunk count.
   std::map<long, std::pair<long*, long>> cso;
+  // Analysis work lists by (kernel family, lmax, n, real, per-group degree bounds).
+  std::map<std::tuple<int, int, int, int, std::vector<int>>, WorkList> worklists;
   double* stage = nullptr;  // host-call staging
   size_t stage_bytes = 0;
   std::mutex mu;
@@ -338,6 +351,81 @@ int ensure_cso(gsh_grid_s* g, int lmax, int n, int real, long** dev, long* total
   g->cso.emplace(key, std::make_pair(d, acc));
   *dev = d;
   *total = acc;
+  return GSH_OK;
+}
+
+// Checkpoints of every order of upper index n (gsh::rec_checkpoints): per
+// order, one row of ntheta states per pass k = 1 .. npass(L) - 1.
+int ensure_ck(gsh_grid_s* g, NData* d, int n, hipStream_t s) {
+  if (d->ck) return GSH_OK;
+  const int L = g->L, nt = g->ntheta();
+  std::vector<long> off(2 * L + 1);
+  long rows = 0;
+  for (int m = -L; m <= L; m++) {
+    off[m + L] = rows;
+    const int l0 = std::max(std::abs(m), std::abs(n));
+    rows += std::max(0, (L - l0 + gsh::kPass) / gsh::kPass - 1);
+  }
+  double2* ck = nullptr;
+  long* ckoff = nullptr;
+  if (hipMalloc(&ck, std::max(rows, 1L) * nt * sizeof(double2)) != hipSuccess)
+    return GSH_ERR_OUT_OF_MEMORY;
+  GSH_HIP(hipMalloc(&ckoff, off.size() * sizeof(long)));
+  GSH_HIP(hipMemcpy(ckoff, off.data(), off.size() * sizeof(long), hipMemcpyHostToDevice));
+  LegParams p{};
+  p.L = L;
+  p.ntheta = nt;
+  p.n = n;
+  p.nabs = std::abs(n);
+  p.cosv = g->d_cos;
+  p.seeds = d->seeds;
+  p.rec = d->rec;
+  p.roff = d->roff;
+  p.ck = ck;
+  p.ckoff = ckoff;
+  GSH_HIP(gsh::launch_rec_checkpoints(p, s));
+  GSH_HIP(hipStreamSynchronize(s));
+  d->ck = ck;
+  d->ckoff = ckoff;
+  return GSH_OK;
+}
+
+int order_of_host(int mw, int real) { return real ? mw : ((mw & 1) ? -((mw + 1) >> 1) : (mw >> 1)); }
+
+// Work list of an analysis launch: items (mw, pass | group << 16) ordered by
+// order, group, pass (the passes of one order and group share their input
+// blocks), padded to a multiple of 8 with (~0u, 0); glmax[g] bounds the
+// degrees of group g (empty: lmax for every group).
+int ensure_worklist(gsh_grid_s* g, int family, int lmax, int n, int real, int ngroups,
+                    const std::vector<int>& glmax, WorkList* out) {
+  auto key = std::make_tuple(family, lmax, n, real,
+                             glmax.empty() ? std::vector<int>{-ngroups} : glmax);
+  auto it = g->worklists.find(key);
+  if (it != g->worklists.end()) {
+    *out = it->second;
+    return GSH_OK;
+  }
+  const int nslot = real ? lmax + 1 : 2 * lmax + 1;
+  std::vector<uint2> items;
+  for (int mw = 0; mw < nslot; mw++) {
+    const int m = order_of_host(mw, real);
+    const int l0 = std::max(std::abs(m), std::abs(n));
+    for (int grp = 0; grp < ngroups; grp++) {
+      const int lw = glmax.empty() ? lmax : glmax[grp];
+      if (l0 > lw) continue;
+      const int np = (lw - l0 + gsh::kPass) / gsh::kPass;
+      for (int k = 0; k < np; k++)
+        items.push_back(make_uint2(static_cast<unsigned>(mw),
+                                   static_cast<unsigned>(k) | (static_cast<unsigned>(grp) << 16)));
+    }
+  }
+  while (items.size() % 8 != 0 || items.empty()) items.push_back(make_uint2(~0u, 0u));
+  WorkList w;
+  w.per = static_cast<int>(items.size() / 8);
+  GSH_HIP(hipMalloc(&w.d, items.size() * sizeof(uint2)));
+  GSH_HIP(hipMemcpy(w.d, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  g->worklists.emplace(key, w);
+  *out = w;
   return GSH_OK;
 }
 
@@ -589,7 +677,10 @@ int gsh_grid_destroy(gsh_grid_t g) {
     (void)hipFree(kv.second.seeds);
     (void)hipFree(kv.second.rec);
     (void)hipFree(kv.second.roff);
+    if (kv.second.ck) (void)hipFree(kv.second.ck);
+    if (kv.second.ckoff) (void)hipFree(kv.second.ckoff);
   }
+  for (auto& kv : g->worklists) (void)hipFree(kv.second.d);
   (void)hipFree(g->d_cos);
   (void)hipFree(g->d_wq);
   (void)hipFree(g->d_one);
@@ -644,6 +735,7 @@ struct RunOpts {
   const int* fmap = nullptr;     // device: column field -> caller's field index
   const int* f_lmax = nullptr;   // device: truncation degree per column field
   const int* cg_lmax = nullptr;  // device: max f_lmax per 16-column group
+  std::vector<int> cg_lmax_host;  // the same on the host (work lists)
   int f32 = 0;                   // fields stored in FP32
   int synth_tiles = 0;           // FP64 synthesis: 16-column tiles per workgroup (0: auto)
   int fft = 0;                   // 0: auto, 1: wavefront-per-ring kernel, 2: LDS kernel
@@ -730,14 +822,36 @@ int run_forward(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
   const size_t fa = (size_t)nslot * ((ncol + 15) / 16) * ((g->ntheta() + 3) / 4) * 64;
   if ((st = ensure_work(g, sizeof(double) * fa))) return st;
   q.F = p.F = g->work;
+  if ((st = ensure_ck(g, d, n, s))) return st;
+  p.ck = d->ck;
+  p.ckoff = d->ckoff;
+  const int ncg16 = (ncol + 15) / 16;
   GSH_HIP(timed(g, GSH_KERNEL_FFT_ANALYSIS, s, [&] {
     return use_wave_fft(q, o) ? gsh::launch_fft_anal_wave(q, s) : gsh::launch_fft_anal(q, s);
   }));
   if (precision == GSH_FP64 || precision == GSH_FP32) {
+    WorkList w;
+    if ((st = ensure_worklist(g, 0, lmax, n, real, ncg16, o.cg_lmax_host, &w))) return st;
+    p.items = w.d;
+    p.items_per = w.per;
     GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_ANALYSIS, s, [&] {
       return gsh::launch_leg_anal_f64(p, s, precision == GSH_FP32);
     }));
     return GSH_OK;
+  }
+  {
+    // 128-column super-groups of the split-BF16 kernel.
+    const int nsg = (ncg16 + 7) / 8;
+    std::vector<int> sg;
+    for (int k = 0; k < nsg && !o.cg_lmax_host.empty(); k++) {
+      int mx = 0;
+      for (int c = 8 * k; c < std::min(8 * k + 8, ncg16); c++) mx = std::max(mx, o.cg_lmax_host[c]);
+      sg.push_back(mx);
+    }
+    WorkList w;
+    if ((st = ensure_worklist(g, 1, lmax, n, real, nsg, sg, &w))) return st;
+    p.items = w.d;
+    p.items_per = w.per;
   }
   const int ns = precision == GSH_BF16X6 ? 3 : 2;
   const int nkc = (g->ntheta() + 31) / 32;
@@ -933,7 +1047,7 @@ int gsh_forward_host(gsh_grid_t g, int lmax, int n, int value_kind, const double
 // ---------------------------------------------------------------------------
 struct PlanGroup {
   int n = 0, lmax = 0, count = 0;
-  std::vector<int> fmap, flmax;
+  std::vector<int> fmap, flmax, cglmax;
   int* d_fmap = nullptr;
   int* d_flmax = nullptr;
   int* d_cglmax = nullptr;
@@ -990,6 +1104,7 @@ RunOpts group_opts(const gsh_plan_s* P, const PlanGroup& G, int dir) {
   o.fmap = G.d_fmap;
   o.f_lmax = G.d_flmax;
   o.cg_lmax = G.d_cglmax;
+  o.cg_lmax_host = G.cglmax;
   o.f32 = P->storage == GSH_STORE_FP32;
   o.synth_tiles = G.synth_tiles;
   o.fft = dir == 0 ? G.fft_inv : G.fft_fwd;
@@ -1189,6 +1304,7 @@ int gsh_plan_create(gsh_grid_t g, int count, const int* lmax, const int* n, int 
     const int ncg = (2 * G.count + 15) / 16;
     std::vector<int> cg(ncg);
     for (int c = 0; c < ncg; c++) cg[c] = G.flmax[8 * c];  // descending: first is largest
+    G.cglmax = cg;
     if ((st = ensure_ndata(g, G.n, &G.nd))) return st;
     GSH_HIP(hipMalloc(&G.d_fmap, G.count * sizeof(int)));
     GSH_HIP(hipMalloc(&G.d_flmax, G.count * sizeof(int)));

@@ -12,6 +12,12 @@ namespace gsh {
 // fixed-size chunks of the l-loop may step past L without bounds checks.
 constexpr int kRecPad = 64;
 
+// Analysis passes: the degree range of an order is cut into passes of
+// kPass degrees that run in separate workgroups; the recurrence state at the
+// start of every pass k >= 1 comes from a checkpoint table computed once per
+// grid and upper index (rec_checkpoints).
+constexpr int kPass = 64;
+
 // Parameters of one Legendre-contraction launch (synthesis, analysis or
 // adjoint) for a batch of fields sharing (lmax, n, value kind).
 //
@@ -72,7 +78,17 @@ struct LegParams {
   const int* fmap;      // column field f -> index of the field in the caller's arrays
   const int* f_lmax;    // truncation degree of column field f (<= lmax)
   const int* cg_lmax;   // max of f_lmax over each 16-column group
+  // Analysis work list: items[(b % 8) * items_per + b / 8] = (mw, pass |
+  // group << 16) for workgroup b, x = ~0u for padding; see work_item().
+  const uint2* items;
+  int items_per;
+  // Recurrence checkpoints: state (e_l, e_{l-1}) at l = l0 + kPass k of order
+  // m, pass k >= 1, colatitude t: ck[(ckoff[m + L] + k - 1) * ntheta + t].
+  const double2* ck;
+  const long* ckoff;
 };
+
+
 
 // Truncation degree and caller index of column field f.
 __device__ __forceinline__ int field_lmax(const LegParams& p, int f) {
@@ -94,6 +110,17 @@ __host__ __device__ inline long f_index(int theta, int slot, int col, int nslot,
 template <class T>
 __device__ __forceinline__ T ld_uniform(const T* p, long i) {
   return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
+// Work item of this workgroup.  Workgroups are dispatched to the device's 8
+// XCDs round-robin (workgroup b on XCD b % 8), so reading the list as
+// items[(b % 8) * per + b / 8] runs consecutive items on one XCD at about the
+// same time, where they share that XCD's L2.
+__device__ __forceinline__ uint2 work_item(const uint2* items, int per) {
+  const int b = blockIdx.x;
+  const unsigned long long v = ld_uniform(reinterpret_cast<const unsigned long long*>(items),
+                                          (long)(b & 7) * per + (b >> 3));
+  return make_uint2(static_cast<unsigned>(v), static_cast<unsigned>(v >> 32));
 }
 
 // Index of (slot, col, theta) in the analysis work layout FA.
@@ -143,6 +170,9 @@ __host__ __device__ inline long xs_index(int slot, int col, int theta, int s, in
 hipError_t launch_leg_synth_f64(const LegParams& p, hipStream_t s, int tiles = 0,
                                 bool f32m = false);
 hipError_t launch_leg_anal_f64(const LegParams& p, hipStream_t s, bool f32m = false);
+// Recurrence checkpoints of every order of upper index p.n (uses L, ntheta,
+// nabs, cosv, seeds, rec, roff, ck, ckoff).
+hipError_t launch_rec_checkpoints(const LegParams& p, hipStream_t s);
 hipError_t launch_fft_synth(const FFTParams& p, hipStream_t s);
 hipError_t launch_fft_anal(const FFTParams& p, hipStream_t s);
 // Wavefront-per-ring FFTs (gsh_fft_wave.hip) for N in {512, 1024, 2048}.

@@ -250,24 +250,39 @@ __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
   __shared__ double tr[4][16][66];  // also holds the cross-wave reduction
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-  const int cg = blockIdx.x;
-  const int mw = blockIdx.y;
+  // Work item: (order, pass, column group).
+  const uint2 it = work_item(p.items, p.items_per);
+  if (it.x == ~0u) return;
+  const int mw = it.x;
+  const int pass = it.y & 0xffff;
+  const int cg = it.y >> 16;
   const int m = order_of(mw, p.real);
   const int l0 = max(abs(m), p.nabs);
   const int lmax = p.lmax;
   const int slot = m - p.m_first;
   const int krow = lane >> 4;
   const int col = cg * 16 + (lane & 15);
-  // Recurrence coefficients of a 64-degree pass (entries lc+1 .. lc+64,
-  // 2 KiB) are staged in LDS, double-buffered by pass parity: threads
-  // 0..127 fetch the next pass's 16 bytes each at the start of a pass and
-  // store them before the pass's last barrier.  Entries up to L + kRecPad
-  // exist.
-  __shared__ uint4 rsa[2][128];
+  const int lc = l0 + kPass * pass;  // first degree of the pass
+  // Recurrence coefficients of the pass (entries lc+1 .. lc+64, 2 KiB),
+  // staged in LDS.  Entries up to L + kRecPad exist.
+  __shared__ uint4 rsa[128];
   const uint4* recq = reinterpret_cast<const uint4*>(p.rec + (ld_uniform(p.roff, m + p.L) - l0));
   const int tid = threadIdx.x;
-  if (tid < 128) rsa[0][tid] = recq[2 * (l0 + 1) + tid];
+  if (tid < 128) rsa[tid] = recq[2 * (lc + 1) + tid];
   __syncthreads();
+  const d4* rl = reinterpret_cast<const d4*>(&rsa[0]);  // entry lc + 1 + j
+  // Recurrence state at degree lc: the seed for pass 0, else the checkpoint.
+  const double2* ckp = p.ck + (ld_uniform(p.ckoff, m + p.L) + pass - 1) * (long)p.ntheta;
+  auto state = [&](int t, double& e0_, double& e1_) {
+    if (pass == 0) {
+      e0_ = p.seeds[(long)(m + p.L) * p.ntheta + t];
+      e1_ = 0.0;
+    } else {
+      const double2 s2 = ckp[t];
+      e0_ = s2.x;
+      e1_ = s2.y;
+    }
+  };
   const bool cv = col < p.ncol;
   const int lf = field_lmax(p, cv ? col >> 1 : 0);  // this column's truncation degree
   const long cbase = (long)field_index(p.fmap, cv ? col >> 1 : 0) * p.cstride;
@@ -290,8 +305,7 @@ __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
   for (int bi = 0; bi < MB; bi++) {
     const int t = 64 * (wave + 4 * bi) + lane;
     cs[bi] = p.cosv[t];
-    e0[bi] = p.seeds[(long)(m + p.L) * nt + t];
-    e1[bi] = 0.0;
+    state(t, e0[bi], e1[bi]);
   }
   // Tail: quads 4*w + 16*i + {0..3} after the full blocks; lane j covers
   // quad i = j/4, colatitude tail0 + 16*(j/4) + 4*wave + j%4.
@@ -301,10 +315,12 @@ __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
   const int tt = tail0 + 16 * (lane >> 2) + 4 * wave + (lane & 3);
   const bool ttv = tt < nt;
   const int ttc = min(tt, nt - 1);
-  const double cs0 = p.cosv[ttc], s0 = p.seeds[(long)(m + p.L) * nt + ttc];
+  double e0s, e1s;
+  state(ttc, e0s, e1s);
+  const double cs0 = p.cosv[ttc];
   double cst = ttv ? cs0 : 0.0;
-  double e0t = ttv ? s0 : 0.0;
-  double e1t = 0.0;
+  double e0t = ttv ? e0s : 0.0;
+  double e1t = ttv ? e1s : 0.0;
 
   auto loadBlk = [&](int bi, double* Bq) {
 #pragma unroll
@@ -321,11 +337,8 @@ __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
     }
   };
 
-  for (int lc = l0; lc <= lwg; lc += 16 * kAnalLT) {
-    const d4* rl = reinterpret_cast<const d4*>(&rsa[((lc - l0) >> 6) & 1][0]);
-    const bool more = lc + 64 <= lwg;
-    uint4 rqn = make_uint4(0, 0, 0, 0);
-    if (tid < 128 && more) rqn = recq[2 * (lc + 65) + tid];
+  (void)lwg;
+  {
     typename M::acc_t acc[kAnalLT];
 #pragma unroll
     for (int lt = 0; lt < kAnalLT; lt++) acc[lt] = M::zero();
@@ -394,18 +407,47 @@ __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
         p.cout[ci] = v;
       }
     }
-    if (tid < 128 && more) rsa[(((lc - l0) >> 6) + 1) & 1][tid] = rqn;
-    __syncthreads();
   }
 }
 
 template <int MB>
 hipError_t launch_anal(const LegParams& p, hipStream_t s, bool f32m) {
+  const dim3 grid(8 * p.items_per);
   if (f32m)
-    hipLaunchKernelGGL((leg_anal_f64<MB, true>), dim3(p.n_cg, p.nslot), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((leg_anal_f64<MB, true>), grid, dim3(256), 0, s, p);
   else
-    hipLaunchKernelGGL((leg_anal_f64<MB, false>), dim3(p.n_cg, p.nslot), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((leg_anal_f64<MB, false>), grid, dim3(256), 0, s, p);
   return hipGetLastError();
+}
+
+// Recurrence checkpoints: order m (blockIdx.y = m + L) runs its recurrence
+// for 256 colatitudes per workgroup from the seed at l0 and stores the state
+// (e_l, e_{l-1}) at l = l0 + kPass k, k = 1 .. npass - 1, with the same
+// arithmetic as the transform kernels.  The pass's coefficients are staged
+// in LDS (all threads of the workgroup share the order).
+__global__ __launch_bounds__(256) void rec_checkpoints(LegParams p) {
+  const int mi = blockIdx.y;
+  const int m = mi - p.L;
+  const int l0 = max(abs(m), p.nabs);
+  const int nt = p.ntheta;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int tc = min(t, nt - 1);
+  const double cs = p.cosv[tc];
+  double e0 = p.seeds[(long)mi * nt + tc], e1 = 0.0;
+  const int npass = (p.L - l0 + kPass) / kPass;
+  __shared__ uint4 rs[2 * kPass];
+  const uint4* recq = reinterpret_cast<const uint4*>(p.rec + (ld_uniform(p.roff, mi) - l0));
+  double2* out = const_cast<double2*>(p.ck) + ld_uniform(p.ckoff, mi) * (long)nt;
+  for (int k = 1; k < npass; k++) {
+    const int lp = l0 + kPass * (k - 1);
+    __syncthreads();
+    if (threadIdx.x < 2 * kPass) rs[threadIdx.x] = recq[2 * (lp + 1) + threadIdx.x];
+    __syncthreads();
+    const d4* rl = reinterpret_cast<const d4*>(rs);
+#pragma unroll 8
+    for (int j = 0; j < kPass; j++) rec_step(rl[j], cs, e0, e1);
+    if (t < nt) out[(long)(k - 1) * nt + t] = make_double2(e0, e1);
+  }
 }
 
 template <bool F32M>
@@ -423,6 +465,12 @@ hipError_t launch_synth(const LegParams& p, int T, hipStream_t s) {
 }
 
 }  // namespace
+
+hipError_t launch_rec_checkpoints(const LegParams& p, hipStream_t s) {
+  hipLaunchKernelGGL(rec_checkpoints, dim3((p.ntheta + 255) / 256, 2 * p.L + 1), dim3(256), 0, s,
+                     p);
+  return hipGetLastError();
+}
 
 hipError_t launch_leg_synth_f64(const LegParams& p0, hipStream_t s, int tiles, bool f32m) {
   LegParams p = p0;

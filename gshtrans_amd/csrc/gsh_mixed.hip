@@ -373,35 +373,43 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   __shared__ double2 st[kAnalBf16MaxBlocks][64];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-  const int csg = blockIdx.x;
-  const int mw = blockIdx.y;
+  // Work item: (order, pass, column super-group).
+  const uint2 it = work_item(p.items, p.items_per);
+  if (it.x == ~0u) return;
+  const int mw = it.x;
+  const int pass = it.y & 0xffff;
+  const int csg = it.y >> 16;
   const int m = order_of(mw, REAL);
   const int slot = m - p.m_first;
   const int l0 = max(abs(m), p.nabs);
-  const int lmax = p.lmax;
   const int nt = p.ntheta;
   const int nblk = (nt + 63) >> 6;
   const int ngrp = (nblk + 1) >> 1;
-  const double* seeds = p.seeds + (long)(m + p.L) * nt;
-  // Recurrence coefficients of a pass (entries lp+1 .. lp+64, 2 KiB) are
-  // staged in LDS, double-buffered by pass parity: threads 0..127 fetch the
-  // next pass's 16 bytes each at the start of a pass and store them before
-  // the pass's last barrier.  Entries up to L + kRecPad exist.
-  __shared__ uint4 rsa[2][128];
+  const int lp = l0 + kPass * pass;  // first degree of the pass
+  // Recurrence coefficients of the pass (entries lp+1 .. lp+64, 2 KiB),
+  // staged in LDS.  Entries up to L + kRecPad exist.
+  __shared__ uint4 rsa[128];
   const uint4* recq = reinterpret_cast<const uint4*>(p.rec + (ld_uniform(p.roff, m + p.L) - l0));
   const int tid = threadIdx.x;
-  if (tid < 128) rsa[0][tid] = recq[2 * (l0 + 1) + tid];
+  if (tid < 128) rsa[tid] = recq[2 * (lp + 1) + tid];
 
-  // Initial recurrence state of every block.
+  // Recurrence state of every block at degree lp: the seed for pass 0, else
+  // the checkpoint.
+  const double2* ckp = p.ck + (ld_uniform(p.ckoff, m + p.L) + pass - 1) * (long)nt;
   for (int b = wave; b < nblk; b += 8) {
     const int t = b * 64 + lane;
-    const double s0 = seeds[min(t, nt - 1)];
-    st[b][lane] = make_double2(t < nt ? s0 : 0.0, 0.0);
+    const int tc = min(t, nt - 1);
+    double2 s2;
+    if (pass == 0)
+      s2 = make_double2(p.seeds[(long)(m + p.L) * nt + tc], 0.0);
+    else
+      s2 = ckp[tc];
+    st[b][lane] = t < nt ? s2 : make_double2(0.0, 0.0);
   }
 
   // Produce the bf16 planes of block b (degrees lp..lp+63) into buffer half bb.
-  auto produce = [&](int b, int lp, int buf, int bb) {
-    const d4* rl = reinterpret_cast<const d4*>(&rsa[((lp - l0) >> 6) & 1][0]);  // entry lp+1+j
+  auto produce = [&](int b, int buf, int bb) {
+    const d4* rl = reinterpret_cast<const d4*>(&rsa[0]);  // entry lp+1+j
     const int t = b * 64 + lane;
     const double cs0 = p.cosv[min(t, nt - 1)];
     const double cs = t < nt ? cs0 : 0.0;
@@ -425,11 +433,11 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
     st[b][lane] = make_double2(e0, e1);
   };
   // Blocks past the last colatitude are never read (their K chunks are >= nkc).
-  auto produce_group = [&](int g, int lp, int buf) {
+  auto produce_group = [&](int g, int buf) {
 #pragma unroll
     for (int bb = 0; bb < 2; bb++) {
       const int b = 2 * g + bb;
-      if ((b & 7) == wave && b < nblk) produce(b, lp, buf, bb);
+      if ((b & 7) == wave && b < nblk) produce(b, buf, bb);
     }
   };
 
@@ -451,25 +459,21 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
     }
   };
 
-  __syncthreads();
-  for (int lp = l0; lp <= lmax; lp += 64) {
+  {
     d4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
       for (int j = 0; j < 2; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-    produce_group(0, lp, 0);
+    __syncthreads();  // rsa and st
+    produce_group(0, 0);
     __syncthreads();
     // B operands are loaded one K chunk ahead of their use.
     uint4 B[2][NS];
     loadB(0, B);
-    const bool more = lp + 64 <= lmax;
-    uint4 rqn = make_uint4(0, 0, 0, 0);
-    if (tid < 128 && more) rqn = recq[2 * (lp + 65) + tid];
     for (int g = 0; g < ngrp; g++) {
-      if (g == ngrp - 1 && tid < 128 && more) rsa[(((lp - l0) >> 6) + 1) & 1][tid] = rqn;
-      if (g + 1 < ngrp) produce_group(g + 1, lp, (g + 1) & 1);
+      if (g + 1 < ngrp) produce_group(g + 1, (g + 1) & 1);
       const int buf = g & 1;
       auto readA = [&](int kq, uint4 (&A)[2][NS]) {
 #pragma unroll
@@ -604,7 +608,7 @@ template <int NS>
 hipError_t launch_anal_bf16_t(const LegParams& p0, hipStream_t s) {
   LegParams p = p0;
   p.n_cg = (p.ntile + 7) / 8;
-  const dim3 grid(p.n_cg, p.nslot);  // (column group, order)
+  const dim3 grid(8 * p.items_per);  // work list: (order, pass, column super-group)
   if (p.real)
     hipLaunchKernelGGL((leg_anal_bf16<NS, 1>), grid, dim3(512), 0, s, p);
   else
