@@ -172,6 +172,7 @@ struct gsh_grid_s {
   double* d_cos = nullptr;
   double* d_wq = nullptr;
   double* d_one = nullptr;     // unit colatitude weights (adjoint of the synthesis)
+  uint4* d_bzero = nullptr;    // zero B operand (3 planes x 64 lanes) of masked MFMA tiles
   double2* d_tw = nullptr;
   double2* d_twa = nullptr;    // wavefront FFT twiddles (N in {512, 1024, 2048})
   double2* d_chirp = nullptr;  // Bluestein tables (ring length not a power of two)
@@ -429,6 +430,35 @@ int ensure_worklist(gsh_grid_s* g, int family, int lmax, int n, int real, int ng
   return GSH_OK;
 }
 
+// Work list of a synthesis launch: every (order, column group, colatitude
+// super-block) item, ordered so that the super-blocks of one order and group
+// (which read the same coefficients) run together on one XCD.
+int ensure_synth_worklist(gsh_grid_s* g, int family, int lmax, int real, int ngroups, int ntsb,
+                          WorkList* out) {
+  auto key = std::make_tuple(family, lmax, 0, real, std::vector<int>{ngroups, ntsb});
+  auto it = g->worklists.find(key);
+  if (it != g->worklists.end()) {
+    *out = it->second;
+    return GSH_OK;
+  }
+  const int nslot = real ? lmax + 1 : 2 * lmax + 1;
+  std::vector<uint2> items;
+  items.reserve((size_t)nslot * ngroups * ntsb + 8);
+  for (int mw = 0; mw < nslot; mw++)
+    for (int grp = 0; grp < ngroups; grp++)
+      for (int t = 0; t < ntsb; t++)
+        items.push_back(make_uint2(static_cast<unsigned>(mw),
+                                   static_cast<unsigned>(t) | (static_cast<unsigned>(grp) << 16)));
+  while (items.size() % 8 != 0 || items.empty()) items.push_back(make_uint2(~0u, 0u));
+  WorkList w;
+  w.per = static_cast<int>(items.size() / 8);
+  GSH_HIP(hipMalloc(&w.d, items.size() * sizeof(uint2)));
+  GSH_HIP(hipMemcpy(w.d, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  g->worklists.emplace(key, w);
+  *out = w;
+  return GSH_OK;
+}
+
 int check_call(const gsh_grid_s* g, int lmax, int n, int value_kind, int precision, int storage,
                int batch) {
   if (!g) return GSH_ERR_INVALID_ARGUMENT;
@@ -459,6 +489,7 @@ LegParams base_params(gsh_grid_s* g, const NData* d, int lmax, int n, int real, 
   p.rec = d->rec;
   p.roff = d->roff;
   p.F = g->work;
+  p.bzero = g->d_bzero;
   return p;
 }
 
@@ -659,6 +690,8 @@ int gsh_grid_create(int lmax, int nmax, int mrange, int nrange, gsh_grid_t* grid
   GSH_HIP(hipMalloc(&g->d_tw, tw.size() * sizeof(double2)));
   GSH_HIP(hipMemcpy(g->d_cos, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice));
   GSH_HIP(hipMemcpy(g->d_wq, wq.data(), wq.size() * sizeof(double), hipMemcpyHostToDevice));
+  GSH_HIP(hipMalloc(&g->d_bzero, 3 * 64 * sizeof(uint4)));
+  GSH_HIP(hipMemset(g->d_bzero, 0, 3 * 64 * sizeof(uint4)));
   {
     const std::vector<double> one(lmax + 1, 1.0);
     GSH_HIP(hipMalloc(&g->d_one, one.size() * sizeof(double)));
@@ -684,6 +717,7 @@ int gsh_grid_destroy(gsh_grid_t g) {
   (void)hipFree(g->d_cos);
   (void)hipFree(g->d_wq);
   (void)hipFree(g->d_one);
+  (void)hipFree(g->d_bzero);
   (void)hipFree(g->d_tw);
   (void)hipFree(g->d_twa);
   if (g->d_chirp) (void)hipFree(g->d_chirp);
@@ -760,11 +794,24 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
   p.fmap = o.fmap;
   p.f_lmax = o.f_lmax;
   p.cg_lmax = o.cg_lmax;
+  const int ntsb = (g->ntheta() + 255) / 256;
   if (precision == GSH_FP64 || precision == GSH_FP32) {
+    const int T = o.synth_tiles == 1 || o.synth_tiles == 2 ? o.synth_tiles : (p.ncol > 16 ? 2 : 1);
+    WorkList w;
+    if ((st = ensure_synth_worklist(g, 10 + T, lmax, real, (p.ncol + 16 * T - 1) / (16 * T), ntsb,
+                                    &w)))
+      return st;
+    p.items = w.d;
+    p.items_per = w.per;
     GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_SYNTHESIS, s, [&] {
-      return gsh::launch_leg_synth_f64(p, s, o.synth_tiles, precision == GSH_FP32);
+      return gsh::launch_leg_synth_f64(p, s, T, precision == GSH_FP32);
     }));
   } else {
+    WorkList w;
+    if ((st = ensure_synth_worklist(g, 20, lmax, real, ((p.ncol + 15) / 16 + 7) / 8, ntsb, &w)))
+      return st;
+    p.items = w.d;
+    p.items_per = w.per;
     const int ns = precision == GSH_BF16X6 ? 3 : 2;
     long* cso = nullptr;
     long nchunks = 0;

@@ -70,6 +70,7 @@ struct LegParams {
   //   chunks, block (slot, kchunk, tile, split).
   const uint4* bs;
   uint4* bs_out;
+  const uint4* bzero;   // 3 x 64 zero uint4: the B operand of masked tiles
   const long* cso;      // per slot: first chunk of that order in bs
   int ntile;            // 16-column tiles, ceil(ncol / 16)
   int nkc;              // analysis: 32-colatitude chunks, ceil(ntheta / 32)

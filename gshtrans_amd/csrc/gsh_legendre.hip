@@ -104,9 +104,12 @@ __global__ __launch_bounds__(256) void leg_synth_f64(LegParams p) {
   __shared__ double tr[4][4][66];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-  const int tsb = blockIdx.x;
-  const int cg = blockIdx.y;
-  const int mw = blockIdx.z;
+  // Work item: (order, column group, colatitude super-block).
+  const uint2 it = work_item(p.items, p.items_per);
+  if (it.x == ~0u) return;
+  const int mw = it.x;
+  const int tsb = it.y & 0xffff;
+  const int cg = it.y >> 16;
   const int m = order_of(mw, REAL);
   const int slot = m - p.m_first;
   const int l0 = max(abs(m), p.nabs);
@@ -452,7 +455,7 @@ __global__ __launch_bounds__(256) void rec_checkpoints(LegParams p) {
 
 template <bool F32M>
 hipError_t launch_synth(const LegParams& p, int T, hipStream_t s) {
-  const dim3 nblocks(p.n_tsb, p.n_cg, p.nslot);  // (theta block, column group, order)
+  const dim3 nblocks(8 * p.items_per);  // work list: (order, column group, theta block)
   if (T == 2 && p.real)
     hipLaunchKernelGGL((leg_synth_f64<2, 1, F32M>), nblocks, dim3(256), 0, s, p);
   else if (T == 2)

@@ -220,9 +220,12 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
   __shared__ uint4 trb[2][NS][4][256];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-  const int tsb = blockIdx.x;
-  const int csg = blockIdx.y;
-  const int mw = blockIdx.z;
+  // Work item: (order, column super-group, colatitude super-block).
+  const uint2 it = work_item(p.items, p.items_per);
+  if (it.x == ~0u) return;
+  const int mw = it.x;
+  const int tsb = it.y & 0xffff;
+  const int csg = it.y >> 16;
   const int m = order_of(mw, REAL);
   const int slot = m - p.m_first;
   const int l0 = max(abs(m), p.nabs);
@@ -282,17 +285,41 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
   // Consumer role.
   const int h = wave >> 2;
   const int tile0 = csg * 8 + (wave & 3) * 2;
-  const bool tv0 = tile0 < p.ntile, tv1 = tile0 + 1 < p.ntile;
-  const uint4* bbase = p.bs + (ld_uniform(p.cso, slot) * (long)p.ntile) * NS * 64 + lane;
-  auto loadB = [&](int chunk, uint4 (&B)[2][NS]) {
-    const long c = (long)min(chunk, nchunk - 1) * p.ntile;
+  // B operands of a chunk (8 column tiles x NS planes x 64 lanes) are staged
+  // in LDS, double-buffered, with direct global-to-LDS loads: every wave
+  // copies NS 1-KiB pieces of the next chunk at the start of a chunk (no
+  // registers involved) and waits for them before the chunk's barrier, so
+  // the copies have a whole chunk of MFMAs to complete.  Tiles past ntile
+  // read the zero block p.bzero.
+  __shared__ uint4 bsh[2][8][NS][64];
+  const uint4* bbase = p.bs + (ld_uniform(p.cso, slot) * (long)p.ntile) * NS * 64;
+  const int tid = threadIdx.x;
+  // Source of piece j of this thread (tile-dependent, fixed over chunks):
+  // chunk c reads src[j] + c * step[j].
+  const uint4* src[NS];
+  long step[NS];
+#pragma unroll
+  for (int j = 0; j < NS; j++) {
+    const int i = tid + 512 * j;  // < 8 * NS * 64
+    const int tile = i / (NS * 64), rem = i - tile * (NS * 64);
+    const int gt = csg * 8 + tile;
+    const bool ok = gt < p.ntile;
+    src[j] = ok ? bbase + (long)gt * NS * 64 + rem : p.bzero + rem;
+    step[j] = ok ? (long)p.ntile * NS * 64 : 0;
+  }
+  // Piece j of wave w lands at LDS uint4 512 j + 64 w + lane of the buffer.
+  auto fetchB = [&](int chunk, int buf) {
+    const long c = min(chunk, nchunk - 1);
+#pragma unroll
+    for (int j = 0; j < NS; j++)
+      __builtin_amdgcn_global_load_lds(src[j] + c * step[j],
+                                       &bsh[buf][0][0][0] + 512 * j + 64 * wave, 16, 0, 0);
+  };
+  auto loadB = [&](int buf, uint4 (&B)[2][NS]) {
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-      const uint4 z = make_uint4(0, 0, 0, 0);
-      const uint4 b0 = bbase[((c + (tv0 ? tile0 : 0)) * NS + s) * 64];
-      const uint4 b1 = bbase[((c + (tv1 ? tile0 + 1 : 0)) * NS + s) * 64];
-      B[0][s] = tv0 ? b0 : z;
-      B[1][s] = tv1 ? b1 : z;
+      B[0][s] = bsh[buf][(wave & 3) * 2][s][lane];
+      B[1][s] = bsh[buf][(wave & 3) * 2 + 1][s][lane];
     }
   };
 
@@ -302,17 +329,12 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
 #pragma unroll
     for (int t = 0; t < 2; t++) acc[g][t] = d4{0.0, 0.0, 0.0, 0.0};
 
-  if (prod) produce(0, 0);
-  uint4 B[2][NS];
-  loadB(0, B);
-  __syncthreads();
-  for (int k = 0; k < nchunk; k++) {
-    if (prod && k + 1 < nchunk) produce(k + 1, (k + 1) & 1);
-    uint4 Bn[2][NS];
-    loadB(k + 1, Bn);
+  // One chunk: every row group of this wave, one at a time (two independent
+  // MFMA chains, its two column tiles), flushed to FP64 after each group.
+  auto consume = [&](int k) {
     const int buf = k & 1;
-    // One row group at a time: two independent MFMA chains (its two column
-    // tiles), flushed to FP64 after each group.
+    uint4 B[2][NS];
+    loadB(buf, B);
 #pragma unroll
     for (int g = 0; g < 8; g++) {
       if (ngp == 4 || g < 2 * ngp) {
@@ -326,11 +348,17 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
         flush(acc[g][1], c[0][1]);
       }
     }
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-      B[0][s] = Bn[0][s];
-      B[1][s] = Bn[1][s];
-    }
+  };
+
+  if (prod) produce(0, 0);
+  fetchB(0, 0);
+  __builtin_amdgcn_s_waitcnt(0);  // the copies of chunk 0
+  __syncthreads();
+  for (int k = 0; k < nchunk; k++) {
+    fetchB(k + 1, (k + 1) & 1);
+    if (prod && k + 1 < nchunk) produce(k + 1, (k + 1) & 1);
+    consume(k);
+    __builtin_amdgcn_s_waitcnt(0);  // the copies of chunk k + 1
     __syncthreads();
   }
 
@@ -446,16 +474,18 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   const int tile0 = csg * 8 + (wave & 3) * 2;
   const bool tv0 = tile0 < p.ntile, tv1 = tile0 + 1 < p.ntile;
   const uint4* bslot = p.bs + (long)slot * p.nkc * p.ntile * NS * 64 + lane;
+  // Chunks past nkc and tiles past ntile read the zero block p.bzero (an
+  // address choice, so nothing waits on the loads before their MFMAs).
   auto loadB = [&](int kc, uint4 (&B)[2][NS]) {
     const bool kv = kc < p.nkc;
-    const long c = (long)(kv ? kc : 0) * p.ntile;
+    const long c = (long)kc * p.ntile;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-      const uint4 z = make_uint4(0, 0, 0, 0);
-      const uint4 b0 = bslot[((c + (tv0 ? tile0 : 0)) * NS + s) * 64];
-      const uint4 b1 = bslot[((c + (tv1 ? tile0 + 1 : 0)) * NS + s) * 64];
-      B[0][s] = (kv && tv0) ? b0 : z;
-      B[1][s] = (kv && tv1) ? b1 : z;
+      const uint4* a0 = kv && tv0 ? bslot + ((c + tile0) * NS + s) * 64 : p.bzero + s * 64 + lane;
+      const uint4* a1 =
+          kv && tv1 ? bslot + ((c + tile0 + 1) * NS + s) * 64 : p.bzero + s * 64 + lane;
+      B[0][s] = *a0;
+      B[1][s] = *a1;
     }
   };
 
@@ -469,9 +499,10 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
     __syncthreads();  // rsa and st
     produce_group(0, 0);
     __syncthreads();
-    // B operands are loaded one K chunk ahead of their use.
-    uint4 B[2][NS];
-    loadB(0, B);
+    // B operands: ping-pong buffers, each loaded one K chunk ahead of its
+    // use (no register copies); B0 holds the group's first chunk on entry.
+    uint4 B0[2][NS], B1[2][NS];
+    loadB(0, B0);
     for (int g = 0; g < ngrp; g++) {
       if (g + 1 < ngrp) produce_group(g + 1, (g + 1) & 1);
       const int buf = g & 1;
@@ -483,58 +514,43 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
             A[lt][s] = *reinterpret_cast<const uint4*>(
                 &Dl[buf][s][(2 * lh + lt) * 16 + (lane & 15)][kq * 32 + 8 * (lane >> 4)]);
       };
-      const int nkq = min(4, p.nkc - 4 * g);
-      if (nkq == 4) {
-        // The chains of chunk kq are flushed to FP64 while those of kq + 1 run.
-        f4 cp[2][2];
-#pragma unroll
-        for (int kq = 0; kq < 4; kq++) {
-          uint4 Bn[2][NS];
-          loadB(4 * g + kq + 1, Bn);
-          uint4 A[2][NS];
-          readA(kq, A);
-          f4 c[2][2];
-          split_mma<NS, 2>(A, B, c);
-          if (kq > 0) {
-#pragma unroll
-            for (int i = 0; i < 2; i++)
-#pragma unroll
-              for (int j = 0; j < 2; j++) flush(acc[i][j], cp[i][j]);
-          }
+      // One K chunk; its chains are flushed to FP64 after the next chunk's
+      // MFMAs are issued (cp carries them).
+      f4 cp[2][2];
+      auto step = [&](int kq, const uint4 (&B)[2][NS], bool flush_prev) {
+        uint4 A[2][NS];
+        readA(kq, A);
+        f4 c[2][2];
+        split_mma<NS, 2>(A, B, c);
+        if (flush_prev) {
 #pragma unroll
           for (int i = 0; i < 2; i++)
 #pragma unroll
-            for (int j = 0; j < 2; j++) cp[i][j] = c[i][j];
-#pragma unroll
-          for (int s = 0; s < NS; s++) {
-            B[0][s] = Bn[0][s];
-            B[1][s] = Bn[1][s];
-          }
+            for (int j = 0; j < 2; j++) flush(acc[i][j], cp[i][j]);
         }
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
-          for (int j = 0; j < 2; j++) flush(acc[i][j], cp[i][j]);
+          for (int j = 0; j < 2; j++) cp[i][j] = c[i][j];
+      };
+      const int kc0 = 4 * g;
+      // Chunks past nkc multiply the zero block (the last group only).
+      loadB(kc0 + 1, B1);
+      step(0, B0, false);
+      loadB(kc0 + 2, B0);
+      step(1, B1, true);
+      if (kc0 + 2 < p.nkc) {
+        loadB(kc0 + 3, B1);
+        step(2, B0, true);
+        loadB(kc0 + 4, B0);
+        step(3, B1, true);
       } else {
-        // Last, partial group: only the K chunks below nkc.
-        for (int kq = 0; kq < nkq; kq++) {
-          uint4 Bn[2][NS];
-          loadB(4 * g + kq + 1, Bn);
-          uint4 A[2][NS];
-          readA(kq, A);
-          f4 c[2][2];
-          split_mma<NS, 2>(A, B, c);
-#pragma unroll
-          for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int j = 0; j < 2; j++) flush(acc[i][j], c[i][j]);
-#pragma unroll
-          for (int s = 0; s < NS; s++) {
-            B[0][s] = Bn[0][s];
-            B[1][s] = Bn[1][s];
-          }
-        }
+        loadB(kc0 + 4, B0);
       }
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) flush(acc[i][j], cp[i][j]);
       __syncthreads();
     }
 
@@ -592,7 +608,7 @@ hipError_t launch_leg_synth_bf16(const LegParams& p0, int ns, hipStream_t s) {
   LegParams p = p0;
   p.n_tsb = (p.ntheta + 255) / 256;
   p.n_cg = (p.ntile + 7) / 8;  // column super-groups of 8 tiles
-  const dim3 grid(p.n_tsb, p.n_cg, p.nslot);  // (theta block, column group, order)
+  const dim3 grid(8 * p.items_per);  // work list: (order, column super-group, theta block)
   if (ns == 3 && p.real)
     hipLaunchKernelGGL((leg_synth_bf16<3, 1>), grid, dim3(512), 0, s, p);
   else if (ns == 3)
