@@ -459,6 +459,42 @@ int ensure_synth_worklist(gsh_grid_s* g, int family, int lmax, int real, int ngr
   return GSH_OK;
 }
 
+// Work list of the coefficient split: items (slot, chunk | tile << 16)
+// chunk-major (all orders' chunk 0, then chunk 1, ...), padded to a multiple
+// of 32 (8 XCDs x 4 waves per workgroup); per = items / 32.
+int ensure_split_worklist(gsh_grid_s* g, int lmax, int n, int real, int ntile, WorkList* out) {
+  auto key = std::make_tuple(30, lmax, n, real, std::vector<int>{ntile});
+  auto it = g->worklists.find(key);
+  if (it != g->worklists.end()) {
+    *out = it->second;
+    return GSH_OK;
+  }
+  const int nslot = real ? lmax + 1 : 2 * lmax + 1;
+  const int m_first = real ? 0 : -lmax;
+  std::vector<int> nch(nslot);
+  int maxch = 0;
+  for (int sl = 0; sl < nslot; sl++) {
+    const int l0 = std::max(std::abs(sl + m_first), std::abs(n));
+    nch[sl] = (lmax - l0 + 32) / 32;
+    maxch = std::max(maxch, nch[sl]);
+  }
+  std::vector<uint2> items;
+  for (int c = 0; c < maxch; c++)
+    for (int sl = 0; sl < nslot; sl++)
+      if (c < nch[sl])
+        for (int t = 0; t < ntile; t++)
+          items.push_back(make_uint2(static_cast<unsigned>(sl),
+                                     static_cast<unsigned>(c) | (static_cast<unsigned>(t) << 16)));
+  while (items.size() % 32 != 0 || items.empty()) items.push_back(make_uint2(~0u, 0u));
+  WorkList w;
+  w.per = static_cast<int>(items.size() / 32);
+  GSH_HIP(hipMalloc(&w.d, items.size() * sizeof(uint2)));
+  GSH_HIP(hipMemcpy(w.d, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  g->worklists.emplace(key, w);
+  *out = w;
+  return GSH_OK;
+}
+
 int check_call(const gsh_grid_s* g, int lmax, int n, int value_kind, int precision, int storage,
                int batch) {
   if (!g) return GSH_ERR_INVALID_ARGUMENT;
@@ -822,8 +858,16 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
     if ((st = ensure_work2(g, (size_t)nblocks * ns * 64 * 16))) return st;
     p.bs = static_cast<const uint4*>(g->work2);
     p.bs_out = static_cast<uint4*>(g->work2);
+    {
+      WorkList ws;
+      if ((st = ensure_split_worklist(g, lmax, n, real, p.ntile, &ws))) return st;
+      p.items = ws.d;
+      p.items_per = ws.per;
+    }
     GSH_HIP(timed(g, GSH_KERNEL_COEF_SPLIT, s,
                   [&] { return gsh::launch_coef_split(p, ns, nblocks, s); }));
+    p.items = w.d;  // the synthesis work list again
+    p.items_per = w.per;
     GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_SYNTHESIS, s,
                   [&] { return gsh::launch_leg_synth_bf16(p, ns, s); }));
   }

@@ -81,6 +81,13 @@ __device__ __forceinline__ void split_pair(double x0, double x1, unsigned* out) 
   }
 }
 
+// 16-byte direct global-to-LDS copy of every lane: lane i's source lands at
+// lds + i (a wave's 64 pieces are contiguous in LDS).  Kept out of the kernel
+// templates: inside them the host compilation pass drops the kernel stubs.
+__device__ __forceinline__ void copy_to_lds16(const uint4* src, uint4* lds) {
+  __builtin_amdgcn_global_load_lds(src, lds, 16, 0, 0);
+}
+
 __device__ __forceinline__ bf16x8 as_bf8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
 __device__ __forceinline__ f4 mfma(uint4 a, uint4 b, f4 c) {
@@ -124,21 +131,23 @@ __device__ __forceinline__ void flush(d4& acc, const f4 c) {
 // ---------------------------------------------------------------------------
 template <int NS, int REAL>
 __global__ __launch_bounds__(256) void coef_split(LegParams p, long nblocks) {
-  const long b = (long)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (b >= nblocks) return;
+  // Item of this wave, (slot, chunk | tile << 16), from a chunk-major work
+  // list: neighbouring orders -- whose coefficients of one degree share
+  // cache lines -- are split together on one XCD (workgroup b runs on XCD
+  // b % 8 and takes items 4 ((b % 8) per + b / 8) + wave).
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int bw = blockIdx.x;
+  const unsigned long long v = ld_uniform(reinterpret_cast<const unsigned long long*>(p.items),
+                                          ((long)(bw & 7) * p.items_per + (bw >> 3)) * 4 + wave);
+  if (static_cast<unsigned>(v) == ~0u) return;
+  (void)nblocks;
   const int lane = threadIdx.x & 63;
-  const long cb = b / p.ntile;
-  const int tile = (int)(b - cb * p.ntile);
-  // slot: largest s with cso[s] <= cb
-  int lo = 0, hi = p.nslot - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (ld_uniform(p.cso, mid) <= cb) lo = mid; else hi = mid - 1;
-  }
-  const int slot = lo;
+  const int slot = static_cast<int>(static_cast<unsigned>(v));
+  const int chunk = static_cast<int>((v >> 32) & 0xffff);
+  const int tile = static_cast<int>(v >> 48);
+  const long b = (ld_uniform(p.cso, slot) + chunk) * p.ntile + tile;  // output block
   const int m = slot + p.m_first;
   const int l0 = max(abs(m), p.nabs);
-  const int chunk = (int)(cb - ld_uniform(p.cso, slot));
   const int kr = lane >> 4;
   const int col = tile * 16 + (lane & 15);
   const bool cv = col < p.ncol;
@@ -312,8 +321,7 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
     const long c = min(chunk, nchunk - 1);
 #pragma unroll
     for (int j = 0; j < NS; j++)
-      __builtin_amdgcn_global_load_lds(src[j] + c * step[j],
-                                       &bsh[buf][0][0][0] + 512 * j + 64 * wave, 16, 0, 0);
+      copy_to_lds16(src[j] + c * step[j], &bsh[buf][0][0][0] + 512 * j + 64 * wave);
   };
   auto loadB = [&](int buf, uint4 (&B)[2][NS]) {
 #pragma unroll
@@ -580,7 +588,7 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
 }  // namespace
 
 hipError_t launch_coef_split(const LegParams& p, int ns, long nblocks, hipStream_t s) {
-  const dim3 grid((nblocks + 3) / 4);
+  const dim3 grid(8 * p.items_per);  // 4 items (waves) per workgroup
   if (ns == 3 && p.real)
     hipLaunchKernelGGL((coef_split<3, 1>), grid, dim3(256), 0, s, p, nblocks);
   else if (ns == 3)
