@@ -365,7 +365,7 @@ int ensure_ck(gsh_grid_s* g, NData* d, int n, hipStream_t s) {
   for (int m = -L; m <= L; m++) {
     off[m + L] = rows;
     const int l0 = std::max(std::abs(m), std::abs(n));
-    rows += std::max(0, (L - l0 + gsh::kPass) / gsh::kPass - 1);
+    rows += std::max(0, (L - l0 + gsh::kCk) / gsh::kCk - 1);
   }
   double2* ck = nullptr;
   long* ckoff = nullptr;
@@ -431,8 +431,11 @@ int ensure_worklist(gsh_grid_s* g, int family, int lmax, int n, int real, int ng
 }
 
 // Work list of a synthesis launch: every (order, column group, colatitude
-// super-block) item, ordered so that the super-blocks of one order and group
-// (which read the same coefficients) run together on one XCD.
+// super-block) item.  An item of order m sums lmax - |m| + 1 degrees.  Whole
+// units (order, column group) -- all their super-blocks, which read the same
+// coefficients -- go to one XCD, heaviest first onto the least loaded XCD, so
+// the eight XCDs finish together; each XCD's list runs heaviest first and is
+// padded to the longest with empty items.
 int ensure_synth_worklist(gsh_grid_s* g, int family, int lmax, int real, int ngroups, int ntsb,
                           WorkList* out) {
   auto key = std::make_tuple(family, lmax, 0, real, std::vector<int>{ngroups, ntsb});
@@ -442,14 +445,32 @@ int ensure_synth_worklist(gsh_grid_s* g, int family, int lmax, int real, int ngr
     return GSH_OK;
   }
   const int nslot = real ? lmax + 1 : 2 * lmax + 1;
-  std::vector<uint2> items;
-  items.reserve((size_t)nslot * ngroups * ntsb + 8);
+  const int m_first = real ? 0 : -lmax;
+  std::vector<std::pair<int, int>> units;  // (weight, mw * ngroups + grp)
+  units.reserve((size_t)nslot * ngroups);
   for (int mw = 0; mw < nslot; mw++)
     for (int grp = 0; grp < ngroups; grp++)
-      for (int t = 0; t < ntsb; t++)
-        items.push_back(make_uint2(static_cast<unsigned>(mw),
-                                   static_cast<unsigned>(t) | (static_cast<unsigned>(grp) << 16)));
-  while (items.size() % 8 != 0 || items.empty()) items.push_back(make_uint2(~0u, 0u));
+      units.emplace_back(lmax - std::abs(mw + m_first) + 1, mw * ngroups + grp);
+  std::stable_sort(units.begin(), units.end(),
+                   [](const auto& a, const auto& b) { return a.first > b.first; });
+  std::vector<uint2> xcd[8];
+  long load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (const auto& u : units) {
+    const int x = static_cast<int>(std::min_element(load, load + 8) - load);
+    load[x] += static_cast<long>(u.first) * ntsb;
+    const int mw = u.second / ngroups, grp = u.second % ngroups;
+    for (int t = 0; t < ntsb; t++)
+      xcd[x].push_back(make_uint2(static_cast<unsigned>(mw),
+                                  static_cast<unsigned>(t) | (static_cast<unsigned>(grp) << 16)));
+  }
+  size_t per = 1;
+  for (const auto& v : xcd) per = std::max(per, v.size());
+  std::vector<uint2> items;
+  items.reserve(8 * per);
+  for (auto& v : xcd) {
+    v.resize(per, make_uint2(~0u, 0u));
+    items.insert(items.end(), v.begin(), v.end());
+  }
   WorkList w;
   w.per = static_cast<int>(items.size() / 8);
   GSH_HIP(hipMalloc(&w.d, items.size() * sizeof(uint2)));
@@ -852,7 +873,9 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
     long* cso = nullptr;
     long nchunks = 0;
     if ((st = ensure_cso(g, lmax, n, real, &cso, &nchunks))) return st;
-    p.ntile = (p.ncol + 15) / 16;
+    // Column tiles padded to whole super-groups of 8 (zero columns), so the
+    // synthesis reads every tile of its super-group without a bounds check.
+    p.ntile = ((p.ncol + 15) / 16 + 7) / 8 * 8;
     p.cso = cso;
     const long nblocks = nchunks * p.ntile;
     if ((st = ensure_work2(g, (size_t)nblocks * ns * 64 * 16))) return st;

@@ -15,8 +15,11 @@ constexpr int kRecPad = 64;
 // Analysis passes: the degree range of an order is cut into passes of
 // kPass degrees that run in separate workgroups; the recurrence state at the
 // start of every pass k >= 1 comes from a checkpoint table computed once per
-// grid and upper index (rec_checkpoints).
+// grid and upper index (rec_checkpoints), which holds the state every kCk
+// degrees (the split-BF16 analysis starts its recurrences at each of them).
 constexpr int kPass = 64;
+constexpr int kCk = 16;
+static_assert(kPass % kCk == 0, "passes start at checkpoints");
 
 // Parameters of one Legendre-contraction launch (synthesis, analysis or
 // adjoint) for a batch of fields sharing (lmax, n, value kind).
@@ -83,8 +86,9 @@ struct LegParams {
   // group << 16) for workgroup b, x = ~0u for padding; see work_item().
   const uint2* items;
   int items_per;
-  // Recurrence checkpoints: state (e_l, e_{l-1}) at l = l0 + kPass k of order
-  // m, pass k >= 1, colatitude t: ck[(ckoff[m + L] + k - 1) * ntheta + t].
+  // Recurrence checkpoints: state (e_l, e_{l-1}) at l = l0 + kCk k of order
+  // m, k >= 1, colatitude t: ck[(ckoff[m + L] + k - 1) * ntheta + t], for
+  // every l0 + kCk k <= L.
   const double2* ck;
   const long* ckoff;
 };

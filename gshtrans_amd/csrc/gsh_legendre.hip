@@ -275,7 +275,8 @@ __global__ __launch_bounds__(256) void leg_anal_f64(LegParams p) {
   __syncthreads();
   const d4* rl = reinterpret_cast<const d4*>(&rsa[0]);  // entry lc + 1 + j
   // Recurrence state at degree lc: the seed for pass 0, else the checkpoint.
-  const double2* ckp = p.ck + (ld_uniform(p.ckoff, m + p.L) + pass - 1) * (long)p.ntheta;
+  const double2* ckp =
+      p.ck + (ld_uniform(p.ckoff, m + p.L) + (kPass / kCk) * pass - 1) * (long)p.ntheta;
   auto state = [&](int t, double& e0_, double& e1_) {
     if (pass == 0) {
       e0_ = p.seeds[(long)(m + p.L) * p.ntheta + t];
@@ -425,9 +426,9 @@ hipError_t launch_anal(const LegParams& p, hipStream_t s, bool f32m) {
 
 // Recurrence checkpoints: order m (blockIdx.y = m + L) runs its recurrence
 // for 256 colatitudes per workgroup from the seed at l0 and stores the state
-// (e_l, e_{l-1}) at l = l0 + kPass k, k = 1 .. npass - 1, with the same
-// arithmetic as the transform kernels.  The pass's coefficients are staged
-// in LDS (all threads of the workgroup share the order).
+// (e_l, e_{l-1}) at l = l0 + kCk k, k = 1 .. nck - 1, with the same
+// arithmetic as the transform kernels.  The coefficients are staged in LDS
+// 64 degrees at a time (all threads of the workgroup share the order).
 __global__ __launch_bounds__(256) void rec_checkpoints(LegParams p) {
   const int mi = blockIdx.y;
   const int m = mi - p.L;
@@ -437,18 +438,23 @@ __global__ __launch_bounds__(256) void rec_checkpoints(LegParams p) {
   const int tc = min(t, nt - 1);
   const double cs = p.cosv[tc];
   double e0 = p.seeds[(long)mi * nt + tc], e1 = 0.0;
-  const int npass = (p.L - l0 + kPass) / kPass;
-  __shared__ uint4 rs[2 * kPass];
+  const int nck = (p.L - l0 + kCk) / kCk;
+  __shared__ uint4 rs[128];
   const uint4* recq = reinterpret_cast<const uint4*>(p.rec + (ld_uniform(p.roff, mi) - l0));
   double2* out = const_cast<double2*>(p.ck) + ld_uniform(p.ckoff, mi) * (long)nt;
-  for (int k = 1; k < npass; k++) {
-    const int lp = l0 + kPass * (k - 1);
-    __syncthreads();
-    if (threadIdx.x < 2 * kPass) rs[threadIdx.x] = recq[2 * (lp + 1) + threadIdx.x];
-    __syncthreads();
-    const d4* rl = reinterpret_cast<const d4*>(rs);
-#pragma unroll 8
-    for (int j = 0; j < kPass; j++) rec_step(rl[j], cs, e0, e1);
+  // Entries up to L + kRecPad exist, so staging 64 entries past any
+  // checkpoint's start stays in bounds.
+  for (int k = 1; k < nck; k++) {
+    const int lp = l0 + kCk * (k - 1);
+    const int j0 = (k - 1) % (64 / kCk) * kCk;  // position in the staged 64
+    if (j0 == 0) {
+      __syncthreads();
+      if (threadIdx.x < 128) rs[threadIdx.x] = recq[2 * (lp + 1) + threadIdx.x];
+      __syncthreads();
+    }
+    const d4* rl = reinterpret_cast<const d4*>(rs) + j0;
+#pragma unroll
+    for (int j = 0; j < kCk; j++) rec_step(rl[j], cs, e0, e1);
     if (t < nt) out[(long)(k - 1) * nt + t] = make_double2(e0, e1);
   }
 }

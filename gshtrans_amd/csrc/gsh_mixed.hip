@@ -81,13 +81,6 @@ __device__ __forceinline__ void split_pair(double x0, double x1, unsigned* out) 
   }
 }
 
-// 16-byte direct global-to-LDS copy of every lane: lane i's source lands at
-// lds + i (a wave's 64 pieces are contiguous in LDS).  Kept out of the kernel
-// templates: inside them the host compilation pass drops the kernel stubs.
-__device__ __forceinline__ void copy_to_lds16(const uint4* src, uint4* lds) {
-  __builtin_amdgcn_global_load_lds(src, lds, 16, 0, 0);
-}
-
 __device__ __forceinline__ bf16x8 as_bf8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
 __device__ __forceinline__ f4 mfma(uint4 a, uint4 b, f4 c) {
@@ -295,33 +288,31 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
   const int h = wave >> 2;
   const int tile0 = csg * 8 + (wave & 3) * 2;
   // B operands of a chunk (8 column tiles x NS planes x 64 lanes) are staged
-  // in LDS, double-buffered, with direct global-to-LDS loads: every wave
-  // copies NS 1-KiB pieces of the next chunk at the start of a chunk (no
-  // registers involved) and waits for them before the chunk's barrier, so
-  // the copies have a whole chunk of MFMAs to complete.  Tiles past ntile
-  // read the zero block p.bzero.
+  // in LDS, double-buffered: every thread loads NS 16-byte pieces of the next
+  // chunk into registers at the start of a chunk and stores them to LDS
+  // after the chunk's MFMAs, so the loads have a whole chunk to complete and
+  // only the store waits for them.  The split coefficients hold whole
+  // super-groups of 8 tiles (ntile is a multiple of 8).
   __shared__ uint4 bsh[2][8][NS][64];
-  const uint4* bbase = p.bs + (ld_uniform(p.cso, slot) * (long)p.ntile) * NS * 64;
+  const uint4* bbase = p.bs + (ld_uniform(p.cso, slot) * (long)p.ntile + csg * 8) * NS * 64;
   const int tid = threadIdx.x;
-  // Source of piece j of this thread (tile-dependent, fixed over chunks):
-  // chunk c reads src[j] + c * step[j].
-  const uint4* src[NS];
-  long step[NS];
+  const int cstride = p.ntile * NS * 64;  // uint4 per chunk
+  // The 8 tiles of the super-group are contiguous within a chunk (8 NS 64
+  // uint4); piece j of this thread is uint4 tid + 512 j of them, in the
+  // source and in the LDS buffer alike.
+  struct Pieces {
+    uint4 v[NS];
+  };
+  auto fetchB = [&](int chunk) {
+    const uint4* cb = bbase + (long)min(chunk, nchunk - 1) * cstride;
+    Pieces q;
 #pragma unroll
-  for (int j = 0; j < NS; j++) {
-    const int i = tid + 512 * j;  // < 8 * NS * 64
-    const int tile = i / (NS * 64), rem = i - tile * (NS * 64);
-    const int gt = csg * 8 + tile;
-    const bool ok = gt < p.ntile;
-    src[j] = ok ? bbase + (long)gt * NS * 64 + rem : p.bzero + rem;
-    step[j] = ok ? (long)p.ntile * NS * 64 : 0;
-  }
-  // Piece j of wave w lands at LDS uint4 512 j + 64 w + lane of the buffer.
-  auto fetchB = [&](int chunk, int buf) {
-    const long c = min(chunk, nchunk - 1);
+    for (int j = 0; j < NS; j++) q.v[j] = cb[tid + 512 * j];
+    return q;
+  };
+  auto storeB = [&](int buf, const Pieces q) {
 #pragma unroll
-    for (int j = 0; j < NS; j++)
-      copy_to_lds16(src[j] + c * step[j], &bsh[buf][0][0][0] + 512 * j + 64 * wave);
+    for (int j = 0; j < NS; j++) (&bsh[buf][0][0][0])[512 * j + tid] = q.v[j];
   };
   auto loadB = [&](int buf, uint4 (&B)[2][NS]) {
 #pragma unroll
@@ -358,15 +349,17 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
     }
   };
 
-  if (prod) produce(0, 0);
-  fetchB(0, 0);
-  __builtin_amdgcn_s_waitcnt(0);  // the copies of chunk 0
+  {
+    const Pieces q = fetchB(0);
+    if (prod) produce(0, 0);
+    storeB(0, q);
+  }
   __syncthreads();
   for (int k = 0; k < nchunk; k++) {
-    fetchB(k + 1, (k + 1) & 1);
+    const Pieces q = fetchB(k + 1);
     if (prod && k + 1 < nchunk) produce(k + 1, (k + 1) & 1);
     consume(k);
-    __builtin_amdgcn_s_waitcnt(0);  // the copies of chunk k + 1
+    storeB((k + 1) & 1, q);  // buffer (k+1)&1 was last read in chunk k-1
     __syncthreads();
   }
 
@@ -387,26 +380,26 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
 }
 
 // ---------------------------------------------------------------------------
-// Analysis.  Workgroup = 8 waves for one order m and 128 columns (column
-// super-group csg), covering every colatitude.  Degrees are processed in
-// passes of 64 (4 MFMA row tiles); within a pass the colatitudes are walked
-// in groups of 128 (two 64-lane blocks = four 32-colatitude K chunks).  The
-// recurrence of block b is advanced by wave b % 8, its state kept in LDS
-// between passes; the produced bf16 planes D[s][l][theta] are
-// double-buffered in LDS.  Wave w multiplies degree tiles 2*(w/4) and
-// 2*(w/4)+1 with column tiles 2*(w%4) and 2*(w%4)+1, so each wave ends a
-// pass with complete sums over all colatitudes and writes them directly.
+// Analysis.  Workgroup = 8 waves for one order m, one pass of 64 degrees (4
+// MFMA row tiles) and 128 columns (column super-group csg), covering every
+// colatitude in groups of 128 (two 64-lane blocks = four 32-colatitude K
+// chunks); the bf16 planes D[s][l][theta] of a group are double-buffered in
+// LDS.  While the waves multiply group g they produce group g + 1: wave w
+// runs the recurrence of block half w % 2 for the 16 degrees
+// lp + 16 (w / 2) .. + 15, starting from the checkpoint there, so every wave
+// does the same share with no branches.  Wave w multiplies degree tiles
+// 2*(w/4) and 2*(w/4)+1 with column tiles 2*(w%4) and 2*(w%4)+1, so each
+// wave ends a pass with complete sums over all colatitudes and writes them
+// directly.
 // ---------------------------------------------------------------------------
-constexpr int kAnalBf16MaxBlocks = 40;  // colatitude blocks: L <= 2559
-
 template <int NS, int REAL>
 __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
+  static_assert(kPass == 64 && kCk == 16, "4 degree quarters of 16 per pass");
   // Rows of 128 colatitudes padded to 136 (272 bytes): the 16 lanes of a
   // 16-byte A-operand read hit 16 consecutive rows, which then start 4 banks
   // apart and cover all 64 banks once (256-byte rows would all map to the
   // same 4 banks).
   __shared__ __bf16 Dl[2][NS][64][136];
-  __shared__ double2 st[kAnalBf16MaxBlocks][64];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   // Work item: (order, pass, column super-group).
@@ -419,8 +412,7 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   const int slot = m - p.m_first;
   const int l0 = max(abs(m), p.nabs);
   const int nt = p.ntheta;
-  const int nblk = (nt + 63) >> 6;
-  const int ngrp = (nblk + 1) >> 1;
+  const int ngrp = (nt + 127) >> 7;
   const int lp = l0 + kPass * pass;  // first degree of the pass
   // Recurrence coefficients of the pass (entries lp+1 .. lp+64, 2 KiB),
   // staged in LDS.  Entries up to L + kRecPad exist.
@@ -429,160 +421,191 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   const int tid = threadIdx.x;
   if (tid < 128) rsa[tid] = recq[2 * (lp + 1) + tid];
 
-  // Recurrence state of every block at degree lp: the seed for pass 0, else
-  // the checkpoint.
-  const double2* ckp = p.ck + (ld_uniform(p.ckoff, m + p.L) + pass - 1) * (long)nt;
-  for (int b = wave; b < nblk; b += 8) {
-    const int t = b * 64 + lane;
-    const int tc = min(t, nt - 1);
-    double2 s2;
-    if (pass == 0)
-      s2 = make_double2(p.seeds[(long)(m + p.L) * nt + tc], 0.0);
-    else
-      s2 = ckp[tc];
-    st[b][lane] = t < nt ? s2 : make_double2(0.0, 0.0);
-  }
-
-  // Produce the bf16 planes of block b (degrees lp..lp+63) into buffer half bb.
-  auto produce = [&](int b, int buf, int bb) {
-    const d4* rl = reinterpret_cast<const d4*>(&rsa[0]);  // entry lp+1+j
-    const int t = b * 64 + lane;
-    const double cs0 = p.cosv[min(t, nt - 1)];
-    const double cs = t < nt ? cs0 : 0.0;
-    double2 s2 = st[b][lane];
-    double e0 = s2.x, e1 = s2.y;
-#pragma unroll 4
-    for (int k = 0; k < 64; k += 2) {
-      const double v0 = e0;
-      rec_step(rl[k], cs, e0, e1);
-      const double v1 = e0;
-      rec_step(rl[k + 1], cs, e0, e1);
+  // Producer share of this wave: block half bh, degree quarter q.  Its
+  // recurrence starts at degree lp + 16 q from checkpoint kq (the seed for
+  // kq = 0).  Quarters past L (only garbage rows, never written) clamp to
+  // the last checkpoint so every load stays in the table.
+  const int bh = wave & 1, q = wave >> 1;
+  const int nck = (p.L - l0 + kCk) / kCk;  // checkpoints 1 .. nck-1 exist
+  const int kq = min((kPass / kCk) * pass + q, nck - 1);
+  const double2* ckp = kq == 0 ? p.ck : p.ck + (ld_uniform(p.ckoff, m + p.L) + kq - 1) * (long)nt;
+  const double* sdp = p.seeds + (long)(m + p.L) * nt;
+  // The state and cosine of group g's colatitude are loaded into registers
+  // one group before they are used, after the previous production (so the
+  // loads land in registers just freed and nothing waits on them until the
+  // next production).  Colatitudes past ntheta load the last one's values:
+  // their products meet zero B operands, and the values stay finite.
+  double nx_seed, nx_cs;
+  double2 nx_ck;
+  auto prefetch = [&](int g) {
+    const int tc = min((2 * g + bh) * 64 + lane, nt - 1);
+    nx_seed = sdp[tc];
+    nx_ck = ckp[tc];
+    nx_cs = p.cosv[tc];
+  };
+  // Production of this wave's 16 degrees x 64 colatitudes of a group into
+  // buffer buf, in four parts of 4 degrees (one per K chunk of the group
+  // multiplied meanwhile, so the scheduler can interleave their VALU work
+  // with that chunk's MFMAs).  begin() takes the prefetched state.
+  double pe0, pe1, pcs;
+  auto begin = [&]() {
+    pcs = nx_cs;
+    pe0 = kq == 0 ? nx_seed : nx_ck.x;
+    pe1 = kq == 0 ? 0.0 : nx_ck.y;
+  };
+  auto produce4 = [&](int buf, int part) {
+    const d4* rl = reinterpret_cast<const d4*>(&rsa[0]) + 16 * q;  // entry lp+16q+1+j
+#pragma unroll
+    for (int k = 4 * part; k < 4 * part + 4; k += 2) {
+      const double v0 = pe0;
+      rec_step(rl[k], pcs, pe0, pe1);
+      const double v1 = pe0;
+      rec_step(rl[k + 1], pcs, pe0, pe1);
       unsigned o[NS];
       split_pair<NS>(v0, v1, o);
 #pragma unroll
       for (int s = 0; s < NS; s++) {
         const bf16x2 pr = __builtin_bit_cast(bf16x2, o[s]);
-        Dl[buf][s][k][bb * 64 + lane] = pr[0];
-        Dl[buf][s][k + 1][bb * 64 + lane] = pr[1];
+        Dl[buf][s][16 * q + k][bh * 64 + lane] = pr[0];
+        Dl[buf][s][16 * q + k + 1][bh * 64 + lane] = pr[1];
       }
-    }
-    st[b][lane] = make_double2(e0, e1);
-  };
-  // Blocks past the last colatitude are never read (their K chunks are >= nkc).
-  auto produce_group = [&](int g, int buf) {
-#pragma unroll
-    for (int bb = 0; bb < 2; bb++) {
-      const int b = 2 * g + bb;
-      if ((b & 7) == wave && b < nblk) produce(b, buf, bb);
     }
   };
 
   // Consumer role.
   const int lh = wave >> 2;
   const int tile0 = csg * 8 + (wave & 3) * 2;
-  const bool tv0 = tile0 < p.ntile, tv1 = tile0 + 1 < p.ntile;
   const uint4* bslot = p.bs + (long)slot * p.nkc * p.ntile * NS * 64 + lane;
-  // Chunks past nkc and tiles past ntile read the zero block p.bzero (an
-  // address choice, so nothing waits on the loads before their MFMAs).
+  // Column tiles past ntile read the zero block p.bzero at every chunk (a
+  // base address and a zero chunk stride, chosen once).  Callers pass chunks
+  // below nkc only.
+  const uint4* bt[2];
+  long bstr[2];
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    const bool tv = tile0 + i < p.ntile;
+    bt[i] = tv ? bslot + (long)(tile0 + i) * NS * 64 : p.bzero + lane;
+    bstr[i] = tv ? (long)p.ntile * NS * 64 : 0;
+  }
   auto loadB = [&](int kc, uint4 (&B)[2][NS]) {
-    const bool kv = kc < p.nkc;
-    const long c = (long)kc * p.ntile;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-      const uint4* a0 = kv && tv0 ? bslot + ((c + tile0) * NS + s) * 64 : p.bzero + s * 64 + lane;
-      const uint4* a1 =
-          kv && tv1 ? bslot + ((c + tile0 + 1) * NS + s) * 64 : p.bzero + s * 64 + lane;
-      B[0][s] = *a0;
-      B[1][s] = *a1;
+      B[0][s] = bt[0][kc * bstr[0] + s * 64];
+      B[1][s] = bt[1][kc * bstr[1] + s * 64];
     }
   };
 
-  {
-    d4 acc[2][2];
+  d4 acc[2][2];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+  for (int i = 0; i < 2; i++)
 #pragma unroll
-      for (int j = 0; j < 2; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-
-    __syncthreads();  // rsa and st
-    produce_group(0, 0);
-    __syncthreads();
-    // B operands: ping-pong buffers, each loaded one K chunk ahead of its
-    // use (no register copies); B0 holds the group's first chunk on entry.
-    uint4 B0[2][NS], B1[2][NS];
-    loadB(0, B0);
-    for (int g = 0; g < ngrp; g++) {
-      if (g + 1 < ngrp) produce_group(g + 1, (g + 1) & 1);
-      const int buf = g & 1;
-      auto readA = [&](int kq, uint4 (&A)[2][NS]) {
+    for (int j = 0; j < 2; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  // One K chunk kc of buffer buf; the chains of the previous chunk (cp, zero
+  // before the first) are flushed to FP64 after this chunk's MFMAs are issued.
+  f4 cp[2][2];
 #pragma unroll
-        for (int lt = 0; lt < 2; lt++)
+  for (int i = 0; i < 2; i++)
 #pragma unroll
-          for (int s = 0; s < NS; s++)
-            A[lt][s] = *reinterpret_cast<const uint4*>(
-                &Dl[buf][s][(2 * lh + lt) * 16 + (lane & 15)][kq * 32 + 8 * (lane >> 4)]);
-      };
-      // One K chunk; its chains are flushed to FP64 after the next chunk's
-      // MFMAs are issued (cp carries them).
-      f4 cp[2][2];
-      auto step = [&](int kq, const uint4 (&B)[2][NS], bool flush_prev) {
-        uint4 A[2][NS];
-        readA(kq, A);
-        f4 c[2][2];
-        split_mma<NS, 2>(A, B, c);
-        if (flush_prev) {
-#pragma unroll
-          for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int j = 0; j < 2; j++) flush(acc[i][j], cp[i][j]);
-        }
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-          for (int j = 0; j < 2; j++) cp[i][j] = c[i][j];
-      };
-      const int kc0 = 4 * g;
-      // Chunks past nkc multiply the zero block (the last group only).
-      loadB(kc0 + 1, B1);
-      step(0, B0, false);
-      loadB(kc0 + 2, B0);
-      step(1, B1, true);
-      if (kc0 + 2 < p.nkc) {
-        loadB(kc0 + 3, B1);
-        step(2, B0, true);
-        loadB(kc0 + 4, B0);
-        step(3, B1, true);
-      } else {
-        loadB(kc0 + 4, B0);
-      }
-#pragma unroll
-      for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) flush(acc[i][j], cp[i][j]);
-      __syncthreads();
-    }
-
-    // Rows l = lp + 16*(2*lh + lt) + 4*(lane/16) + r, column 16*tile + lane%16.
+    for (int j = 0; j < 2; j++) cp[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  auto step = [&](int buf, int kc, const uint4 (&B)[2][NS]) {
+    uint4 A[2][NS];
 #pragma unroll
     for (int lt = 0; lt < 2; lt++)
 #pragma unroll
-      for (int ct = 0; ct < 2; ct++) {
-        const int col = (tile0 + ct) * 16 + (lane & 15);
-        if (col >= p.ncol) continue;
-        const int lf = field_lmax(p, col >> 1);
-        const long cbase = (long)field_index(p.fmap, col >> 1) * p.cstride;
+      for (int s = 0; s < NS; s++)
+        A[lt][s] = *reinterpret_cast<const uint4*>(
+            &Dl[buf][s][(2 * lh + lt) * 16 + (lane & 15)][(kc & 3) * 32 + 8 * (lane >> 4)]);
+    f4 c[2][2];
+    split_mma<NS, 2>(A, B, c);
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int l = lp + 16 * (2 * lh + lt) + 4 * (lane >> 4) + r;
-          if (l > lf) continue;
-          const long ci = (cbase + coef_index(l, m, p.nabs, REAL)) * 2 + (col & 1);
-          double v = acc[lt][ct][r];
-          if (p.accumulate) v += p.cout[ci];
-          if (p.zero_last && lf == p.L && l == lf && m == lf) v = 0.0;
-          p.cout[ci] = v;
-        }
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        flush(acc[i][j], cp[i][j]);
+        cp[i][j] = c[i][j];
       }
+  };
+
+  prefetch(0);
+  __syncthreads();  // rsa
+  begin();
+#pragma unroll
+  for (int part = 0; part < 4; part++) produce4(0, part);
+  prefetch(1);
+  __syncthreads();
+  // B operands: ping-pong buffers, each loaded one K chunk ahead of its use
+  // (no register copies); B0 holds the group's first chunk on entry.  Every
+  // group but the last has four chunks below nkc (nt > 128 (ngrp-1)), and the
+  // last group's first chunk is below nkc too.  The scheduling barriers keep
+  // each chunk's loads, production part and MFMAs together.
+  uint4 B0[2][NS], B1[2][NS];
+  loadB(0, B0);
+  for (int g = 0; g + 1 < ngrp; g++) {
+    const int buf = g & 1;
+    const int kc0 = 4 * g;
+    begin();
+    prefetch(g + 2);  // a whole group ahead of its use
+    loadB(kc0 + 1, B1);
+    produce4(buf ^ 1, 0);
+    step(buf, kc0, B0);
+    __builtin_amdgcn_sched_barrier(0);
+    loadB(kc0 + 2, B0);
+    produce4(buf ^ 1, 1);
+    step(buf, kc0 + 1, B1);
+    __builtin_amdgcn_sched_barrier(0);
+    loadB(kc0 + 3, B1);
+    produce4(buf ^ 1, 2);
+    step(buf, kc0 + 2, B0);
+    __builtin_amdgcn_sched_barrier(0);
+    loadB(kc0 + 4, B0);
+    produce4(buf ^ 1, 3);
+    step(buf, kc0 + 3, B1);
+    __builtin_amdgcn_sched_barrier(0);
+    __syncthreads();
   }
+  {
+    // Last group: chunks 4 (ngrp-1) .. nkc-1 (loads clamp to nkc-1; what
+    // they bring past the last chunk is never multiplied).
+    const int buf = (ngrp - 1) & 1;
+    const int kc0 = 4 * (ngrp - 1);
+    const int kcl = p.nkc - 1;
+    loadB(min(kc0 + 1, kcl), B1);
+    step(buf, kc0, B0);
+    if (kc0 + 1 < p.nkc) {
+      loadB(min(kc0 + 2, kcl), B0);
+      step(buf, kc0 + 1, B1);
+      if (kc0 + 2 < p.nkc) {
+        loadB(min(kc0 + 3, kcl), B1);
+        step(buf, kc0 + 2, B0);
+        if (kc0 + 3 < p.nkc) step(buf, kc0 + 3, B1);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) flush(acc[i][j], cp[i][j]);
+  }
+
+  // Rows l = lp + 16*(2*lh + lt) + 4*(lane/16) + r, column 16*tile + lane%16.
+#pragma unroll
+  for (int lt = 0; lt < 2; lt++)
+#pragma unroll
+    for (int ct = 0; ct < 2; ct++) {
+      const int col = (tile0 + ct) * 16 + (lane & 15);
+      if (col >= p.ncol) continue;
+      const int lf = field_lmax(p, col >> 1);
+      const long cbase = (long)field_index(p.fmap, col >> 1) * p.cstride;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int l = lp + 16 * (2 * lh + lt) + 4 * (lane >> 4) + r;
+        if (l > lf) continue;
+        const long ci = (cbase + coef_index(l, m, p.nabs, REAL)) * 2 + (col & 1);
+        double v = acc[lt][ct][r];
+        if (p.accumulate) v += p.cout[ci];
+        if (p.zero_last && lf == p.L && l == lf && m == lf) v = 0.0;
+        p.cout[ci] = v;
+      }
+    }
 }
 
 }  // namespace
@@ -641,7 +664,6 @@ hipError_t launch_anal_bf16_t(const LegParams& p0, hipStream_t s) {
 }
 
 hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s) {
-  if ((p.ntheta + 63) / 64 > kAnalBf16MaxBlocks) return hipErrorInvalidValue;
   return ns == 3 ? launch_anal_bf16_t<3>(p, s) : launch_anal_bf16_t<2>(p, s);
 }
 
