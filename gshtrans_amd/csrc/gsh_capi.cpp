@@ -870,6 +870,10 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
     p.items = w.d;
     p.items_per = w.per;
     const int ns = precision == GSH_BF16X6 ? 3 : 2;
+    // The split-BF16 synthesis starts its recurrences at the checkpoints.
+    if ((st = ensure_ck(g, d, n, s))) return st;
+    p.ck = d->ck;
+    p.ckoff = d->ckoff;
     long* cso = nullptr;
     long nchunks = 0;
     if ((st = ensure_cso(g, lmax, n, real, &cso, &nchunks))) return st;

@@ -490,6 +490,7 @@ hipError_t launch_leg_synth_f64(const LegParams& p0, hipStream_t s, int tiles, b
 }
 
 hipError_t launch_leg_anal_f64(const LegParams& p0, hipStream_t s, bool f32m) {
+  if (!p0.ck || !p0.ckoff) return hipErrorInvalidValue;  // checkpoint table required
   LegParams p = p0;
   p.n_cg = (p.ncol + 15) / 16;
   p.nquad = (p.ntheta + 3) / 4;

@@ -212,15 +212,26 @@ __global__ __launch_bounds__(256) void bin_split(LegParams p, long nblocks) {
 
 // ---------------------------------------------------------------------------
 // Synthesis.  Workgroup = 8 waves: (256 colatitudes of super-block tsb) x
-// (128 columns of column super-group csg) for one order m.  Wave w < 4
-// produces the split D of colatitudes 64w..64w+63; wave w consumes
-// colatitude half h = w/4 (128 colatitudes = 8 MFMA row groups) for column
-// group w%4 (2 tiles of 16 columns).
+// (128 columns of column super-group csg) for one order m, walking the
+// degrees in 32-degree K chunks whose bf16 planes of D are double-buffered in
+// LDS.  While the waves multiply chunk c they produce chunk c + 1: wave w
+// runs the recurrence of colatitude block w % 4 (64 colatitudes) for the 16
+// degrees of half w / 4, starting from the checkpoint there, so every wave
+// does the same share.  Wave w multiplies colatitude half h = w / 4 (8 MFMA
+// row groups) with column tiles 2 (w % 4) and 2 (w % 4) + 1.  Each row
+// group's MFMAs share a scheduling region with one recurrence pair, so the
+// VALU and MFMA work interleave.
 // ---------------------------------------------------------------------------
 template <int NS, int REAL>
 __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
+  static_assert(kCk == 16, "two 16-degree halves per 32-degree chunk");
   __shared__ uint4 trb[2][NS][4][256];
+  // B operands of a chunk (8 column tiles x NS planes x 64 lanes) and the
+  // recurrence coefficients of a chunk (32 entries, 1 KiB), double-buffered.
+  __shared__ uint4 bsh[2][8][NS][64];
+  __shared__ uint4 rsc[2][64];
   const int lane = threadIdx.x & 63;
+  const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   // Work item: (order, column super-group, colatitude super-block).
   const uint2 it = work_item(p.items, p.items_per);
@@ -231,88 +242,97 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
   const int m = order_of(mw, REAL);
   const int slot = m - p.m_first;
   const int l0 = max(abs(m), p.nabs);
-  const int lmax = p.lmax;
-  const int nchunk = (lmax - l0 + 32) >> 5;
-  // Producers whose 64 colatitudes all lie past ntheta have nothing to do;
-  // consumers skip row groups past ntheta (2 ngp of 8 remain).
-  const bool prod = wave < 4 && tsb * 256 + (wave & 3) * 64 < p.ntheta;
-  const int ngp = max(0, min(4, (p.ntheta - tsb * 256 - (wave >> 2) * 128 + 31) / 32));
+  const int nt = p.ntheta;
+  const int nchunk = (p.lmax - l0 + 32) >> 5;
+  const int rows = min(256, nt - tsb * 256);  // valid colatitudes of the item
 
-  // Producer state: colatitude tp.
-  const int tp = tsb * 256 + (wave & 3) * 64 + lane;
-  const bool tpv = prod && tp < p.ntheta;
-  // Branch-free loads at a clamped colatitude keep the control flow uniform.
-  const int tpc = min(tp, p.ntheta - 1);
-  const double cs0 = p.cosv[tpc], s0 = p.seeds[(long)(m + p.L) * p.ntheta + tpc];
-  const double cs = tpv ? cs0 : 0.0;
-  double e0 = tpv ? s0 : 0.0;
-  double e1 = 0.0;
+  // Producer share: colatitude block pb, degree half ph.
+  const int pb = wave & 3, ph = wave >> 2;
+  const int tpc = min(tsb * 256 + 64 * pb + lane, nt - 1);  // past ntheta: the last one
+  const double pcs = p.cosv[tpc];
+  const int nck = (p.L - l0 + kCk) / kCk;  // checkpoints 1 .. nck-1 exist
+  // State at checkpoint k >= 1; past L it clamps to the last checkpoint
+  // (those degrees only meet zero coefficients, and the values stay finite).
+  // An order without checkpoints (nck < 2, so a single chunk) reads entry 0
+  // of the table instead and never uses it.
+  const double2* ckm =
+      (nck > 1 ? p.ck + ld_uniform(p.ckoff, m + p.L) * (long)nt : p.ck) + tpc;
+  auto ck_state = [&](int k) { return ckm[(long)(nck > 1 ? min(k, nck - 1) - 1 : 0) * nt]; };
+  double pe0, pe1;
+  {
+    // Chunk 0: the seed for half 0, checkpoint 1 for half 1.
+    const double seed = p.seeds[(long)(m + p.L) * nt + tpc];
+    const double2 c1 = ck_state(1);
+    pe0 = ph == 0 || nck < 2 ? seed : c1.x;
+    pe1 = ph == 0 || nck < 2 ? 0.0 : c1.y;
+  }
 
-  // Recurrence coefficients of a chunk (entries lb+1 .. lb+32, 1 KiB) are
-  // staged through a per-wave LDS slot: lane j fetches 16 bytes of the next
-  // chunk one chunk ahead, so the steps read them from LDS instead of
-  // waiting on scalar loads.  Entries up to L + kRecPad exist, so fetching
-  // chunk nchunk is in bounds.
-  __shared__ uint4 rst[4][64];
   const uint4* recq = reinterpret_cast<const uint4*>(p.rec + (ld_uniform(p.roff, m + p.L) - l0));
-  auto fetch = [&](int chunk) { return recq[2 * (l0 + 32 * chunk + 1) + lane]; };
-  uint4 rq = prod ? fetch(0) : make_uint4(0, 0, 0, 0);
-  auto produce = [&](int chunk, int buf) {
-    rst[wave & 3][lane] = rq;
-    wave_sync();
-    rq = fetch(chunk + 1);
-    const d4* rl = reinterpret_cast<const d4*>(&rst[wave & 3][0]);  // rl[j]: entry lb + 1 + j
+  // Entries up to L + kRecPad exist: chunk c's entries l0 + 32c + 1 .. + 32
+  // stay in the table for every c <= nchunk.
+  auto fetch_rec = [&](int c) { return recq[2 * (l0 + 32 * c + 1) + (tid & 63)]; };
+
+  // B operand staging: piece j of this thread is uint4 tid + 512 j of the
+  // chunk's 8 contiguous tiles (ntile is a multiple of 8), in the source and
+  // in the LDS buffer alike.
+  const uint4* bbase = p.bs + (ld_uniform(p.cso, slot) * (long)p.ntile + csg * 8) * NS * 64;
+  const int cstride = p.ntile * NS * 64;  // uint4 per chunk
+  struct Pieces {
+    uint4 v[NS];
+    uint4 r;
+  };
+  auto fetch = [&](int chunk) {
+    const uint4* cb = bbase + (long)min(chunk, nchunk - 1) * cstride;
+    Pieces q;
 #pragma unroll
-    for (int kr = 0; kr < 4; kr++) {
-      unsigned pl[NS][4];
+    for (int j = 0; j < NS; j++) q.v[j] = cb[tid + 512 * j];
+    q.r = fetch_rec(min(chunk + 1, nchunk));
+    return q;
+  };
+  auto store = [&](int buf, const Pieces q) {
 #pragma unroll
-      for (int e2 = 0; e2 < 4; e2++) {
-        const int j = 8 * kr + 2 * e2;
-        const double v0 = e0;
-        rec_step(rl[j], cs, e0, e1);
-        const double v1 = e0;
-        rec_step(rl[j + 1], cs, e0, e1);
-        unsigned o[NS];
-        split_pair<NS>(v0, v1, o);
-#pragma unroll
-        for (int s = 0; s < NS; s++) pl[s][e2] = o[s];
-      }
-#pragma unroll
-      for (int s = 0; s < NS; s++)
-        trb[buf][s][kr][(wave & 3) * 64 + lane] = make_uint4(pl[s][0], pl[s][1], pl[s][2], pl[s][3]);
-    }
-    wave_sync();
+    for (int j = 0; j < NS; j++) (&bsh[buf][0][0][0])[512 * j + tid] = q.v[j];
+    if (tid < 64) rsc[buf ^ 1][tid] = q.r;
   };
 
   // Consumer role.
   const int h = wave >> 2;
   const int tile0 = csg * 8 + (wave & 3) * 2;
-  // B operands of a chunk (8 column tiles x NS planes x 64 lanes) are staged
-  // in LDS, double-buffered: every thread loads NS 16-byte pieces of the next
-  // chunk into registers at the start of a chunk and stores them to LDS
-  // after the chunk's MFMAs, so the loads have a whole chunk to complete and
-  // only the store waits for them.  The split coefficients hold whole
-  // super-groups of 8 tiles (ntile is a multiple of 8).
-  __shared__ uint4 bsh[2][8][NS][64];
-  const uint4* bbase = p.bs + (ld_uniform(p.cso, slot) * (long)p.ntile + csg * 8) * NS * 64;
-  const int tid = threadIdx.x;
-  const int cstride = p.ntile * NS * 64;  // uint4 per chunk
-  // The 8 tiles of the super-group are contiguous within a chunk (8 NS 64
-  // uint4); piece j of this thread is uint4 tid + 512 j of them, in the
-  // source and in the LDS buffer alike.
-  struct Pieces {
-    uint4 v[NS];
-  };
-  auto fetchB = [&](int chunk) {
-    const uint4* cb = bbase + (long)min(chunk, nchunk - 1) * cstride;
-    Pieces q;
+  d4 acc[8][2];
 #pragma unroll
-    for (int j = 0; j < NS; j++) q.v[j] = cb[tid + 512 * j];
-    return q;
-  };
-  auto storeB = [&](int buf, const Pieces q) {
+  for (int g = 0; g < 8; g++)
 #pragma unroll
-    for (int j = 0; j < NS; j++) (&bsh[buf][0][0][0])[512 * j + tid] = q.v[j];
+    for (int t = 0; t < 2; t++) acc[g][t] = d4{0.0, 0.0, 0.0, 0.0};
+
+  // One recurrence pair (steps 2i, 2i+1 of this wave's 16 degrees of a
+  // chunk): its bf16 pair of each plane is dword i % 4 of the uint4 that
+  // holds degrees 8 kr .. 8 kr + 7 of the colatitude (kr = 2 ph + i / 4).
+  // Lanes write 16 bytes apart, so a 16-lane phase covers 64 banks once.
+  auto produce_pair = [&](int rbuf, int tbuf, int i) {
+    const d4* rl = reinterpret_cast<const d4*>(&rsc[rbuf][0]) + 16 * ph;
+    const double v0 = pe0;
+    rec_step(rl[2 * i], pcs, pe0, pe1);
+    const double v1 = pe0;
+    rec_step(rl[2 * i + 1], pcs, pe0, pe1);
+    unsigned o[NS];
+    split_pair<NS>(v0, v1, o);
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+      reinterpret_cast<unsigned*>(&trb[tbuf][s][2 * ph + (i >> 2)][64 * pb + lane])[i & 3] = o[s];
+  };
+  // Row group g's MFMAs and their flush to FP64 (when its rows hold
+  // colatitudes).
+  auto consume_group = [&](int buf, int g, const uint4 (&B)[2][NS], bool valid) {
+    if (valid) {
+      uint4 A[1][NS];
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+        A[0][s] = trb[buf][s][lane >> 4][h * 128 + g * 16 + (lane & 15)];
+      f4 c[1][2];
+      split_mma<NS, 1>(A, B, c);
+      flush(acc[g][0], c[0][0]);
+      flush(acc[g][1], c[0][1]);
+    }
   };
   auto loadB = [&](int buf, uint4 (&B)[2][NS]) {
 #pragma unroll
@@ -322,61 +342,70 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
     }
   };
 
-  d4 acc[8][2];
+  // Prologue: coefficients and B operands of chunk 0, production of chunk 0,
+  // coefficients of chunk 1.
+  if (tid < 64) rsc[0][tid] = fetch_rec(0);
+  {
+    const Pieces q = fetch(0);
 #pragma unroll
-  for (int g = 0; g < 8; g++)
+    for (int j = 0; j < NS; j++) (&bsh[0][0][0][0])[512 * j + tid] = q.v[j];
+    if (tid < 64) rsc[1][tid] = q.r;
+  }
+  __syncthreads();
 #pragma unroll
-    for (int t = 0; t < 2; t++) acc[g][t] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int i = 0; i < 8; i++) produce_pair(0, 0, i);
+  __syncthreads();
 
-  // One chunk: every row group of this wave, one at a time (two independent
-  // MFMA chains, its two column tiles), flushed to FP64 after each group.
-  auto consume = [&](int k) {
-    const int buf = k & 1;
+  // Chunks 0 .. nchunk-2 with the production of the next one.  Producer
+  // blocks and row groups past ntheta (the last super-block only) are
+  // skipped with uniform branches; the branches also bound the scheduling
+  // regions, which keeps the register pressure of the unrolled chunk low.
+  const bool pv = 64 * pb < rows;  // this wave's producer block holds colatitudes
+  // The state of chunk c + 1 (checkpoint 2 (c + 1) + ph, k >= 2) is loaded
+  // one chunk before its use.
+  double2 nx = ck_state(2 + ph);
+  for (int c = 0; c + 1 < nchunk; c++) {
+    const int buf = c & 1;
+    const Pieces q = fetch(c + 1);
+    pe0 = nx.x;
+    pe1 = nx.y;
+    nx = ck_state(2 * (c + 2) + ph);
     uint4 B[2][NS];
     loadB(buf, B);
 #pragma unroll
     for (int g = 0; g < 8; g++) {
-      if (ngp == 4 || g < 2 * ngp) {
-        uint4 A[1][NS];
-#pragma unroll
-        for (int s = 0; s < NS; s++)
-          A[0][s] = trb[buf][s][lane >> 4][h * 128 + g * 16 + (lane & 15)];
-        f4 c[1][2];
-        split_mma<NS, 1>(A, B, c);
-        flush(acc[g][0], c[0][0]);
-        flush(acc[g][1], c[0][1]);
-      }
+      if (pv) produce_pair(buf ^ 1, buf ^ 1, g);
+      consume_group(buf, g, B, h * 128 + g * 16 < rows);
+      __builtin_amdgcn_sched_barrier(0);
     }
-  };
-
-  {
-    const Pieces q = fetchB(0);
-    if (prod) produce(0, 0);
-    storeB(0, q);
-  }
-  __syncthreads();
-  for (int k = 0; k < nchunk; k++) {
-    const Pieces q = fetchB(k + 1);
-    if (prod && k + 1 < nchunk) produce(k + 1, (k + 1) & 1);
-    consume(k);
-    storeB((k + 1) & 1, q);  // buffer (k+1)&1 was last read in chunk k-1
+    store(buf ^ 1, q);  // buffers last read in chunk c - 1
     __syncthreads();
   }
+  {
+    const int buf = (nchunk - 1) & 1;
+    uint4 B[2][NS];
+    loadB(buf, B);
+#pragma unroll
+    for (int g = 0; g < 8; g++) consume_group(buf, g, B, h * 128 + g * 16 < rows);
+  }
 
-  // C layout of the 16x16x32 MFMA: lane holds rows 4*(lane/16)+r, column lane%16.
+  // C layout of the 16x16x32 MFMA: lane holds rows 4*(lane/16)+r, column
+  // lane%16.  F is linear in the colatitude (stride rs), so every store is
+  // one base pointer plus a uniform offset.
+  const int ncg16 = (p.ncol + 15) >> 4;
+  const long rs = (long)ncg16 * p.nslot * 16;
+  const int row0 = tsb * 256 + h * 128 + 4 * (lane >> 4);
 #pragma unroll
-  for (int g = 0; g < 8; g++)
+  for (int t = 0; t < 2; t++) {
+    const int col = (tile0 + t) * 16 + (lane & 15);
+    if (col >= p.ncol) continue;
+    double* fp = p.F + f_index(row0, slot, col, p.nslot, ncg16);
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
-      const int col = (tile0 + t) * 16 + (lane & 15);
-      if (col >= p.ncol) continue;
+    for (int g = 0; g < 8; g++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int row = tsb * 256 + h * 128 + g * 16 + 4 * (lane >> 4) + r;
-        if (row < p.ntheta)
-          p.F[f_index(row, slot, col, p.nslot, (p.ncol + 15) >> 4)] = acc[g][t][r];
-      }
-    }
+      for (int r = 0; r < 4; r++)
+        if (row0 + g * 16 + r < nt) fp[(g * 16 + r) * rs] = acc[g][t][r];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -636,6 +665,7 @@ hipError_t launch_bin_split(const LegParams& p0, int ns, long nblocks, hipStream
 }
 
 hipError_t launch_leg_synth_bf16(const LegParams& p0, int ns, hipStream_t s) {
+  if (!p0.ck || !p0.ckoff) return hipErrorInvalidValue;  // checkpoint table required
   LegParams p = p0;
   p.n_tsb = (p.ntheta + 255) / 256;
   p.n_cg = (p.ntile + 7) / 8;  // column super-groups of 8 tiles
@@ -664,6 +694,7 @@ hipError_t launch_anal_bf16_t(const LegParams& p0, hipStream_t s) {
 }
 
 hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s) {
+  if (!p.ck || !p.ckoff) return hipErrorInvalidValue;  // checkpoint table required
   return ns == 3 ? launch_anal_bf16_t<3>(p, s) : launch_anal_bf16_t<2>(p, s);
 }
 

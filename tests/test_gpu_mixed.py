@@ -94,3 +94,18 @@ def test_mixed_accumulate_and_zero_last(torch):
     ref[:, -1] = 0
     assert float(per_field_rel(torch, out, ref).max()) < 1e-7
     assert bool((out[:, -1] == 0).all())
+
+
+@pytest.mark.parametrize("prec", ["bf16x6", "bf16x3"])
+def test_mixed_synthesis_on_fresh_grid(torch, prec):
+    """The split-BF16 synthesis starts its recurrences at checkpoints that the
+    first call on a grid must build: synthesis before any analysis."""
+    say(f"[mixed synthesis, fresh grid] {prec} L=96 n=1")
+    g = G.GaussLegendreGrid(96, 1)
+    c = torch.from_numpy(G.random_coefficients(96, 1, real=False, seed=SEED0 + 7,
+                                               batch=3)).cuda()
+    fmx = g.inverse(96, 1, c, real=False, precision=prec)
+    f64 = g.inverse(96, 1, c, real=False)
+    e = per_field_rel(torch, fmx, f64)
+    say(f"    max {e.max():.2e}")
+    assert e.max() <= BARS[prec]
