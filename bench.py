@@ -11,8 +11,11 @@ timing starts.
 Printed: one JSON line.  `value` is transforms/s of the headline mode, the
 bf16x6 split-BF16 Legendre contraction (<= 1e-7 relative L2 against FP64);
 the FP64 figure, the speedup, the bf16x3 and FP32-MFMA modes, and the
-per-field relative L2 error of each mode against FP64 are reported beside it.  `roofline` is the dominant kernel's algorithmic rate
-(HIP events around each launch on its own stream) against the MI355X peak;
+per-field relative L2 error of each mode against FP64 are reported beside it.
+`roofline` is the dominant kernel's algorithmic rate (HIP events around each
+launch on its own stream) against the MI355X peak of the unit it runs on (for
+split-BF16 the MFMA flops of all products against the bf16 peak), with its
+per-launch HBM traffic from the committed counter summary under profiles/;
 `cpu_baseline` times the CPU oracle (a restatement of the reference's
 serial transform with its precomputed Wigner table) on a bounded sample.
 
@@ -32,8 +35,23 @@ L = 1024
 NMAX = 2
 BATCH = 64
 SEED = 20261202 + 2
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3, "bf16x6": 2500.0 / 6}
+# Published MI355X dense peaks of the MFMA type each mode runs on, and the
+# MFMA products each mode spends per logical product (split-BF16 planes).
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "bf16x6": 2500.0}
+PRODUCTS = {"fp64": 1, "fp32": 1, "bf16x3": 3, "bf16x6": 6}
 HBM_GBPS = 8000.0
+# Per-dispatch HBM bytes of each kernel, from FETCH_SIZE / WRITE_SIZE counter
+# runs scaled by a plain copy's reading (rocprofv3 summary, text).
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "transform_kernels_lmax1024_b64_counters.md")
+# Kernel instantiations of each work family per mode (real and complex batch).
+FAMILY_KERNELS = {
+    "legendre_analysis": {"fp64": "leg_anal_f64<4, false>", "fp32": "leg_anal_f64<4, true>",
+                          "bf16x3": "leg_anal_bf16<2,", "bf16x6": "leg_anal_bf16<3,"},
+    "legendre_synthesis": {"fp64": "leg_synth_f64<2, ", "fp32": "leg_synth_f64<2, ",
+                           "bf16x3": "leg_synth_bf16<2,", "bf16x6": "leg_synth_bf16<3,"},
+    "fft_analysis": "fft_anal_w<32, ",
+    "fft_synthesis": "fft_synth_w<32, ",
+}
 HEADLINE = "bf16x6"            # <= 1e-7 relative L2 against FP64
 REPORTED = ["bf16x3", "fp32"]  # reported beside it (throughput and errors)
 
@@ -164,7 +182,9 @@ def roofline(results, headline):
     total = per_step * steps
     sec = ms / 1e3
     if bound == "mfma":
-        achieved = total / sec / 1e12
+        # MFMA flops actually issued for the algorithmic products: each
+        # logical product costs PRODUCTS[mode] MFMA products.
+        achieved = total * PRODUCTS[headline] / sec / 1e12
         peak = PEAK_TFLOPS[headline]
         unit = "TFLOP/s"
     else:
@@ -174,8 +194,23 @@ def roofline(results, headline):
     per_kernel = {k: {"avg_launch_ms": v[0] / max(v[1], 1), "launches": v[1]}
                   for k, v in kt.items()}
     return {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-            "frac": achieved / peak, "traffic": None,
+            "frac": achieved / peak, "traffic": traffic_per_launch(name, headline),
             "avg_launch_ms": ms / max(launches, 1), "per_kernel": per_kernel}
+
+
+def traffic_per_launch(family, mode):
+    """HBM bytes per launch of a work family (mean over its real and complex
+    instantiations) from the committed counter summary, or None."""
+    from gshtrans_amd.profile_summary import load_table
+    t = load_table(TRAFFIC_FILE)
+    if not t:
+        return None
+    pre = FAMILY_KERNELS[family]
+    pre = pre[mode] if isinstance(pre, dict) else pre
+    rows = [v for k, v in t["kernels"].items() if k.startswith(pre)]
+    if not rows:
+        return None
+    return sum(r["fetch_bytes"] + r["write_bytes"] for r in rows) / len(rows)
 
 
 def cpu_baseline(args):

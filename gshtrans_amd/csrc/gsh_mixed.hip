@@ -305,9 +305,10 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
     for (int t = 0; t < 2; t++) acc[g][t] = d4{0.0, 0.0, 0.0, 0.0};
 
   // One recurrence pair (steps 2i, 2i+1 of this wave's 16 degrees of a
-  // chunk): its bf16 pair of each plane is dword i % 4 of the uint4 that
-  // holds degrees 8 kr .. 8 kr + 7 of the colatitude (kr = 2 ph + i / 4).
-  // Lanes write 16 bytes apart, so a 16-lane phase covers 64 banks once.
+  // chunk) into the packed planes pl; every 4 pairs fill the uint4 of each
+  // plane that holds degrees 8 kr .. 8 kr + 7 of the colatitude (kr =
+  // 2 ph + i / 4), stored whole (lanes 16 bytes apart: conflict-free).
+  unsigned pl[NS][4];
   auto produce_pair = [&](int rbuf, int tbuf, int i) {
     const d4* rl = reinterpret_cast<const d4*>(&rsc[rbuf][0]) + 16 * ph;
     const double v0 = pe0;
@@ -317,21 +318,34 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
     unsigned o[NS];
     split_pair<NS>(v0, v1, o);
 #pragma unroll
-    for (int s = 0; s < NS; s++)
-      reinterpret_cast<unsigned*>(&trb[tbuf][s][2 * ph + (i >> 2)][64 * pb + lane])[i & 3] = o[s];
-  };
-  // Row group g's MFMAs and their flush to FP64 (when its rows hold
-  // colatitudes).
-  auto consume_group = [&](int buf, int g, const uint4 (&B)[2][NS], bool valid) {
-    if (valid) {
-      uint4 A[1][NS];
+    for (int s = 0; s < NS; s++) pl[s][i & 3] = o[s];
+    if ((i & 3) == 3) {
 #pragma unroll
       for (int s = 0; s < NS; s++)
-        A[0][s] = trb[buf][s][lane >> 4][h * 128 + g * 16 + (lane & 15)];
-      f4 c[1][2];
-      split_mma<NS, 1>(A, B, c);
-      flush(acc[g][0], c[0][0]);
-      flush(acc[g][1], c[0][1]);
+        trb[tbuf][s][2 * ph + (i >> 2)][64 * pb + lane] =
+            make_uint4(pl[s][0], pl[s][1], pl[s][2], pl[s][3]);
+    }
+  };
+  // Row groups g and g + 1: their MFMAs as four independent chains, then the
+  // flush to FP64.  A pair is skipped when its first group holds no
+  // colatitude; otherwise both are computed, and the rows past ntheta (their
+  // D planes may be stale) only reach accumulator rows that are never stored
+  // (each MFMA output row depends on its own A row only).
+  auto consume_groups = [&](int buf, int g, const uint4 (&B)[2][NS], bool valid) {
+    if (valid) {
+      uint4 A[2][NS];
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+          A[u][s] = trb[buf][s][lane >> 4][h * 128 + (g + u) * 16 + (lane & 15)];
+      f4 c[2][2];
+      split_mma<NS, 2>(A, B, c);
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        flush(acc[g + u][0], c[u][0]);
+        flush(acc[g + u][1], c[u][1]);
+      }
     }
   };
   auto loadB = [&](int buf, uint4 (&B)[2][NS]) {
@@ -373,9 +387,12 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
     uint4 B[2][NS];
     loadB(buf, B);
 #pragma unroll
-    for (int g = 0; g < 8; g++) {
-      if (pv) produce_pair(buf ^ 1, buf ^ 1, g);
-      consume_group(buf, g, B, h * 128 + g * 16 < rows);
+    for (int g = 0; g < 8; g += 2) {
+      if (pv) {
+        produce_pair(buf ^ 1, buf ^ 1, g);
+        produce_pair(buf ^ 1, buf ^ 1, g + 1);
+      }
+      consume_groups(buf, g, B, h * 128 + g * 16 < rows);
       __builtin_amdgcn_sched_barrier(0);
     }
     store(buf ^ 1, q);  // buffers last read in chunk c - 1
@@ -386,7 +403,7 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
     uint4 B[2][NS];
     loadB(buf, B);
 #pragma unroll
-    for (int g = 0; g < 8; g++) consume_group(buf, g, B, h * 128 + g * 16 < rows);
+    for (int g = 0; g < 8; g += 2) consume_groups(buf, g, B, h * 128 + g * 16 < rows);
   }
 
   // C layout of the 16x16x32 MFMA: lane holds rows 4*(lane/16)+r, column
