@@ -8,8 +8,8 @@
 // longitudes the 2L points j pi / L, fields row-major (theta, phi), and
 // coefficients in GSHIndices order (All for complex fields, NonNegative for
 // real fields).  The transforms take contiguous ranges (every caller in the
-// reference passes vectors) and run on the GPU; the reference's asserts on
-// the arguments become asserts on the library's status codes.
+// reference passes vectors) and run on the GPU; where the reference asserts
+// on the arguments, a non-OK status of the library throws GSHTrans::Error.
 //
 // Differences that the C ABI fixes: Real must be double (the library is
 // FP64), and the FFTW planner flag of the reference's constructor is
