@@ -26,8 +26,14 @@
 namespace gsh {
 namespace {
 
-constexpr int kWaves = 8;          // wavefronts (rings) per workgroup
-constexpr int kLdsPerWave = 1024;  // double2 per wavefront
+// LDS per wavefront (double2): one transpose round of R / H rows of 64.
+constexpr int lds_per_wave(int R) { return (R >= 32 ? R / 2 : R) * 64; }
+// Wavefronts (rings) per workgroup: 8 complex or 16 real fields of one
+// colatitude, so the workgroup gathers and scatters the order bins of F / FA
+// in whole 128-byte lines (4-wavefront workgroups, two per CU, measured
+// slower: their 64-byte pieces cost more than the overlap gained).  At most
+// 256 VGPRs (two wavefronts per SIMD), as the launch bounds require.
+constexpr int waves_for(int) { return 8; }
 
 // cos(pi k / 16) and sin(pi k / 16), k = 0..16.
 constexpr double kCos16[17] = {
@@ -227,14 +233,15 @@ __device__ __forceinline__ void st2(double* p, double2 v) {
 
 // Synthesis: order bins of F[theta][slot][col] -> backward DFT -> field ring.
 template <int R, int REAL, int F32>
-__global__ __launch_bounds__(512) void fft_synth_w(FFTParams p) {
-  __shared__ double2 lds_all[kWaves * kLdsPerWave];
+__global__ __launch_bounds__(64 * waves_for(R), 2) void fft_synth_w(FFTParams p) {
+  constexpr int W = waves_for(R);
+  __shared__ double2 lds_all[W * lds_per_wave(R)];
   constexpr int N = 64 * R;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const int fg = blockIdx.x;
   const int theta = blockIdx.y;
-  const int f0 = (fg * kWaves + wave) * (REAL ? 2 : 1);
+  const int f0 = (fg * W + wave) * (REAL ? 2 : 1);
   if (f0 >= p.nfield) return;
   const bool f1v = REAL && f0 + 1 < p.nfield;
   const int lmax = p.lmax;
@@ -266,7 +273,7 @@ __global__ __launch_bounds__(512) void fft_synth_w(FFTParams p) {
     }
     v[n1] = X;
   }
-  wave_fft<R, +1>(v, lds_all + wave * kLdsPerWave, p.twa, p.tw, lane);
+  wave_fft<R, +1>(v, lds_all + wave * lds_per_wave(R), p.twa, p.tw, lane);
   const long oa = (long)field_index(p.fmap, f0) * p.fstride + (long)theta * N + lane;
   if (!REAL) {
 #pragma unroll
@@ -286,14 +293,15 @@ __global__ __launch_bounds__(512) void fft_synth_w(FFTParams p) {
 // layout FA (fa_index); the last ring also zeroes the padding colatitudes
 // of the last quad.
 template <int R, int REAL, int F32>
-__global__ __launch_bounds__(512) void fft_anal_w(FFTParams p) {
-  __shared__ double2 lds_all[kWaves * kLdsPerWave];
+__global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) {
+  constexpr int W = waves_for(R);
+  __shared__ double2 lds_all[W * lds_per_wave(R)];
   constexpr int N = 64 * R;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const int fg = blockIdx.x;
   const int theta = blockIdx.y;
-  const int f0 = (fg * kWaves + wave) * (REAL ? 2 : 1);
+  const int f0 = (fg * W + wave) * (REAL ? 2 : 1);
   if (f0 >= p.nfield) return;
   const bool f1v = REAL && f0 + 1 < p.nfield;
   const int lmax = p.lmax;
@@ -312,7 +320,7 @@ __global__ __launch_bounds__(512) void fft_anal_w(FFTParams p) {
       v[n1] = make_double2(xa, f1v ? xb : 0.0);
     }
   }
-  wave_fft<R, -1>(v, lds_all + wave * kLdsPerWave, p.twa, p.tw, lane);
+  wave_fft<R, -1>(v, lds_all + wave * lds_per_wave(R), p.twa, p.tw, lane);
   const double w = p.wq[theta];
   const int n_cg = p.n_cg, nquad = p.nquad;
   if (!REAL) {
@@ -366,7 +374,7 @@ __global__ __launch_bounds__(512) void fft_anal_w(FFTParams p) {
 
 template <int R>
 hipError_t launch_synth_r(const FFTParams& p, hipStream_t s) {
-  const dim3 grid(p.nfg, p.ntheta), block(64 * kWaves);
+  const dim3 grid(p.nfg, p.ntheta), block(64 * waves_for(R));
   if (p.real && p.f32)
     hipLaunchKernelGGL((fft_synth_w<R, 1, 1>), grid, block, 0, s, p);
   else if (p.real)
@@ -380,7 +388,7 @@ hipError_t launch_synth_r(const FFTParams& p, hipStream_t s) {
 
 template <int R>
 hipError_t launch_anal_r(const FFTParams& p, hipStream_t s) {
-  const dim3 grid(p.nfg, p.ntheta), block(64 * kWaves);
+  const dim3 grid(p.nfg, p.ntheta), block(64 * waves_for(R));
   if (p.real && p.f32)
     hipLaunchKernelGGL((fft_anal_w<R, 1, 1>), grid, block, 0, s, p);
   else if (p.real)
@@ -396,10 +404,15 @@ hipError_t launch_anal_r(const FFTParams& p, hipStream_t s) {
 
 bool wave_fft_supported(int N) { return N == 512 || N == 1024 || N == 2048; }
 
+// Field groups of a launch: waves_for(R) rings per workgroup, 2 real fields per ring.
+static int field_groups(const FFTParams& p) {
+  const int per = waves_for(p.N / 64) * (p.real ? 2 : 1);
+  return (p.nfield + per - 1) / per;
+}
+
 hipError_t launch_fft_synth_wave(const FFTParams& p0, hipStream_t s) {
   FFTParams p = p0;
-  const int per = kWaves * (p.real ? 2 : 1);
-  p.nfg = (p.nfield + per - 1) / per;
+  p.nfg = field_groups(p);
   switch (p.N) {
     case 512: return launch_synth_r<8>(p, s);
     case 1024: return launch_synth_r<16>(p, s);
@@ -410,8 +423,7 @@ hipError_t launch_fft_synth_wave(const FFTParams& p0, hipStream_t s) {
 
 hipError_t launch_fft_anal_wave(const FFTParams& p0, hipStream_t s) {
   FFTParams p = p0;
-  const int per = kWaves * (p.real ? 2 : 1);
-  p.nfg = (p.nfield + per - 1) / per;
+  p.nfg = field_groups(p);
   p.n_cg = (p.ncol + 15) / 16;
   p.nquad = (p.ntheta + 3) / 4;
   switch (p.N) {

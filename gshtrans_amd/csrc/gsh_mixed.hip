@@ -87,26 +87,26 @@ __device__ __forceinline__ f4 mfma(uint4 a, uint4 b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(a), as_bf8(b), c, 0, 0, 0);
 }
 
-// Split products of NA A operands (rows) with two B operands (columns), NS
-// planes each, as 2 NA independent MFMA chains advanced one product at a
+// Split products of NA A operands (rows) with NB B operands (columns), NS
+// planes each, as NA NB independent MFMA chains advanced one product at a
 // time (so consecutive MFMAs never depend on each other); within each chain
 // the cross products go first and the leading product last.
-template <int NS, int NA>
-__device__ __forceinline__ void split_mma(const uint4 (&A)[NA][NS], const uint4 (&B)[2][NS],
-                                          f4 (&c)[NA][2]) {
+template <int NS, int NA, int NB = 2>
+__device__ __forceinline__ void split_mma(const uint4 (&A)[NA][NS], const uint4 (&B)[NB][NS],
+                                          f4 (&c)[NA][NB]) {
   constexpr int np = NS == 3 ? 6 : 3;
   constexpr int pa[6] = {1, 0, 2, 0, 1, 0};  // plane of A in product q
   constexpr int pb[6] = {1, 2, 0, 1, 0, 0};  // plane of B in product q
 #pragma unroll
   for (int i = 0; i < NA; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) c[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < NB; j++) c[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int q = 6 - np; q < 6; q++)
 #pragma unroll
     for (int i = 0; i < NA; i++)
 #pragma unroll
-      for (int j = 0; j < 2; j++) c[i][j] = mfma(A[i][pa[q]], B[j][pb[q]], c[i][j]);
+      for (int j = 0; j < NB; j++) c[i][j] = mfma(A[i][pa[q]], B[j][pb[q]], c[i][j]);
 }
 
 __device__ __forceinline__ void flush(d4& acc, const f4 c) {
@@ -433,9 +433,9 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
 // LDS.  While the waves multiply group g they produce group g + 1: wave w
 // runs the recurrence of block half w % 2 for the 16 degrees
 // lp + 16 (w / 2) .. + 15, starting from the checkpoint there, so every wave
-// does the same share with no branches.  Wave w multiplies degree tiles
-// 2*(w/4) and 2*(w/4)+1 with column tiles 2*(w%4) and 2*(w%4)+1, so each
-// wave ends a pass with complete sums over all colatitudes and writes them
+// does the same share with no branches.  Wave w multiplies the 4 degree
+// tiles with column tile w (so one wave loads each B operand block), and
+// ends a pass with complete sums over all colatitudes, which it writes
 // directly.
 // ---------------------------------------------------------------------------
 template <int NS, int REAL>
@@ -519,57 +519,45 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   };
 
   // Consumer role.
-  const int lh = wave >> 2;
-  const int tile0 = csg * 8 + (wave & 3) * 2;
+  // Wave w multiplies all 4 degree tiles with column tile w, so every B
+  // operand block is loaded by one wave of the workgroup (the A operands
+  // come from LDS).
+  const int tile = csg * 8 + wave;
   const uint4* bslot = p.bs + (long)slot * p.nkc * p.ntile * NS * 64 + lane;
-  // Column tiles past ntile read the zero block p.bzero at every chunk (a
+  // A column tile past ntile reads the zero block p.bzero at every chunk (a
   // base address and a zero chunk stride, chosen once).  Callers pass chunks
   // below nkc only.
-  const uint4* bt[2];
-  long bstr[2];
+  const bool tvalid = tile < p.ntile;
+  const uint4* bt = tvalid ? bslot + (long)tile * NS * 64 : p.bzero + lane;
+  const long bstr = tvalid ? (long)p.ntile * NS * 64 : 0;
+  auto loadB = [&](int kc, uint4 (&B)[1][NS]) {
 #pragma unroll
-  for (int i = 0; i < 2; i++) {
-    const bool tv = tile0 + i < p.ntile;
-    bt[i] = tv ? bslot + (long)(tile0 + i) * NS * 64 : p.bzero + lane;
-    bstr[i] = tv ? (long)p.ntile * NS * 64 : 0;
-  }
-  auto loadB = [&](int kc, uint4 (&B)[2][NS]) {
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-      B[0][s] = bt[0][kc * bstr[0] + s * 64];
-      B[1][s] = bt[1][kc * bstr[1] + s * 64];
-    }
+    for (int s = 0; s < NS; s++) B[0][s] = bt[kc * bstr + s * 64];
   };
 
-  d4 acc[2][2];
+  d4 acc[4][1];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int i = 0; i < 4; i++) acc[i][0] = d4{0.0, 0.0, 0.0, 0.0};
   // One K chunk kc of buffer buf; the chains of the previous chunk (cp, zero
   // before the first) are flushed to FP64 after this chunk's MFMAs are issued.
-  f4 cp[2][2];
+  f4 cp[4][1];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < 4; i++) cp[i][0] = f4{0.f, 0.f, 0.f, 0.f};
+  auto step = [&](int buf, int kc, const uint4 (&B)[1][NS]) {
+    uint4 A[4][NS];
 #pragma unroll
-    for (int j = 0; j < 2; j++) cp[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  auto step = [&](int buf, int kc, const uint4 (&B)[2][NS]) {
-    uint4 A[2][NS];
-#pragma unroll
-    for (int lt = 0; lt < 2; lt++)
+    for (int lt = 0; lt < 4; lt++)
 #pragma unroll
       for (int s = 0; s < NS; s++)
         A[lt][s] = *reinterpret_cast<const uint4*>(
-            &Dl[buf][s][(2 * lh + lt) * 16 + (lane & 15)][(kc & 3) * 32 + 8 * (lane >> 4)]);
-    f4 c[2][2];
-    split_mma<NS, 2>(A, B, c);
+            &Dl[buf][s][lt * 16 + (lane & 15)][(kc & 3) * 32 + 8 * (lane >> 4)]);
+    f4 c[4][1];
+    split_mma<NS, 4, 1>(A, B, c);
 #pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int j = 0; j < 2; j++) {
-        flush(acc[i][j], cp[i][j]);
-        cp[i][j] = c[i][j];
-      }
+    for (int i = 0; i < 4; i++) {
+      flush(acc[i][0], cp[i][0]);
+      cp[i][0] = c[i][0];
+    }
   };
 
   prefetch(0);
@@ -584,7 +572,7 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   // group but the last has four chunks below nkc (nt > 128 (ngrp-1)), and the
   // last group's first chunk is below nkc too.  The scheduling barriers keep
   // each chunk's loads, production part and MFMAs together.
-  uint4 B0[2][NS], B1[2][NS];
+  uint4 B0[1][NS], B1[1][NS];
   loadB(0, B0);
   for (int g = 0; g + 1 < ngrp; g++) {
     const int buf = g & 1;
@@ -627,30 +615,25 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
       }
     }
 #pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int j = 0; j < 2; j++) flush(acc[i][j], cp[i][j]);
+    for (int i = 0; i < 4; i++) flush(acc[i][0], cp[i][0]);
   }
 
-  // Rows l = lp + 16*(2*lh + lt) + 4*(lane/16) + r, column 16*tile + lane%16.
+  // Rows l = lp + 16*lt + 4*(lane/16) + r, column 16*tile + lane%16.
+  const int col = tile * 16 + (lane & 15);
+  if (col >= p.ncol) return;
+  const int lf = field_lmax(p, col >> 1);
+  const long cbase = (long)field_index(p.fmap, col >> 1) * p.cstride;
 #pragma unroll
-  for (int lt = 0; lt < 2; lt++)
+  for (int lt = 0; lt < 4; lt++)
 #pragma unroll
-    for (int ct = 0; ct < 2; ct++) {
-      const int col = (tile0 + ct) * 16 + (lane & 15);
-      if (col >= p.ncol) continue;
-      const int lf = field_lmax(p, col >> 1);
-      const long cbase = (long)field_index(p.fmap, col >> 1) * p.cstride;
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int l = lp + 16 * (2 * lh + lt) + 4 * (lane >> 4) + r;
-        if (l > lf) continue;
-        const long ci = (cbase + coef_index(l, m, p.nabs, REAL)) * 2 + (col & 1);
-        double v = acc[lt][ct][r];
-        if (p.accumulate) v += p.cout[ci];
-        if (p.zero_last && lf == p.L && l == lf && m == lf) v = 0.0;
-        p.cout[ci] = v;
-      }
+    for (int r = 0; r < 4; r++) {
+      const int l = lp + 16 * lt + 4 * (lane >> 4) + r;
+      if (l > lf) continue;
+      const long ci = (cbase + coef_index(l, m, p.nabs, REAL)) * 2 + (col & 1);
+      double v = acc[lt][0][r];
+      if (p.accumulate) v += p.cout[ci];
+      if (p.zero_last && lf == p.L && l == lf && m == lf) v = 0.0;
+      p.cout[ci] = v;
     }
 }
 
