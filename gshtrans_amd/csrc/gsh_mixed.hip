@@ -219,8 +219,9 @@ __global__ __launch_bounds__(256) void bin_split(LegParams p, long nblocks) {
 // degrees of half w / 4, starting from the checkpoint there, so every wave
 // does the same share.  Wave w multiplies colatitude half h = w / 4 (8 MFMA
 // row groups) with column tiles 2 (w % 4) and 2 (w % 4) + 1.  Each row
-// group's MFMAs share a scheduling region with one recurrence pair, so the
-// VALU and MFMA work interleave.
+// group's MFMAs share a scheduling region with one recurrence pair and the
+// LDS loads of the next group's A operands, so the VALU work, the MFMAs and
+// the LDS latency overlap.
 // ---------------------------------------------------------------------------
 template <int NS, int REAL>
 __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
@@ -326,26 +327,21 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
             make_uint4(pl[s][0], pl[s][1], pl[s][2], pl[s][3]);
     }
   };
-  // Row groups g and g + 1: their MFMAs as four independent chains, then the
-  // flush to FP64.  A pair is skipped when its first group holds no
-  // colatitude; otherwise both are computed, and the rows past ntheta (their
-  // D planes may be stale) only reach accumulator rows that are never stored
-  // (each MFMA output row depends on its own A row only).
-  auto consume_groups = [&](int buf, int g, const uint4 (&B)[2][NS], bool valid) {
-    if (valid) {
-      uint4 A[2][NS];
+  // A operands of row group g (one uint4 per plane; always in bounds), and
+  // row group g's MFMAs (two independent chains) with their flush to FP64.
+  // A group whose rows hold no colatitude is skipped.  The A operands of
+  // group g + 1 are loaded while group g multiplies (ping-pong registers, so
+  // the LDS latency hides behind the MFMAs).
+  auto loadA = [&](int buf, int g, uint4 (&A)[1][NS]) {
 #pragma unroll
-      for (int u = 0; u < 2; u++)
-#pragma unroll
-        for (int s = 0; s < NS; s++)
-          A[u][s] = trb[buf][s][lane >> 4][h * 128 + (g + u) * 16 + (lane & 15)];
-      f4 c[2][2];
-      split_mma<NS, 2>(A, B, c);
-#pragma unroll
-      for (int u = 0; u < 2; u++) {
-        flush(acc[g + u][0], c[u][0]);
-        flush(acc[g + u][1], c[u][1]);
-      }
+    for (int s = 0; s < NS; s++) A[0][s] = trb[buf][s][lane >> 4][h * 128 + g * 16 + (lane & 15)];
+  };
+  auto consume_group = [&](int g, const uint4 (&A)[1][NS], const uint4 (&B)[2][NS]) {
+    if (h * 128 + g * 16 < rows) {
+      f4 c[1][2];
+      split_mma<NS, 1>(A, B, c);
+      flush(acc[g][0], c[0][0]);
+      flush(acc[g][1], c[0][1]);
     }
   };
   auto loadB = [&](int buf, uint4 (&B)[2][NS]) {
@@ -386,13 +382,15 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
     nx = ck_state(2 * (c + 2) + ph);
     uint4 B[2][NS];
     loadB(buf, B);
+    uint4 Ab[2][1][NS];
+    loadA(buf, 0, Ab[0]);
+    // Region g: load group g + 1's A operands, produce pair g, multiply
+    // group g.
 #pragma unroll
-    for (int g = 0; g < 8; g += 2) {
-      if (pv) {
-        produce_pair(buf ^ 1, buf ^ 1, g);
-        produce_pair(buf ^ 1, buf ^ 1, g + 1);
-      }
-      consume_groups(buf, g, B, h * 128 + g * 16 < rows);
+    for (int g = 0; g < 8; g++) {
+      if (g < 7) loadA(buf, g + 1, Ab[(g + 1) & 1]);
+      if (pv) produce_pair(buf ^ 1, buf ^ 1, g);
+      consume_group(g, Ab[g & 1], B);
       __builtin_amdgcn_sched_barrier(0);
     }
     store(buf ^ 1, q);  // buffers last read in chunk c - 1
@@ -402,8 +400,13 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
     const int buf = (nchunk - 1) & 1;
     uint4 B[2][NS];
     loadB(buf, B);
+    uint4 Ab[2][1][NS];
+    loadA(buf, 0, Ab[0]);
 #pragma unroll
-    for (int g = 0; g < 8; g += 2) consume_groups(buf, g, B, h * 128 + g * 16 < rows);
+    for (int g = 0; g < 8; g++) {
+      if (g < 7) loadA(buf, g + 1, Ab[(g + 1) & 1]);
+      consume_group(g, Ab[g & 1], B);
+    }
   }
 
   // C layout of the 16x16x32 MFMA: lane holds rows 4*(lane/16)+r, column
