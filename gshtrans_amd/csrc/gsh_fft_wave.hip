@@ -231,49 +231,141 @@ __device__ __forceinline__ void st2(double* p, double2 v) {
   *reinterpret_cast<double2*>(p) = v;
 }
 
+// Order bins are exchanged between the rings of a workgroup through LDS, so
+// that every global access of F / FA covers whole 128-byte lines.  Round r of
+// H = N / LPW rounds covers LPW = lds_per_wave(R) bins (ring bins k = r LPW +
+// b for complex rings; bins k = r LPW / 2 + b, xa and xb apart, when the
+// analysis separates a real pair).  Thread tid moves bins b = 64 i + tid / 8
+// of ring c = tid % 8; ring c's entry b sits at lds[c LPW + (b ^ 2c)], which
+// spreads every 16-lane phase of the b128 accesses over all 64 banks, both
+// for the 8 rings x 2 bins of a cooperative access and for the 16
+// consecutive bins a ring reads or writes.
+__device__ __forceinline__ int xch(int c, int b) { return b ^ (2 * c); }
+
 // Synthesis: order bins of F[theta][slot][col] -> backward DFT -> field ring.
+// Every wave stays to the end (the bin exchange synchronises the workgroup);
+// a wave without a field transforms zeros and stores nothing.  All global
+// loads of the exchange are issued before its first barrier.
+//  - Complex rings: bin k holds order m = k (k <= lmax) or m = k - N
+//    (k >= N - lmax), one load per bin; bin N/2 holds both when lmax = N/2,
+//    the second added by the threads of bin offset 0 (N/2 is a multiple of
+//    64).
+//  - Real pairs: the exchange moves the slots kk < N/2 (xa, xb apart, BH
+//    per round); register n1 of bin k = 64 n1 + lane takes slot kk = k for
+//    k < N/2 and the conjugates of slot N - k for k > N/2 (c2r symmetry);
+//    slot N/2 (lane 0, register R/2) is loaded by the ring itself.  c2r
+//    ignores Im of bins 0 and N/2.
 template <int R, int REAL, int F32>
 __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_synth_w(FFTParams p) {
   constexpr int W = waves_for(R);
-  __shared__ double2 lds_all[W * lds_per_wave(R)];
+  constexpr int LPW = lds_per_wave(R);
   constexpr int N = 64 * R;
+  constexpr int H = N / LPW;      // exchange rounds
+  constexpr int CPT = LPW / 64;   // complex rings: bins per thread and round
+  constexpr int BH = LPW / 2;     // real pairs: slots per round
+  constexpr int CPH = BH / 64;
+  static_assert(W == 8 && LPW % 128 == 0, "8 rings x 16 bytes = one line of F per bin");
+  __shared__ double2 lds_all[W * LPW];
   const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
   const int fg = blockIdx.x;
   const int theta = blockIdx.y;
   const int f0 = (fg * W + wave) * (REAL ? 2 : 1);
-  if (f0 >= p.nfield) return;
+  const bool fv = f0 < p.nfield;
   const bool f1v = REAL && f0 + 1 < p.nfield;
-  const int lmax = p.lmax;
-  // Order bins of this field: slot s at Frow[16 s] (F[theta][col/16][slot][16]).
-  const double* Frow = p.F + f_index(theta, 0, 2 * f0, p.nslot, (p.ncol + 15) >> 4);
+  const int lmax = p.lmax, m_first = p.m_first;
+  const int ncg16 = (p.ncol + 15) >> 4;
+  // This thread's share of the cooperative gather: ring c, bins 64 i + bq.
+  const int c = tid & 7, bq = tid >> 3;
+  const int cf = (fg * W + c) * (REAL ? 2 : 1);
+  const bool cv = cf < p.nfield, cv1 = REAL && cf + 1 < p.nfield;
+  const double* Fc = p.F + f_index(theta, 0, 2 * cf, p.nslot, ncg16);
   double2 v[R];
+  if (!REAL) {
+    double2 st[H][CPT];
 #pragma unroll
-  for (int n1 = 0; n1 < R; n1++) {
-    const int k = 64 * n1 + lane;
-    double2 X = make_double2(0.0, 0.0);
-    if (!REAL) {
-      if (k <= lmax) X = ld2(Frow + 16 * (long)(k - p.m_first));
-      if (k >= N - lmax) X = cadd(X, ld2(Frow + 16 * (long)(k - N - p.m_first)));
-    } else {
-      const int kk = k <= N / 2 ? k : N - k;
-      if (kk <= lmax) {
-        double2 xa = ld2(Frow + 16 * (long)kk);
-        double2 xb = f1v ? ld2(Frow + 16 * (long)kk + 2) : make_double2(0.0, 0.0);
-        // c2r reads bins 0..N/2 only and ignores Im of bins 0 and N/2.
-        if (kk == 0 || kk == N / 2) {
+    for (int r = 0; r < H; r++)
+#pragma unroll
+      for (int i = 0; i < CPT; i++) {
+        const int k = r * LPW + 64 * i + bq;
+        const bool lo = k <= lmax;
+        st[r][i] = make_double2(0.0, 0.0);
+        if (cv && (lo || k >= N - lmax)) st[r][i] = ld2(Fc + 16 * (long)(lo ? k - m_first : k - N - m_first));
+      }
+    constexpr int RH = (N / 2) / LPW, IH = ((N / 2) % LPW) / 64;  // bin N/2
+    const bool both = cv && bq == 0 && N / 2 <= lmax;
+    double2 xm = make_double2(0.0, 0.0);
+    if (both) xm = ld2(Fc + 16 * (long)(-N / 2 - m_first));
+#pragma unroll
+    for (int r = 0; r < H; r++) {
+      if (r > 0) __syncthreads();  // round r - 1 read back
+      if (r == RH) st[RH][IH] = cadd(st[RH][IH], xm);
+#pragma unroll
+      for (int i = 0; i < CPT; i++) lds_all[c * LPW + xch(c, 64 * i + bq)] = st[r][i];
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < CPT; j++)
+        v[r * CPT + j] = lds_all[wave * LPW + xch(wave, 64 * j + lane)];
+    }
+  } else {
+    double2 sa[H][CPH], sb[H][CPH];
+#pragma unroll
+    for (int r = 0; r < H; r++)
+#pragma unroll
+      for (int i = 0; i < CPH; i++) {
+        const int kk = r * BH + 64 * i + bq;
+        sa[r][i] = make_double2(0.0, 0.0);
+        sb[r][i] = make_double2(0.0, 0.0);
+        if (cv && kk <= lmax) sa[r][i] = ld2(Fc + 16 * (long)kk);
+        if (cv1 && kk <= lmax) sb[r][i] = ld2(Fc + 16 * (long)kk + 2);
+      }
+    // Slot N/2 of this ring's pair (register R/2, lane 0).
+    double2 na = make_double2(0.0, 0.0), nb = make_double2(0.0, 0.0);
+    if (fv && lane == 0 && N / 2 <= lmax) {
+      const double* Fw = p.F + f_index(theta, 0, 2 * f0, p.nslot, ncg16);
+      na = ld2(Fw + 16 * (long)(N / 2));
+      if (f1v) nb = ld2(Fw + 16 * (long)(N / 2) + 2);
+    }
+#pragma unroll
+    for (int r = 0; r < H; r++) {
+      if (r > 0) __syncthreads();
+#pragma unroll
+      for (int i = 0; i < CPH; i++) {
+        lds_all[c * LPW + xch(c, 64 * i + bq)] = sa[r][i];
+        lds_all[c * LPW + BH + xch(c, 64 * i + bq)] = sb[r][i];
+      }
+      __syncthreads();
+      const double2* lw = lds_all + wave * LPW;
+#pragma unroll
+      for (int j = 0; j < CPH; j++) {  // k = kk = r BH + 64 j + lane < N/2
+        double2 xa = lw[xch(wave, 64 * j + lane)];
+        double2 xb = lw[BH + xch(wave, 64 * j + lane)];
+        if (r == 0 && j == 0 && lane == 0) {
           xa.y = 0.0;
           xb.y = 0.0;
-        } else if (k > N / 2) {
-          xa.y = -xa.y;
-          xb.y = -xb.y;
         }
-        X = make_double2(xa.x - xb.y, xa.y + xb.x);  // xa + i xb
+        v[r * CPH + j] = make_double2(xa.x - xb.y, xa.y + xb.x);  // xa + i xb
+      }
+      // k = 64 n1 + lane > N/2 with slot N - k in this round.  Register
+      // R - r CPH (r > 0) already holds its lanes >= 1 from round r - 1; the
+      // others are first touched here (lane 0 of register R/2: slot N/2).
+#pragma unroll
+      for (int n1 = R - (r + 1) * CPH; n1 <= R - r * CPH && n1 < R; n1++) {
+        const int kk = N - 64 * n1 - lane;
+        const bool in = kk >= r * BH && kk < (r + 1) * BH;
+        const double2 xa = lw[xch(wave, in ? kk - r * BH : 0)];
+        const double2 xb = lw[BH + xch(wave, in ? kk - r * BH : 0)];
+        const double2 X = make_double2(xa.x + xb.y, xb.x - xa.y);  // conj xa + i conj xb
+        const double2 keep = n1 == R - r * CPH ? v[n1] : make_double2(0.0, 0.0);
+        v[n1] = in ? X : keep;
       }
     }
-    v[n1] = X;
+    if (lane == 0) v[R / 2] = make_double2(na.x, nb.x);
   }
-  wave_fft<R, +1>(v, lds_all + wave * lds_per_wave(R), p.twa, p.tw, lane);
+  // Each wave now works in its own LDS region only.
+  wave_fft<R, +1>(v, lds_all + wave * LPW, p.twa, p.tw, lane);
+  if (!fv) return;
   const long oa = (long)field_index(p.fmap, f0) * p.fstride + (long)theta * N + lane;
   if (!REAL) {
 #pragma unroll
@@ -289,86 +381,144 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_synth_w(FFTParams p)
   }
 }
 
+// Separation of a real pair transformed as Z = FFT(x_a + i x_b), weighted by
+// w: X_a[k] = (Z[k] + conj Z[N-k]) / 2, X_b[k] = (Z[k] - conj Z[N-k]) / (2 i)
+// for k = lane + 64 t <= N/2.  Z[N - k] sits in lane (64 - lane) % 64,
+// register R-1-t (lane 0: its own register (R - t) % R).  All lanes take
+// part (ds_bpermute).
+template <int R>
+__device__ __forceinline__ void separate(const double2 (&v)[R], int t, int lane, double w,
+                                         bool adjoint, double2& xa, double2& xb) {
+  constexpr int N = 64 * R;
+  const int src = ((64 - lane) & 63) << 2;
+  const double2 q = v[R - 1 - t];
+  const int2 qx = __builtin_bit_cast(int2, q.x), qy = __builtin_bit_cast(int2, q.y);
+  double2 zp;
+  zp.x = __builtin_bit_cast(double, make_int2(__builtin_amdgcn_ds_bpermute(src, qx.x),
+                                              __builtin_amdgcn_ds_bpermute(src, qx.y)));
+  zp.y = __builtin_bit_cast(double, make_int2(__builtin_amdgcn_ds_bpermute(src, qy.x),
+                                              __builtin_amdgcn_ds_bpermute(src, qy.y)));
+  if (lane == 0) zp = v[(R - t) % R];
+  const int k = lane + 64 * t;
+  const double2 z = v[t];
+  // Adjoint of the c2r synthesis: weight 2 for 0 < k < N/2, real part only
+  // at k = 0 and N/2.
+  const bool edge = k == 0 || 2 * k == N;
+  const double hw = (adjoint && !edge ? 1.0 : 0.5) * w;
+  const double im = adjoint && edge ? 0.0 : 1.0;
+  xa = make_double2((z.x + zp.x) * hw, (z.y - zp.y) * hw * im);
+  xb = make_double2((z.y + zp.y) * hw, (zp.x - z.x) * hw * im);
+}
+
 // Analysis: field ring -> forward DFT -> weighted bins into the analysis
-// layout FA (fa_index); the last ring also zeroes the padding colatitudes
-// of the last quad.
+// layout FA (fa_index), exchanged through LDS as in the synthesis; the last
+// ring row also zeroes the padding colatitudes of the last quad.  A wave
+// without a field transforms zeros and stores nothing.
 template <int R, int REAL, int F32>
 __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) {
   constexpr int W = waves_for(R);
-  __shared__ double2 lds_all[W * lds_per_wave(R)];
+  constexpr int LPW = lds_per_wave(R);
   constexpr int N = 64 * R;
+  constexpr int H = N / LPW;      // exchange rounds
+  constexpr int CPT = LPW / 64;   // complex rings: bins per thread and round
+  constexpr int BH = LPW / 2;     // real pairs: bins per round (xa, xb apart)
+  constexpr int CPH = BH / 64;
+  static_assert(W == 8 && LPW % 128 == 0, "8 rings x 16 bytes = one line of FA per bin");
+  __shared__ double2 lds_all[W * LPW];
   const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
   const int fg = blockIdx.x;
   const int theta = blockIdx.y;
   const int f0 = (fg * W + wave) * (REAL ? 2 : 1);
-  if (f0 >= p.nfield) return;
+  const bool fv = f0 < p.nfield;
   const bool f1v = REAL && f0 + 1 < p.nfield;
-  const int lmax = p.lmax;
+  const int lmax = p.lmax, m_first = p.m_first;
   double2 v[R];
-  const long ia = (long)field_index(p.fmap, f0) * p.fstride + (long)theta * N + lane;
-  if (!REAL) {
+  if (fv) {
+    const long ia = (long)field_index(p.fmap, f0) * p.fstride + (long)theta * N + lane;
+    if (!REAL) {
 #pragma unroll
-    for (int n1 = 0; n1 < R; n1++) v[n1] = ld_cplx<F32>(p.fin, ia + 64 * n1);
-  } else {
-    const long ib =
-        f1v ? (long)field_index(p.fmap, f0 + 1) * p.fstride + (long)theta * N + lane : ia;
+      for (int n1 = 0; n1 < R; n1++) v[n1] = ld_cplx<F32>(p.fin, ia + 64 * n1);
+    } else {
+      const long ib =
+          f1v ? (long)field_index(p.fmap, f0 + 1) * p.fstride + (long)theta * N + lane : ia;
 #pragma unroll
-    for (int n1 = 0; n1 < R; n1++) {
-      const double xa = ld_real<F32>(p.fin, ia + 64 * n1);
-      const double xb = ld_real<F32>(p.fin, ib + 64 * n1);
-      v[n1] = make_double2(xa, f1v ? xb : 0.0);
+      for (int n1 = 0; n1 < R; n1++) {
+        const double xa = ld_real<F32>(p.fin, ia + 64 * n1);
+        const double xb = ld_real<F32>(p.fin, ib + 64 * n1);
+        v[n1] = make_double2(xa, f1v ? xb : 0.0);
+      }
     }
+  } else {
+#pragma unroll
+    for (int n1 = 0; n1 < R; n1++) v[n1] = make_double2(0.0, 0.0);
   }
-  wave_fft<R, -1>(v, lds_all + wave * lds_per_wave(R), p.twa, p.tw, lane);
+  double2* lw = lds_all + wave * LPW;
+  wave_fft<R, -1>(v, lw, p.twa, p.tw, lane);
   const double w = p.wq[theta];
   const int n_cg = p.n_cg, nquad = p.nquad;
+  // This thread's share of the cooperative scatter: ring c, bins 64 i + bq.
+  const int c = tid & 7, bq = tid >> 3;
+  const int cf = (fg * W + c) * (REAL ? 2 : 1);
+  const bool cv = cf < p.nfield, cv1 = REAL && cf + 1 < p.nfield;
   if (!REAL) {
-    const int col = 2 * f0;
 #pragma unroll
-    for (int t = 0; t < R; t++) {
-      const int k = lane + 64 * t;
-      const double2 X = make_double2(v[t].x * w, v[t].y * w);
-      if (k <= lmax) st2(p.F + fa_index(k - p.m_first, col, theta, n_cg, nquad), X);
-      if (k >= N - lmax) st2(p.F + fa_index(k - N - p.m_first, col, theta, n_cg, nquad), X);
+    for (int r = 0; r < H; r++) {
+      if (r > 0) __syncthreads();  // round r - 1 stored
+#pragma unroll
+      for (int j = 0; j < CPT; j++)
+        lw[xch(wave, 64 * j + lane)] = make_double2(v[r * CPT + j].x * w, v[r * CPT + j].y * w);
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < CPT; i++) {
+        const int b = 64 * i + bq;
+        const int k = r * LPW + b;
+        const double2 X = lds_all[c * LPW + xch(c, b)];
+        if (cv && k <= lmax) st2(p.F + fa_index(k - m_first, 2 * cf, theta, n_cg, nquad), X);
+        if (cv && k >= N - lmax)
+          st2(p.F + fa_index(k - N - m_first, 2 * cf, theta, n_cg, nquad), X);
+      }
     }
   } else {
-    // Z = FFT(x_a + i x_b): X_a[k] = (Z[k] + conj Z[N-k]) / 2,
-    // X_b[k] = (Z[k] - conj Z[N-k]) / (2 i), k <= N/2.  Z[N - k] with
-    // k = lane + 64 t sits in lane (64 - lane) % 64, register R-1-t (lane 0:
-    // its own register (R - t) % R).
-    const int src = ((64 - lane) & 63) << 2;
-    const int col = 2 * f0;
 #pragma unroll
-    for (int t = 0; t <= R / 2; t++) {
-      const double2 q = v[R - 1 - t];
-      const int2 qx = __builtin_bit_cast(int2, q.x), qy = __builtin_bit_cast(int2, q.y);
-      double2 zp;
-      zp.x = __builtin_bit_cast(double, make_int2(__builtin_amdgcn_ds_bpermute(src, qx.x),
-                                                  __builtin_amdgcn_ds_bpermute(src, qx.y)));
-      zp.y = __builtin_bit_cast(double, make_int2(__builtin_amdgcn_ds_bpermute(src, qy.x),
-                                                  __builtin_amdgcn_ds_bpermute(src, qy.y)));
-      if (lane == 0) zp = v[(R - t) % R];
-      const int k = lane + 64 * t;
-      if (k <= lmax) {
-        const double2 z = v[t];
-        // Adjoint of the c2r synthesis: weight 2 for 0 < k < N/2, real
-        // part only at k = 0 and N/2.
-        const bool edge = k == 0 || 2 * k == N;
-        const double hw = (p.adjoint && !edge ? 1.0 : 0.5) * w;
-        const double im = p.adjoint && edge ? 0.0 : 1.0;
-        const double2 xa = make_double2((z.x + zp.x) * hw, (z.y - zp.y) * hw * im);
-        const double2 xb = make_double2((z.y + zp.y) * hw, (zp.x - z.x) * hw * im);
-        st2(p.F + fa_index(k, col, theta, n_cg, nquad), xa);
-        if (f1v) st2(p.F + fa_index(k, col + 2, theta, n_cg, nquad), xb);
+    for (int r = 0; r < H; r++) {
+      if (r > 0) __syncthreads();
+#pragma unroll
+      for (int j = 0; j < CPH; j++) {
+        double2 xa, xb;
+        separate<R>(v, r * CPH + j, lane, w, p.adjoint, xa, xb);
+        lw[xch(wave, 64 * j + lane)] = xa;
+        lw[BH + xch(wave, 64 * j + lane)] = xb;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < CPH; i++) {
+        const int b = 64 * i + bq;
+        const int k = r * BH + b;
+        if (cv && k <= lmax) {
+          st2(p.F + fa_index(k, 2 * cf, theta, n_cg, nquad), lds_all[c * LPW + xch(c, b)]);
+          if (cv1)
+            st2(p.F + fa_index(k, 2 * cf + 2, theta, n_cg, nquad),
+                lds_all[c * LPW + BH + xch(c, b)]);
+        }
+      }
+    }
+    // Bin N/2 (lane 0, register R/2) exists only when lmax = N/2.
+    if (fv && N / 2 <= lmax) {
+      double2 xa, xb;
+      separate<R>(v, R / 2, lane, w, p.adjoint, xa, xb);
+      if (lane == 0) {
+        st2(p.F + fa_index(N / 2, 2 * f0, theta, n_cg, nquad), xa);
+        if (f1v) st2(p.F + fa_index(N / 2, 2 * f0 + 2, theta, n_cg, nquad), xb);
       }
     }
   }
-  if (theta == p.ntheta - 1) {
+  if (fv && theta == p.ntheta - 1) {
     const int ncw = REAL && f1v ? 4 : 2;
     for (int tp = theta + 1; tp < 4 * nquad; tp++)
       for (int s = lane; s < p.nslot; s += 64)
-        for (int c = 0; c < ncw; c++) p.F[fa_index(s, 2 * f0 + c, tp, n_cg, nquad)] = 0.0;
+        for (int cc = 0; cc < ncw; cc++) p.F[fa_index(s, 2 * f0 + cc, tp, n_cg, nquad)] = 0.0;
   }
 }
 

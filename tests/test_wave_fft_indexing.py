@@ -10,7 +10,14 @@ fft_anal_w), re-traced lane by lane in numpy:
   touches 16 distinct 16-byte bank groups (no bank conflicts on 64 banks);
 * the real-pair unpacking X_a = (Z[k] + conj Z[N-k]) / 2,
   X_b = (Z[k] - conj Z[N-k]) / 2i with the partner lane / register used by
-  the kernel reproduces the FFTs of the two real rings.
+  the kernel reproduces the FFTs of the two real rings;
+* the exchange of order bins between the 8 rings of a workgroup through LDS
+  (fft_synth_w gather, fft_anal_w scatter): every LDS entry of a round is
+  written exactly once and stays inside the workgroup buffer, every 16-lane
+  phase is free of bank conflicts, each ring receives exactly the bins the
+  per-ring formulas give (complex rings, real pairs, lmax below and at N/2,
+  rings without a field), and every wave instruction of the cooperative
+  global access covers whole 128-byte lines of F / FA.
 """
 import numpy as np
 import pytest
@@ -139,3 +146,287 @@ def test_real_pair_split(R):
         xb_k = (Z[k] - np.conj(q)) / 2j
         assert np.max(np.abs(xa_k[m] - Fa[k[m]])) < 1e-12 * np.sqrt(N)
         assert np.max(np.abs(xb_k[m] - Fb[k[m]])) < 1e-12 * np.sqrt(N)
+
+
+
+# ---------------------------------------------------------------------------
+# Bin exchange between the rings of a workgroup (fft_synth_w / fft_anal_w).
+# ---------------------------------------------------------------------------
+def lds_per_wave(R):
+    return (R // 2 if R >= 32 else R) * 64
+
+
+def xch(c, b):
+    return b ^ (2 * c)
+
+
+def check_b128_phases(addr16):
+    """addr16: the 64 lane addresses, in 16-byte units, of one b128 access."""
+    for ph in range(4):
+        s = np.asarray(addr16[16 * ph:16 * ph + 16]) % 16
+        assert len(set(s.tolist())) == 16
+
+
+def check_whole_lines(lanes, line, piece):
+    """Global accesses of one wave instruction (or of the xa / xb pair of a
+    real pair): every 128-byte line touched by lanes `lanes` is covered by all
+    8 of its 16-byte pieces."""
+    got = {}
+    for t in np.flatnonzero(lanes):
+        got.setdefault(int(line[t]), set()).add(int(piece[t]))
+    for ln, pcs in got.items():
+        assert pcs == set(range(8)), (ln, sorted(pcs))
+
+
+CASES = [(R, real, part, nf) for R in (8, 16, 32) for real in (False, True)
+         for part in (False, True) for nf in ("full", "short")]
+
+
+def case_params(R, real, part, nf):
+    N = 64 * R
+    lmax = N // 2 - (37 if part else 0)
+    nfield = (16 if real else 8) if nf == "full" else (11 if real else 5)
+    m_first = 0 if real else -lmax
+    nslot = lmax + 1 if real else 2 * lmax + 1
+    return N, lmax, nfield, m_first, nslot
+
+
+def coop_threads(real, nfield):
+    """Cooperative share of thread tid: ring c, bins 64 i + bq; first column
+    2 cf of the ring in its workgroup's columns."""
+    tid = np.arange(512)
+    c, bq = tid & 7, tid >> 3
+    cf = c * (2 if real else 1)
+    return c, bq, cf, cf < nfield, real & (cf + 1 < nfield)
+
+
+@pytest.mark.parametrize("R,real,part,nf", CASES)
+def test_synthesis_bin_gather(R, real, part, nf):
+    N, lmax, nfield, m_first, nslot = case_params(R, real, part, nf)
+    LPW = lds_per_wave(R)
+    H, CPT, BH = N // LPW, LPW // 64, LPW // 2
+    CPH = BH // 64
+    rng = np.random.default_rng(R + 7 * real + 3 * part)
+    ncg = 2 if real else 1
+    F = rng.standard_normal((ncg, nslot, 16))   # one colatitude: F[cg][slot][16]
+    c, bq, cf, cv, cv1 = coop_threads(real, nfield)
+    col = 2 * cf
+    lane = np.arange(64)
+
+    def ld2(colv, slot, mask):
+        out = np.zeros(len(slot), complex)
+        cg, cc, sl = colv[mask] // 16, colv[mask] % 16, slot[mask]
+        out[mask] = F[cg, sl, cc] + 1j * F[cg, sl, cc + 1]
+        return out
+
+    def whole_lines(mask, colv, slot):
+        """mask / colv / slot over 512 threads, or 1024 for the xa, xb pair."""
+        if nf != "full":
+            return
+        n = len(mask) // 512
+        for wv in range(8):
+            sel = np.zeros(len(mask), bool)
+            for h in range(n):
+                sel[512 * h + 64 * wv:512 * h + 64 * wv + 64] = True
+            check_whole_lines(mask & sel, (colv // 16) * nslot + slot, (colv % 16) // 2)
+
+    v = np.full((8, R, 64), np.nan + 0j)
+    if not real:
+        st = {}
+        for r in range(H):
+            for i in range(CPT):
+                k = r * LPW + 64 * i + bq
+                lo = k <= lmax
+                m = cv & (lo | (k >= N - lmax))
+                sl = np.where(m, np.where(lo, k - m_first, k - N - m_first), 0)
+                whole_lines(m, col, sl)
+                st[r, i] = ld2(col, sl, m)
+        RH, IH = (N // 2) // LPW, ((N // 2) % LPW) // 64
+        both = cv & (bq == 0) & (N // 2 <= lmax)
+        st[RH, IH] = st[RH, IH] + ld2(col, np.where(both, -N // 2 - m_first, 0), both)
+        for r in range(H):
+            lds = np.full(8 * LPW, np.nan + 0j)
+            written = np.zeros(8 * LPW, int)
+            for i in range(CPT):
+                addr = c * LPW + xch(c, 64 * i + bq)
+                assert addr.min() >= 0 and addr.max() < 8 * LPW
+                for wv in range(8):
+                    check_b128_phases(addr[64 * wv:64 * wv + 64])
+                lds[addr] = st[r, i]
+                np.add.at(written, addr, 1)
+            assert (written == 1).all()
+            for w in range(8):
+                for j in range(CPT):
+                    a = w * LPW + xch(w, 64 * j + lane)
+                    check_b128_phases(a)
+                    v[w, r * CPT + j] = lds[a]
+    else:
+        sa, sb = {}, {}
+        for r in range(H):
+            for i in range(CPH):
+                kk = r * BH + 64 * i + bq
+                ma, mb = cv & (kk <= lmax), cv1 & (kk <= lmax)
+                ks = np.where(ma, kk, 0)
+                whole_lines(np.concatenate([ma, mb]), np.concatenate([col, col + 2]),
+                            np.concatenate([ks, ks]))
+                sa[r, i], sb[r, i] = ld2(col, ks, ma), ld2(col + 2, ks, mb)
+        for r in range(H):
+            lds = np.full(8 * LPW, np.nan + 0j)
+            written = np.zeros(8 * LPW, int)
+            for i in range(CPH):
+                addr = c * LPW + xch(c, 64 * i + bq)
+                assert addr.min() >= 0 and addr.max() + BH < 8 * LPW
+                for wv in range(8):
+                    check_b128_phases(addr[64 * wv:64 * wv + 64])
+                    check_b128_phases(addr[64 * wv:64 * wv + 64] + BH)
+                lds[addr], lds[addr + BH] = sa[r, i], sb[r, i]
+                np.add.at(written, addr, 1)
+                np.add.at(written, addr + BH, 1)
+            assert (written == 1).all()
+            for w in range(8):
+                for j in range(CPH):
+                    a = w * LPW + xch(w, 64 * j + lane)
+                    check_b128_phases(a)
+                    xa, xb = lds[a].copy(), lds[a + BH].copy()
+                    if r == 0 and j == 0:
+                        xa[0], xb[0] = xa[0].real, xb[0].real
+                    v[w, r * CPH + j] = xa + 1j * xb
+                for n1 in range(R - (r + 1) * CPH, min(R - r * CPH, R - 1) + 1):
+                    kk = N - 64 * n1 - lane
+                    m = (kk >= r * BH) & (kk < (r + 1) * BH)
+                    a = w * LPW + xch(w, np.where(m, kk - r * BH, 0))
+                    assert (a >= w * LPW).all() and (a < w * LPW + BH).all()
+                    xa, xb = lds[a], lds[a + BH]
+                    # first touch of the register unless round r - 1 set it
+                    keep = v[w, n1] if n1 == R - r * CPH else np.zeros(64, complex)
+                    v[w, n1] = np.where(m, np.conj(xa) + 1j * np.conj(xb), keep)
+        for w in range(8):   # slot N/2: lane 0 of register R/2, loaded by the ring
+            f0 = 2 * w
+            na = nb = 0
+            if f0 < nfield and N // 2 <= lmax:
+                cg, cc = (2 * f0) // 16, (2 * f0) % 16
+                na = F[cg, N // 2, cc]
+                nb = F[cg, N // 2, cc + 2] if f0 + 1 < nfield else 0
+            v[w, R // 2, 0] = na + 1j * nb
+    # Reference: the bins of ring w from the per-ring formulas.
+    for w in range(8):
+        f0 = w * (2 if real else 1)
+        if f0 >= nfield:
+            continue
+        for n1 in range(R):
+            k = 64 * n1 + lane
+            if not real:
+                ref = np.zeros(64, complex)
+                for m, sl in ((k <= lmax, k - m_first), (k >= N - lmax, k - N - m_first)):
+                    ref[m] += F[0, sl[m], 2 * f0] + 1j * F[0, sl[m], 2 * f0 + 1]
+            else:
+                kk = np.where(k <= N // 2, k, N - k)
+                m = kk <= lmax
+                ks = np.minimum(kk, nslot - 1)
+                cg, cc = (2 * f0) // 16, (2 * f0) % 16
+                xa = np.where(m, F[cg, ks, cc] + 1j * F[cg, ks, cc + 1], 0)
+                xb = (np.where(m, F[cg, ks, cc + 2] + 1j * F[cg, ks, cc + 3], 0)
+                      if f0 + 1 < nfield else np.zeros(64, complex))
+                edge = (kk == 0) | (kk == N // 2)
+                xa = np.where(edge, xa.real, np.where(k > N // 2, np.conj(xa), xa))
+                xb = np.where(edge, xb.real, np.where(k > N // 2, np.conj(xb), xb))
+                ref = xa + 1j * xb
+            assert np.array_equal(v[w, n1], ref), (w, n1)
+
+
+@pytest.mark.parametrize("R,real,part,nf", CASES)
+def test_analysis_bin_scatter(R, real, part, nf):
+    N, lmax, nfield, m_first, nslot = case_params(R, real, part, nf)
+    LPW = lds_per_wave(R)
+    H, CPT, BH = N // LPW, LPW // 64, LPW // 2
+    CPH = BH // 64
+    rng = np.random.default_rng(50 + R + 7 * real + 3 * part)
+    c, bq, cf, cv, cv1 = coop_threads(real, nfield)
+    lane = np.arange(64)
+    ncg = 2 if real else 1
+    FA = {}   # (slot, column) -> value
+
+    def store(mask, slot, colv, val):
+        for t in np.flatnonzero(mask):
+            key = (int(slot[t]), int(colv[t]))
+            assert key not in FA, key
+            FA[key] = val[t]
+
+    def whole_lines(masks_slots_cols):
+        if nf != "full":
+            return
+        for wv in range(8):
+            line, piece, sel = [], [], []
+            for mask, slot, colv in masks_slots_cols:
+                sl = slice(64 * wv, 64 * wv + 64)
+                sel.append(mask[sl])
+                line.append((colv[sl] // 16) * nslot + slot[sl])
+                piece.append((colv[sl] % 16) // 2)
+            check_whole_lines(np.concatenate(sel), np.concatenate(line), np.concatenate(piece))
+
+    if not real:
+        X = rng.standard_normal((8, N)) + 1j * rng.standard_normal((8, N))   # weighted bins
+        for r in range(H):
+            lds = np.full(8 * LPW, np.nan + 0j)
+            for w in range(8):
+                for j in range(CPT):
+                    a = w * LPW + xch(w, 64 * j + lane)
+                    check_b128_phases(a)
+                    assert np.isnan(lds[a]).all()
+                    lds[a] = X[w, 64 * (r * CPT + j) + lane]
+            for i in range(CPT):
+                b = 64 * i + bq
+                k = r * LPW + b
+                a = c * LPW + xch(c, b)
+                for wv in range(8):
+                    check_b128_phases(a[64 * wv:64 * wv + 64])
+                val = lds[a]
+                assert not np.isnan(val).any()
+                m1, m2 = cv & (k <= lmax), cv & (k >= N - lmax)
+                s1, s2 = np.where(m1, k - m_first, 0), np.where(m2, k - N - m_first, 0)
+                store(m1, s1, 2 * cf, val)
+                store(m2, s2, 2 * cf, val)
+                whole_lines([(m1, s1, 2 * cf)])
+                whole_lines([(m2, s2, 2 * cf)])
+        expect = {}
+        for w in range(min(8, nfield)):
+            for k in range(N):
+                if k <= lmax:
+                    expect[(k - m_first, 2 * w)] = X[w, k]
+                if k >= N - lmax:
+                    expect[(k - N - m_first, 2 * w)] = X[w, k]
+    else:
+        XA = rng.standard_normal((8, N // 2)) + 1j * rng.standard_normal((8, N // 2))
+        XB = rng.standard_normal((8, N // 2)) + 1j * rng.standard_normal((8, N // 2))
+        for r in range(H):
+            lds = np.full(8 * LPW, np.nan + 0j)
+            for w in range(8):
+                for j in range(CPH):
+                    kl = 64 * (r * CPH + j) + lane
+                    a = w * LPW + xch(w, 64 * j + lane)
+                    check_b128_phases(a)
+                    check_b128_phases(a + BH)
+                    lds[a], lds[a + BH] = XA[w, kl], XB[w, kl]
+            for i in range(CPH):
+                b = 64 * i + bq
+                k = r * BH + b
+                a = c * LPW + xch(c, b)
+                assert a.max() + BH < 8 * LPW
+                for wv in range(8):
+                    check_b128_phases(a[64 * wv:64 * wv + 64])
+                m = cv & (k <= lmax)
+                mb = m & cv1
+                store(m, k, 2 * cf, lds[a])
+                store(mb, k, 2 * cf + 2, lds[a + BH])
+                whole_lines([(m, k, 2 * cf), (mb, k, 2 * cf + 2)])
+        expect = {}
+        for w in range(8):
+            f0 = 2 * w
+            for k in range(min(lmax + 1, N // 2)):
+                if f0 < nfield:
+                    expect[(k, 2 * f0)] = XA[w, k]
+                if f0 + 1 < nfield:
+                    expect[(k, 2 * f0 + 2)] = XB[w, k]
+    assert FA.keys() == expect.keys()
+    for key, val in expect.items():
+        assert FA[key] == val, key
