@@ -937,7 +937,9 @@ int run_forward(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
   p.fmap = o.fmap;
   p.f_lmax = o.f_lmax;
   p.cg_lmax = o.cg_lmax;
-  const size_t fa = (size_t)nslot * ((ncol + 15) / 16) * ((g->ntheta() + 3) / 4) * 64;
+  // FA, plus 8 quads of 64 doubles: the split-BF16 analysis reads whole
+  // 32-colatitude chunks, up to 8 quads past the last block.
+  const size_t fa = (size_t)nslot * ((ncol + 15) / 16) * ((g->ntheta() + 3) / 4) * 64 + 8 * 64;
   if ((st = ensure_work(g, sizeof(double) * fa))) return st;
   q.F = p.F = g->work;
   if ((st = ensure_ck(g, d, n, s))) return st;
@@ -972,16 +974,19 @@ int run_forward(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
     p.items_per = w.per;
   }
   const int ns = precision == GSH_BF16X6 ? 3 : 2;
-  const int nkc = (g->ntheta() + 31) / 32;
-  const int ntile = (ncol + 15) / 16;
-  const long nblocks = (long)nslot * nkc * ntile;
-  if ((st = ensure_work2(g, (size_t)nblocks * ns * 64 * 16))) return st;
-  p.bs = static_cast<const uint4*>(g->work2);
-  p.bs_out = static_cast<uint4*>(g->work2);
-  p.nkc = nkc;
-  p.ntile = ntile;
-  GSH_HIP(timed(g, GSH_KERNEL_BIN_SPLIT, s,
-                [&] { return gsh::launch_bin_split(p, ns, nblocks, s); }));
+  if (ns == 2) {
+    // bf16x3 reads split operand blocks (bf16x6 splits FA in registers).
+    const int nkc = (g->ntheta() + 31) / 32;
+    const int ntile = (ncol + 15) / 16;
+    const long nblocks = (long)nslot * nkc * ntile;
+    if ((st = ensure_work2(g, (size_t)nblocks * ns * 64 * 16))) return st;
+    p.bs = static_cast<const uint4*>(g->work2);
+    p.bs_out = static_cast<uint4*>(g->work2);
+    p.nkc = nkc;
+    p.ntile = ntile;
+    GSH_HIP(timed(g, GSH_KERNEL_BIN_SPLIT, s,
+                  [&] { return gsh::launch_bin_split(p, ns, nblocks, s); }));
+  }
   GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_ANALYSIS, s,
                 [&] { return gsh::launch_leg_anal_bf16(p, ns, s); }));
   return GSH_OK;

@@ -524,18 +524,69 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   // Consumer role.
   // Wave w multiplies all 4 degree tiles with column tile w, so every B
   // operand block is loaded by one wave of the workgroup (the A operands
-  // come from LDS).
+  // come from LDS).  Lane (q4, c) = (lane / 16, lane % 16) of K chunk kc
+  // takes colatitudes 32 kc + 8 q4 + e (e = 0..7) of column 16 tile + c.
+  // Callers pass chunks below nkc only.
+  //  - bf16x6 (NS = 3): straight from the FP64 bins FA, i.e. FA entries 16 e
+  //    past the lane's base in quad 8 kc + 2 q4 (each e a whole 128-byte row
+  //    per 16 lanes), split into bf16 planes just before the chunk's MFMAs;
+  //    the six products per operand hide the longer loads.  A tile past the
+  //    last column group reads the last one: its outputs are never stored,
+  //    and a column only meets its own B values, so nothing else needs those
+  //    columns zero.  Reads of the last block run up to 8 quads past it (the
+  //    FA buffer is padded for that); colatitudes from ntheta on are zeroed
+  //    at the split of the last group's chunks.
+  //  - bf16x3 (NS = 2): from the split blocks of bin_split (p.bs); a tile
+  //    past ntile reads the zero block p.bzero at every chunk (a base address
+  //    and a zero chunk stride, chosen once).
+  constexpr bool kFA = NS == 3;
+  struct RawB {
+    double x[kFA ? 8 : 1];
+    uint4 b[kFA ? 1 : NS];
+  };
   const int tile = csg * 8 + wave;
-  const uint4* bslot = p.bs + (long)slot * p.nkc * p.ntile * NS * 64 + lane;
-  // A column tile past ntile reads the zero block p.bzero at every chunk (a
-  // base address and a zero chunk stride, chosen once).  Callers pass chunks
-  // below nkc only.
+  const double* fb =
+      p.F + (((long)slot * p.n_cg + min(tile, p.n_cg - 1)) * p.nquad + 2 * (lane >> 4)) * 64 +
+      (lane & 15);
   const bool tvalid = tile < p.ntile;
-  const uint4* bt = tvalid ? bslot + (long)tile * NS * 64 : p.bzero + lane;
+  const uint4* bt = kFA ? nullptr
+                        : tvalid ? p.bs + (long)slot * p.nkc * p.ntile * NS * 64 + lane +
+                                       (long)tile * NS * 64
+                                 : p.bzero + lane;
   const long bstr = tvalid ? (long)p.ntile * NS * 64 : 0;
-  auto loadB = [&](int kc, uint4 (&B)[1][NS]) {
+  auto loadB = [&](int kc, RawB& X) {
+    if constexpr (kFA) {
 #pragma unroll
-    for (int s = 0; s < NS; s++) B[0][s] = bt[kc * bstr + s * 64];
+      for (int e = 0; e < 8; e++) X.x[e] = fb[(long)kc * 512 + 16 * e];
+    } else {
+#pragma unroll
+      for (int s = 0; s < NS; s++) X.b[s] = bt[kc * bstr + s * 64];
+    }
+  };
+  auto splitB = [&](const RawB& X, uint4 (&B)[1][NS], int kc, bool last) {
+    if constexpr (kFA) {
+      unsigned pl[NS][4];
+      const int th = 32 * kc + 8 * (lane >> 4);  // colatitude of e = 0
+#pragma unroll
+      for (int e2 = 0; e2 < 4; e2++) {
+        double x0 = X.x[2 * e2], x1 = X.x[2 * e2 + 1];
+        if (last) {
+          x0 = th + 2 * e2 < nt ? x0 : 0.0;
+          x1 = th + 2 * e2 + 1 < nt ? x1 : 0.0;
+        }
+        unsigned o[NS];
+        split_pair<NS>(x0, x1, o);
+#pragma unroll
+        for (int s = 0; s < NS; s++) pl[s][e2] = o[s];
+      }
+#pragma unroll
+      for (int s = 0; s < NS; s++) B[0][s] = make_uint4(pl[s][0], pl[s][1], pl[s][2], pl[s][3]);
+    } else {
+      (void)kc;
+      (void)last;
+#pragma unroll
+      for (int s = 0; s < NS; s++) B[0][s] = X.b[s];
+    }
   };
 
   d4 acc[4][1];
@@ -546,7 +597,9 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   f4 cp[4][1];
 #pragma unroll
   for (int i = 0; i < 4; i++) cp[i][0] = f4{0.f, 0.f, 0.f, 0.f};
-  auto step = [&](int buf, int kc, const uint4 (&B)[1][NS]) {
+  auto step = [&](int buf, int kc, const RawB& X, bool last) {
+    uint4 B[1][NS];
+    splitB(X, B, kc, last);
     uint4 A[4][NS];
 #pragma unroll
     for (int lt = 0; lt < 4; lt++)
@@ -571,32 +624,33 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   prefetch(1);
   __syncthreads();
   // B operands: ping-pong buffers, each loaded one K chunk ahead of its use
-  // (no register copies); B0 holds the group's first chunk on entry.  Every
+  // (no register copies); X0 holds the group's first chunk on entry.  Every
   // group but the last has four chunks below nkc (nt > 128 (ngrp-1)), and the
-  // last group's first chunk is below nkc too.  The scheduling barriers keep
-  // each chunk's loads, production part and MFMAs together.
-  uint4 B0[1][NS], B1[1][NS];
-  loadB(0, B0);
+  // last group's first chunk is below nkc too; only the last group's chunks
+  // hold colatitudes past ntheta.  The scheduling barriers keep each chunk's
+  // loads, production part and MFMAs together.
+  RawB X0, X1;
+  loadB(0, X0);
   for (int g = 0; g + 1 < ngrp; g++) {
     const int buf = g & 1;
     const int kc0 = 4 * g;
     begin();
     prefetch(g + 2);  // a whole group ahead of its use
-    loadB(kc0 + 1, B1);
+    loadB(kc0 + 1, X1);
     produce4(buf ^ 1, 0);
-    step(buf, kc0, B0);
+    step(buf, kc0, X0, false);
     __builtin_amdgcn_sched_barrier(0);
-    loadB(kc0 + 2, B0);
+    loadB(kc0 + 2, X0);
     produce4(buf ^ 1, 1);
-    step(buf, kc0 + 1, B1);
+    step(buf, kc0 + 1, X1, false);
     __builtin_amdgcn_sched_barrier(0);
-    loadB(kc0 + 3, B1);
+    loadB(kc0 + 3, X1);
     produce4(buf ^ 1, 2);
-    step(buf, kc0 + 2, B0);
+    step(buf, kc0 + 2, X0, false);
     __builtin_amdgcn_sched_barrier(0);
-    loadB(kc0 + 4, B0);
+    loadB(kc0 + 4, X0);
     produce4(buf ^ 1, 3);
-    step(buf, kc0 + 3, B1);
+    step(buf, kc0 + 3, X1, false);
     __builtin_amdgcn_sched_barrier(0);
     __syncthreads();
   }
@@ -606,15 +660,15 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
     const int buf = (ngrp - 1) & 1;
     const int kc0 = 4 * (ngrp - 1);
     const int kcl = p.nkc - 1;
-    loadB(min(kc0 + 1, kcl), B1);
-    step(buf, kc0, B0);
+    loadB(min(kc0 + 1, kcl), X1);
+    step(buf, kc0, X0, true);
     if (kc0 + 1 < p.nkc) {
-      loadB(min(kc0 + 2, kcl), B0);
-      step(buf, kc0 + 1, B1);
+      loadB(min(kc0 + 2, kcl), X0);
+      step(buf, kc0 + 1, X1, true);
       if (kc0 + 2 < p.nkc) {
-        loadB(min(kc0 + 3, kcl), B1);
-        step(buf, kc0 + 2, B0);
-        if (kc0 + 3 < p.nkc) step(buf, kc0 + 3, B1);
+        loadB(min(kc0 + 3, kcl), X1);
+        step(buf, kc0 + 2, X0, true);
+        if (kc0 + 3 < p.nkc) step(buf, kc0 + 3, X1, true);
       }
     }
 #pragma unroll
@@ -687,7 +741,10 @@ hipError_t launch_leg_synth_bf16(const LegParams& p0, int ns, hipStream_t s) {
 template <int NS>
 hipError_t launch_anal_bf16_t(const LegParams& p0, hipStream_t s) {
   LegParams p = p0;
-  p.n_cg = (p.ntile + 7) / 8;
+  p.n_cg = (p.ncol + 15) / 16;  // the layout FA
+  p.nquad = (p.ntheta + 3) / 4;
+  p.nkc = (p.ntheta + 31) / 32;
+  p.ntile = (p.ncol + 15) / 16;
   const dim3 grid(8 * p.items_per);  // work list: (order, pass, column super-group)
   if (p.real)
     hipLaunchKernelGGL((leg_anal_bf16<NS, 1>), grid, dim3(512), 0, s, p);
