@@ -483,8 +483,11 @@ int ensure_synth_worklist(gsh_grid_s* g, int family, int lmax, int real, int ngr
 // Work list of the coefficient split: items (slot, chunk | tile << 16)
 // chunk-major (all orders' chunk 0, then chunk 1, ...), padded to a multiple
 // of 32 (8 XCDs x 4 waves per workgroup); per = items / 32.
+// Work list of coef_split: items (group gs of the 8 slots 8 gs .. 8 gs + 7,
+// chunk | tile << 16), chunks innermost, so that consecutive items -- which
+// run on one XCD -- share the degrees their windows overlap in.
 int ensure_split_worklist(gsh_grid_s* g, int lmax, int n, int real, int ntile, WorkList* out) {
-  auto key = std::make_tuple(30, lmax, n, real, std::vector<int>{ntile});
+  auto key = std::make_tuple(31, lmax, n, real, std::vector<int>{ntile});
   auto it = g->worklists.find(key);
   if (it != g->worklists.end()) {
     *out = it->second;
@@ -492,23 +495,21 @@ int ensure_split_worklist(gsh_grid_s* g, int lmax, int n, int real, int ntile, W
   }
   const int nslot = real ? lmax + 1 : 2 * lmax + 1;
   const int m_first = real ? 0 : -lmax;
-  std::vector<int> nch(nslot);
-  int maxch = 0;
-  for (int sl = 0; sl < nslot; sl++) {
-    const int l0 = std::max(std::abs(sl + m_first), std::abs(n));
-    nch[sl] = (lmax - l0 + 32) / 32;
-    maxch = std::max(maxch, nch[sl]);
-  }
   std::vector<uint2> items;
-  for (int c = 0; c < maxch; c++)
-    for (int sl = 0; sl < nslot; sl++)
-      if (c < nch[sl])
-        for (int t = 0; t < ntile; t++)
-          items.push_back(make_uint2(static_cast<unsigned>(sl),
-                                     static_cast<unsigned>(c) | (static_cast<unsigned>(t) << 16)));
-  while (items.size() % 32 != 0 || items.empty()) items.push_back(make_uint2(~0u, 0u));
+  for (int gs = 0; 8 * gs < nslot; gs++) {
+    int nch = 0;  // chunks of the group's order with the most
+    for (int sl = 8 * gs; sl < std::min(8 * gs + 8, nslot); sl++) {
+      const int l0 = std::max(std::abs(sl + m_first), std::abs(n));
+      nch = std::max(nch, (lmax - l0 + 32) / 32);
+    }
+    for (int t = 0; t < ntile; t++)
+      for (int c = 0; c < nch; c++)
+        items.push_back(make_uint2(static_cast<unsigned>(gs),
+                                   static_cast<unsigned>(c) | (static_cast<unsigned>(t) << 16)));
+  }
+  while (items.size() % 8 != 0 || items.empty()) items.push_back(make_uint2(~0u, 0u));
   WorkList w;
-  w.per = static_cast<int>(items.size() / 32);
+  w.per = static_cast<int>(items.size() / 8);
   GSH_HIP(hipMalloc(&w.d, items.size() * sizeof(uint2)));
   GSH_HIP(hipMemcpy(w.d, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice));
   g->worklists.emplace(key, w);
@@ -892,7 +893,7 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
       p.items_per = ws.per;
     }
     GSH_HIP(timed(g, GSH_KERNEL_COEF_SPLIT, s,
-                  [&] { return gsh::launch_coef_split(p, ns, nblocks, s); }));
+                  [&] { return gsh::launch_coef_split(p, ns, s); }));
     p.items = w.d;  // the synthesis work list again
     p.items_per = w.per;
     GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_SYNTHESIS, s,

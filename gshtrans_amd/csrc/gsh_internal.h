@@ -185,7 +185,7 @@ bool wave_fft_supported(int N);
 hipError_t launch_fft_synth_wave(const FFTParams& p, hipStream_t s);
 hipError_t launch_fft_anal_wave(const FFTParams& p, hipStream_t s);
 // Mixed precision kernels (gsh_mixed.hip); ns = 2 (bf16x3) or 3 (bf16x6).
-hipError_t launch_coef_split(const LegParams& p, int ns, long nblocks, hipStream_t s);
+hipError_t launch_coef_split(const LegParams& p, int ns, hipStream_t s);
 hipError_t launch_leg_synth_bf16(const LegParams& p, int ns, hipStream_t s);
 hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s);
 // FA (FP64 analysis bins) -> split bf16 analysis operands (xs_index layout).

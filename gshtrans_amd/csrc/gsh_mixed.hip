@@ -120,45 +120,74 @@ __device__ __forceinline__ void flush(d4& acc, const f4 c) {
 // Coefficient split: synthesis B operands.  Block (cso[slot] + chunk, tile,
 // s) holds, for lane = 16*kr + c, the bf16 plane s of the coefficients of
 // degrees l0 + 32*chunk + 8*kr + e (e = 0..7) and column 16*tile + c; degrees
-// past lmax and columns past ncol are zero.
+// past the column's truncation degree and columns past ncol are zero.
+//
+// A workgroup of 8 waves writes the blocks (slot 8 gs + w, chunk, tile) of
+// waves w = 0..7.  The degrees they need lie in a window of 40 from lw0, the
+// smallest first degree l0 + 32 chunk of the 8 orders (l0 = max(|m|, |n|)
+// changes by at most one from one order to the next).  The workgroup first
+// reads that window for its 8 fields and 8 orders into LDS -- for each
+// (degree, field) the 8 orders are 128 contiguous bytes of the coefficient
+// array -- and each wave then takes its order's 32 degrees from LDS.
 // ---------------------------------------------------------------------------
+constexpr int kSplitWin = 40;          // degree window of a workgroup
+constexpr int kSplitRow = 8 * 16 + 2;  // doubles per window degree (8 orders x 16 columns + pad)
+
 template <int NS, int REAL>
-__global__ __launch_bounds__(256) void coef_split(LegParams p, long nblocks) {
-  // Item of this wave, (slot, chunk | tile << 16), from a chunk-major work
-  // list: neighbouring orders -- whose coefficients of one degree share
-  // cache lines -- are split together on one XCD (workgroup b runs on XCD
-  // b % 8 and takes items 4 ((b % 8) per + b / 8) + wave).
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int bw = blockIdx.x;
-  const unsigned long long v = ld_uniform(reinterpret_cast<const unsigned long long*>(p.items),
-                                          ((long)(bw & 7) * p.items_per + (bw >> 3)) * 4 + wave);
-  if (static_cast<unsigned>(v) == ~0u) return;
-  (void)nblocks;
-  const int lane = threadIdx.x & 63;
-  const int slot = static_cast<int>(static_cast<unsigned>(v));
-  const int chunk = static_cast<int>((v >> 32) & 0xffff);
-  const int tile = static_cast<int>(v >> 48);
-  const long b = (ld_uniform(p.cso, slot) + chunk) * p.ntile + tile;  // output block
-  const int m = slot + p.m_first;
+__global__ __launch_bounds__(512) void coef_split(LegParams p) {
+  __shared__ double win[kSplitWin * kSplitRow];
+  const uint2 it = work_item(p.items, p.items_per);
+  if (it.x == ~0u) return;
+  const int gs = it.x;
+  const int chunk = it.y & 0xffff;
+  const int tile = it.y >> 16;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+  const int s0 = 8 * gs;
+  int lw0 = 1 << 30;
+#pragma unroll
+  for (int w = 0; w < 8; w++) {
+    const int sl = s0 + w;
+    if (sl < p.nslot) lw0 = min(lw0, max(abs(sl + p.m_first), p.nabs) + 32 * chunk);
+  }
+  // Window: piece (l_off, f, mo) = (tid + 512 r) / 64, (tid / 8) % 8, tid % 8.
+  {
+    const int mo = tid & 7, f = (tid >> 3) & 7;
+    const int sl = s0 + mo;
+    const int m = sl + p.m_first;
+    const int fld = 8 * tile + f;  // field of columns 2 fld, 2 fld + 1
+    const bool ok = sl < p.nslot && fld < p.nfield;
+    const int lf = ok ? field_lmax(p, fld) : -1;
+    const double* cp = p.cin + (ok ? (long)field_index(p.fmap, fld) * p.cstride * 2 : 0);
+    double2 v[kSplitWin / 8];
+#pragma unroll
+    for (int r = 0; r < kSplitWin / 8; r++) {
+      const int l = lw0 + 8 * r + (tid >> 6);
+      v[r] = make_double2(0.0, 0.0);
+      if (l <= lf && l >= abs(m) && l >= p.nabs)
+        v[r] = *reinterpret_cast<const double2*>(cp + 2 * coef_index(l, m, p.nabs, REAL));
+    }
+#pragma unroll
+    for (int r = 0; r < kSplitWin / 8; r++)
+      *reinterpret_cast<double2*>(&win[(8 * r + (tid >> 6)) * kSplitRow + 16 * mo + 2 * f]) = v[r];
+  }
+  __syncthreads();
+  const int sl = s0 + wave;
+  if (sl >= p.nslot) return;
+  const int m = sl + p.m_first;
   const int l0 = max(abs(m), p.nabs);
-  const int kr = lane >> 4;
-  const int col = tile * 16 + (lane & 15);
-  const bool cv = col < p.ncol;
-  const int f = cv ? col >> 1 : 0;
-  const int lf = field_lmax(p, f);  // this column's truncation degree
-  const double* cp = p.cin + (long)field_index(p.fmap, f) * p.cstride * 2 + (col & 1);
+  if (chunk >= (p.lmax - l0 + 32) / 32) return;  // no such block for this order
+  const long b = (ld_uniform(p.cso, sl) + chunk) * p.ntile + tile;  // output block
+  const int kr = lane >> 4, c = lane & 15;
+  const int lo = l0 + 32 * chunk + 8 * kr - lw0;  // window offset of e = 0
   unsigned planes[NS][4];
 #pragma unroll
   for (int e2 = 0; e2 < 4; e2++) {
-    double x[2];
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int l = l0 + 32 * chunk + 8 * kr + 2 * e2 + u;
-      const double v = cp[lf >= l0 ? 2 * coef_index(min(l, lf), m, p.nabs, REAL) : 0];
-      x[u] = (cv && l <= lf) ? v : 0.0;
-    }
+    const double x0 = win[(lo + 2 * e2) * kSplitRow + 16 * wave + c];
+    const double x1 = win[(lo + 2 * e2 + 1) * kSplitRow + 16 * wave + c];
     unsigned o[NS];
-    split_pair<NS>(x[0], x[1], o);
+    split_pair<NS>(x0, x1, o);
 #pragma unroll
     for (int s = 0; s < NS; s++) planes[s][e2] = o[s];
   }
@@ -696,16 +725,16 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
 
 }  // namespace
 
-hipError_t launch_coef_split(const LegParams& p, int ns, long nblocks, hipStream_t s) {
-  const dim3 grid(8 * p.items_per);  // 4 items (waves) per workgroup
+hipError_t launch_coef_split(const LegParams& p, int ns, hipStream_t s) {
+  const dim3 grid(8 * p.items_per);  // one item (8 orders) per workgroup
   if (ns == 3 && p.real)
-    hipLaunchKernelGGL((coef_split<3, 1>), grid, dim3(256), 0, s, p, nblocks);
+    hipLaunchKernelGGL((coef_split<3, 1>), grid, dim3(512), 0, s, p);
   else if (ns == 3)
-    hipLaunchKernelGGL((coef_split<3, 0>), grid, dim3(256), 0, s, p, nblocks);
+    hipLaunchKernelGGL((coef_split<3, 0>), grid, dim3(512), 0, s, p);
   else if (p.real)
-    hipLaunchKernelGGL((coef_split<2, 1>), grid, dim3(256), 0, s, p, nblocks);
+    hipLaunchKernelGGL((coef_split<2, 1>), grid, dim3(512), 0, s, p);
   else
-    hipLaunchKernelGGL((coef_split<2, 0>), grid, dim3(256), 0, s, p, nblocks);
+    hipLaunchKernelGGL((coef_split<2, 0>), grid, dim3(512), 0, s, p);
   return hipGetLastError();
 }
 
