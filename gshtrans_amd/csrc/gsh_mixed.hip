@@ -652,52 +652,45 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   for (int part = 0; part < 4; part++) produce4(0, part);
   prefetch(1);
   __syncthreads();
-  // B operands: ping-pong buffers, each loaded one K chunk ahead of its use
-  // (no register copies); X0 holds the group's first chunk on entry.  Every
-  // group but the last has four chunks below nkc (nt > 128 (ngrp-1)), and the
-  // last group's first chunk is below nkc too; only the last group's chunks
-  // hold colatitudes past ntheta.  The scheduling barriers keep each chunk's
-  // loads, production part and MFMAs together.
-  RawB X0, X1;
-  loadB(0, X0);
+  // B operands: chunk c sits in buffer X[c & 3], loaded kAhead chunks before
+  // its MFMAs (into the buffer chunk c - 4 + kAhead used).  Loads clamp to
+  // the last chunk nkc - 1; what they bring past it is never multiplied.
+  // Every group but the last has four chunks below nkc (nt > 128 (ngrp-1)),
+  // and the last group's first chunk is below nkc too; only the last group's
+  // chunks hold colatitudes past ntheta.  The scheduling barriers keep each
+  // chunk's loads, production part and MFMAs together.
+  constexpr int kAhead = 3;
+  const int kcl = p.nkc - 1;
+  RawB X[4];
+#pragma unroll
+  for (int j = 0; j < kAhead; j++) loadB(min(j, kcl), X[j]);
   for (int g = 0; g + 1 < ngrp; g++) {
     const int buf = g & 1;
     const int kc0 = 4 * g;
     begin();
     prefetch(g + 2);  // a whole group ahead of its use
-    loadB(kc0 + 1, X1);
-    produce4(buf ^ 1, 0);
-    step(buf, kc0, X0, false);
-    __builtin_amdgcn_sched_barrier(0);
-    loadB(kc0 + 2, X0);
-    produce4(buf ^ 1, 1);
-    step(buf, kc0 + 1, X1, false);
-    __builtin_amdgcn_sched_barrier(0);
-    loadB(kc0 + 3, X1);
-    produce4(buf ^ 1, 2);
-    step(buf, kc0 + 2, X0, false);
-    __builtin_amdgcn_sched_barrier(0);
-    loadB(kc0 + 4, X0);
-    produce4(buf ^ 1, 3);
-    step(buf, kc0 + 3, X1, false);
-    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      loadB(min(kc0 + j + kAhead, kcl), X[(j + kAhead) & 3]);
+      produce4(buf ^ 1, j);
+      step(buf, kc0 + j, X[j], false);
+      __builtin_amdgcn_sched_barrier(0);
+    }
     __syncthreads();
   }
   {
-    // Last group: chunks 4 (ngrp-1) .. nkc-1 (loads clamp to nkc-1; what
-    // they bring past the last chunk is never multiplied).
+    // Last group: chunks 4 (ngrp-1) .. nkc-1.
     const int buf = (ngrp - 1) & 1;
     const int kc0 = 4 * (ngrp - 1);
-    const int kcl = p.nkc - 1;
-    loadB(min(kc0 + 1, kcl), X1);
-    step(buf, kc0, X0, true);
+    step(buf, kc0, X[0], true);
     if (kc0 + 1 < p.nkc) {
-      loadB(min(kc0 + 2, kcl), X0);
-      step(buf, kc0 + 1, X1, true);
+      step(buf, kc0 + 1, X[1], true);
       if (kc0 + 2 < p.nkc) {
-        loadB(min(kc0 + 3, kcl), X1);
-        step(buf, kc0 + 2, X0, true);
-        if (kc0 + 3 < p.nkc) step(buf, kc0 + 3, X1, true);
+        step(buf, kc0 + 2, X[2], true);
+        if (kc0 + 3 < p.nkc) {
+          loadB(kc0 + 3, X[3]);
+          step(buf, kc0 + 3, X[3], true);
+        }
       }
     }
 #pragma unroll
