@@ -172,7 +172,6 @@ struct gsh_grid_s {
   double* d_cos = nullptr;
   double* d_wq = nullptr;
   double* d_one = nullptr;     // unit colatitude weights (adjoint of the synthesis)
-  uint4* d_bzero = nullptr;    // zero B operand (3 planes x 64 lanes) of masked MFMA tiles
   double2* d_tw = nullptr;
   double2* d_twa = nullptr;    // wavefront FFT twiddles (N in {512, 1024, 2048})
   double2* d_chirp = nullptr;  // Bluestein tables (ring length not a power of two)
@@ -547,7 +546,6 @@ LegParams base_params(gsh_grid_s* g, const NData* d, int lmax, int n, int real, 
   p.rec = d->rec;
   p.roff = d->roff;
   p.F = g->work;
-  p.bzero = g->d_bzero;
   return p;
 }
 
@@ -748,8 +746,6 @@ int gsh_grid_create(int lmax, int nmax, int mrange, int nrange, gsh_grid_t* grid
   GSH_HIP(hipMalloc(&g->d_tw, tw.size() * sizeof(double2)));
   GSH_HIP(hipMemcpy(g->d_cos, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice));
   GSH_HIP(hipMemcpy(g->d_wq, wq.data(), wq.size() * sizeof(double), hipMemcpyHostToDevice));
-  GSH_HIP(hipMalloc(&g->d_bzero, 3 * 64 * sizeof(uint4)));
-  GSH_HIP(hipMemset(g->d_bzero, 0, 3 * 64 * sizeof(uint4)));
   {
     const std::vector<double> one(lmax + 1, 1.0);
     GSH_HIP(hipMalloc(&g->d_one, one.size() * sizeof(double)));
@@ -775,7 +771,6 @@ int gsh_grid_destroy(gsh_grid_t g) {
   (void)hipFree(g->d_cos);
   (void)hipFree(g->d_wq);
   (void)hipFree(g->d_one);
-  (void)hipFree(g->d_bzero);
   (void)hipFree(g->d_tw);
   (void)hipFree(g->d_twa);
   if (g->d_chirp) (void)hipFree(g->d_chirp);
@@ -938,11 +933,15 @@ int run_forward(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
   p.fmap = o.fmap;
   p.f_lmax = o.f_lmax;
   p.cg_lmax = o.cg_lmax;
-  // FA, plus 8 quads of 64 doubles: the split-BF16 analysis reads whole
-  // 32-colatitude chunks, up to 8 quads past the last block.
+  // FA (FP32 in the split-BF16 modes), plus 8 quads of 64 entries: the
+  // split-BF16 analysis reads whole 32-colatitude chunks, up to 8 quads past
+  // the last block.
+  const bool mixed = precision == GSH_BF16X6 || precision == GSH_BF16X3;
   const size_t fa = (size_t)nslot * ((ncol + 15) / 16) * ((g->ntheta() + 3) / 4) * 64 + 8 * 64;
-  if ((st = ensure_work(g, sizeof(double) * fa))) return st;
+  if ((st = ensure_work(g, (mixed ? sizeof(float) : sizeof(double)) * fa))) return st;
   q.F = p.F = g->work;
+  q.fa32 = mixed ? 1 : 0;
+  p.FA32 = mixed ? reinterpret_cast<const float*>(g->work) : nullptr;
   if ((st = ensure_ck(g, d, n, s))) return st;
   p.ck = d->ck;
   p.ckoff = d->ckoff;
@@ -975,19 +974,6 @@ int run_forward(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
     p.items_per = w.per;
   }
   const int ns = precision == GSH_BF16X6 ? 3 : 2;
-  if (ns == 2) {
-    // bf16x3 reads split operand blocks (bf16x6 splits FA in registers).
-    const int nkc = (g->ntheta() + 31) / 32;
-    const int ntile = (ncol + 15) / 16;
-    const long nblocks = (long)nslot * nkc * ntile;
-    if ((st = ensure_work2(g, (size_t)nblocks * ns * 64 * 16))) return st;
-    p.bs = static_cast<const uint4*>(g->work2);
-    p.bs_out = static_cast<uint4*>(g->work2);
-    p.nkc = nkc;
-    p.ntile = ntile;
-    GSH_HIP(timed(g, GSH_KERNEL_BIN_SPLIT, s,
-                  [&] { return gsh::launch_bin_split(p, ns, nblocks, s); }));
-  }
   GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_ANALYSIS, s,
                 [&] { return gsh::launch_leg_anal_bf16(p, ns, s); }));
   return GSH_OK;

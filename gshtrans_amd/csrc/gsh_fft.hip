@@ -258,15 +258,26 @@ __global__ __launch_bounds__(kFFTThreads) void fft_anal(FFTParams p) {
       if (k == 0 || 2 * k == N) X.y = 0.0;
       else X = make_double2(2.0 * X.x, 2.0 * X.y);
     }
-    double* Fp = p.F + fa_index(s, 2 * f, theta, p.n_cg, p.nquad);
-    Fp[0] = X.x * w;
-    Fp[1] = X.y * w;
+    const long fi = fa_index(s, 2 * f, theta, p.n_cg, p.nquad);
+    if (p.fa32) {
+      float* Fp = reinterpret_cast<float*>(p.F) + fi;
+      Fp[0] = (float)(X.x * w);
+      Fp[1] = (float)(X.y * w);
+    } else {
+      p.F[fi] = X.x * w;
+      p.F[fi + 1] = X.y * w;
+    }
     // The last ring also zeroes the padding rows of the last colatitude
     // quad, which the Legendre contraction reads as B operands.
     for (int tp = theta + 1; theta == p.ntheta - 1 && tp < 4 * p.nquad; tp++) {
-      double* Zp = p.F + fa_index(s, 2 * f, tp, p.n_cg, p.nquad);
-      Zp[0] = 0.0;
-      Zp[1] = 0.0;
+      const long zi = fa_index(s, 2 * f, tp, p.n_cg, p.nquad);
+      if (p.fa32) {
+        reinterpret_cast<float*>(p.F)[zi] = 0.0f;
+        reinterpret_cast<float*>(p.F)[zi + 1] = 0.0f;
+      } else {
+        p.F[zi] = 0.0;
+        p.F[zi + 1] = 0.0;
+      }
     }
   }
 }

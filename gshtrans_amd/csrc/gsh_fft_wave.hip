@@ -410,11 +410,21 @@ __device__ __forceinline__ void separate(const double2 (&v)[R], int t, int lane,
   xb = make_double2((z.y + zp.y) * hw, (zp.x - z.x) * hw * im);
 }
 
+// One weighted bin into FA: FP64, or FP32 (the same indices in a float
+// array) for the split-BF16 modes.
+template <int FA32>
+__device__ __forceinline__ void st_bin(double* F, long i, double2 X) {
+  if (FA32)
+    *reinterpret_cast<float2*>(reinterpret_cast<float*>(F) + i) = make_float2((float)X.x, (float)X.y);
+  else
+    st2(F + i, X);
+}
+
 // Analysis: field ring -> forward DFT -> weighted bins into the analysis
 // layout FA (fa_index), exchanged through LDS as in the synthesis; the last
 // ring row also zeroes the padding colatitudes of the last quad.  A wave
 // without a field transforms zeros and stores nothing.
-template <int R, int REAL, int F32>
+template <int R, int REAL, int F32, int FA32>
 __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) {
   constexpr int W = waves_for(R);
   constexpr int LPW = lds_per_wave(R);
@@ -475,9 +485,9 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
         const int b = 64 * i + bq;
         const int k = r * LPW + b;
         const double2 X = lds_all[c * LPW + xch(c, b)];
-        if (cv && k <= lmax) st2(p.F + fa_index(k - m_first, 2 * cf, theta, n_cg, nquad), X);
+        if (cv && k <= lmax) st_bin<FA32>(p.F, fa_index(k - m_first, 2 * cf, theta, n_cg, nquad), X);
         if (cv && k >= N - lmax)
-          st2(p.F + fa_index(k - N - m_first, 2 * cf, theta, n_cg, nquad), X);
+          st_bin<FA32>(p.F, fa_index(k - N - m_first, 2 * cf, theta, n_cg, nquad), X);
       }
     }
   } else {
@@ -497,10 +507,10 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
         const int b = 64 * i + bq;
         const int k = r * BH + b;
         if (cv && k <= lmax) {
-          st2(p.F + fa_index(k, 2 * cf, theta, n_cg, nquad), lds_all[c * LPW + xch(c, b)]);
+          st_bin<FA32>(p.F, fa_index(k, 2 * cf, theta, n_cg, nquad), lds_all[c * LPW + xch(c, b)]);
           if (cv1)
-            st2(p.F + fa_index(k, 2 * cf + 2, theta, n_cg, nquad),
-                lds_all[c * LPW + BH + xch(c, b)]);
+            st_bin<FA32>(p.F, fa_index(k, 2 * cf + 2, theta, n_cg, nquad),
+                         lds_all[c * LPW + BH + xch(c, b)]);
         }
       }
     }
@@ -509,8 +519,8 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
       double2 xa, xb;
       separate<R>(v, R / 2, lane, w, p.adjoint, xa, xb);
       if (lane == 0) {
-        st2(p.F + fa_index(N / 2, 2 * f0, theta, n_cg, nquad), xa);
-        if (f1v) st2(p.F + fa_index(N / 2, 2 * f0 + 2, theta, n_cg, nquad), xb);
+        st_bin<FA32>(p.F, fa_index(N / 2, 2 * f0, theta, n_cg, nquad), xa);
+        if (f1v) st_bin<FA32>(p.F, fa_index(N / 2, 2 * f0 + 2, theta, n_cg, nquad), xb);
       }
     }
   }
@@ -518,7 +528,13 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
     const int ncw = REAL && f1v ? 4 : 2;
     for (int tp = theta + 1; tp < 4 * nquad; tp++)
       for (int s = lane; s < p.nslot; s += 64)
-        for (int cc = 0; cc < ncw; cc++) p.F[fa_index(s, 2 * f0 + cc, tp, n_cg, nquad)] = 0.0;
+        for (int cc = 0; cc < ncw; cc++) {
+          const long zi = fa_index(s, 2 * f0 + cc, tp, n_cg, nquad);
+          if (FA32)
+            reinterpret_cast<float*>(p.F)[zi] = 0.0f;
+          else
+            p.F[zi] = 0.0;
+        }
   }
 }
 
@@ -539,14 +555,23 @@ hipError_t launch_synth_r(const FFTParams& p, hipStream_t s) {
 template <int R>
 hipError_t launch_anal_r(const FFTParams& p, hipStream_t s) {
   const dim3 grid(p.nfg, p.ntheta), block(64 * waves_for(R));
-  if (p.real && p.f32)
-    hipLaunchKernelGGL((fft_anal_w<R, 1, 1>), grid, block, 0, s, p);
+  if (p.fa32) {
+    if (p.real && p.f32)
+      hipLaunchKernelGGL((fft_anal_w<R, 1, 1, 1>), grid, block, 0, s, p);
+    else if (p.real)
+      hipLaunchKernelGGL((fft_anal_w<R, 1, 0, 1>), grid, block, 0, s, p);
+    else if (p.f32)
+      hipLaunchKernelGGL((fft_anal_w<R, 0, 1, 1>), grid, block, 0, s, p);
+    else
+      hipLaunchKernelGGL((fft_anal_w<R, 0, 0, 1>), grid, block, 0, s, p);
+  } else if (p.real && p.f32)
+    hipLaunchKernelGGL((fft_anal_w<R, 1, 1, 0>), grid, block, 0, s, p);
   else if (p.real)
-    hipLaunchKernelGGL((fft_anal_w<R, 1, 0>), grid, block, 0, s, p);
+    hipLaunchKernelGGL((fft_anal_w<R, 1, 0, 0>), grid, block, 0, s, p);
   else if (p.f32)
-    hipLaunchKernelGGL((fft_anal_w<R, 0, 1>), grid, block, 0, s, p);
+    hipLaunchKernelGGL((fft_anal_w<R, 0, 1, 0>), grid, block, 0, s, p);
   else
-    hipLaunchKernelGGL((fft_anal_w<R, 0, 0>), grid, block, 0, s, p);
+    hipLaunchKernelGGL((fft_anal_w<R, 0, 0, 0>), grid, block, 0, s, p);
   return hipGetLastError();
 }
 

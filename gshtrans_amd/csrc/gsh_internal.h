@@ -35,7 +35,9 @@ static_assert(kPass % kCk == 0, "passes start at checkpoints");
 //              X'_m(theta) in the lane order of the 4x16 MFMA B operand,
 //              quad = theta/4, cg = col/16, lane = 16*(theta%4) + col%16,
 //              so one colatitude quad of one column group is 512 contiguous
-//              bytes; see fa_index().
+//              bytes; see fa_index().  The split-BF16 modes keep FA in FP32
+//              (the same indices in a float array): their split rounds every
+//              operand to FP32 first.
 struct LegParams {
   int L;          // grid degree; ntheta = L + 1
   int ntheta;
@@ -69,11 +71,10 @@ struct LegParams {
   // Mixed precision (split-BF16 MFMA):
   //   synthesis: bs holds the split coefficients, block (cso[slot] + chunk,
   //   tile, split) = 64 lanes x 8 bf16, see coef_split in gsh_mixed.hip;
-  //   analysis: bs holds the split X' in the same lane order with 32-colatitude
-  //   chunks, block (slot, kchunk, tile, split).
+  //   analysis: FA32 holds the bins FA in FP32.
   const uint4* bs;
   uint4* bs_out;
-  const uint4* bzero;   // 3 x 64 zero uint4: the B operand of masked tiles
+  const float* FA32;
   const long* cso;      // per slot: first chunk of that order in bs
   int ntile;            // 16-column tiles, ceil(ncol / 16)
   int nkc;              // analysis: 32-colatitude chunks, ceil(ntheta / 32)
@@ -152,22 +153,14 @@ struct FFTParams {
   const double* wq;   // analysis weights w_t * 2 pi / N (all 1 for the adjoint)
   int adjoint;        // adjoint of the synthesis: real fields take the c2r
                       // bin weights (2 for 0 < k < N/2, Re only at 0, N/2)
-  double* F;
+  double* F;          // bins: FA (analysis; float elements when fa32) or F
+  int fa32;           // analysis: FA in FP32 (split-BF16 modes)
   const void* fin;    // analysis input field
   void* fout;         // synthesis output field
   long fstride;       // elements between fields
   int f32;            // fields stored in FP32 (float / complex float)
   const int* fmap;    // column field f -> index of the field in the caller's arrays
 };
-
-// bf16 index of (slot, col, theta, plane s) in the split analysis layout:
-// block ((slot * nkc + theta/32) * ntile + col/16) * ns + s of 64 lanes x 8,
-// lane = 16 * ((theta % 32) / 8) + col % 16, element theta % 8.
-__host__ __device__ inline long xs_index(int slot, int col, int theta, int s, int ns, int nkc,
-                                         int ntile) {
-  const long blk = (((long)slot * nkc + (theta >> 5)) * ntile + (col >> 4)) * ns + s;
-  return (blk * 64 + 16 * ((theta & 31) >> 3) + (col & 15)) * 8 + (theta & 7);
-}
 
 // tiles: 16-column tiles per workgroup (1 or 2; 0 picks 2 when ncol > 16);
 // f32m: FP32 MFMA on FP32-rounded operands with FP32 accumulation (the
@@ -188,8 +181,6 @@ hipError_t launch_fft_anal_wave(const FFTParams& p, hipStream_t s);
 hipError_t launch_coef_split(const LegParams& p, int ns, hipStream_t s);
 hipError_t launch_leg_synth_bf16(const LegParams& p, int ns, hipStream_t s);
 hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s);
-// FA (FP64 analysis bins) -> split bf16 analysis operands (xs_index layout).
-hipError_t launch_bin_split(const LegParams& p, int ns, long nblocks, hipStream_t s);
 
 // FP32 <-> FP64 coefficient staging of batch plans (gsh_batch.hip); strides
 // and sizes in complex elements.

@@ -66,10 +66,7 @@ __device__ __forceinline__ unsigned pack2(__bf16 a, __bf16 b) {
 // Split two doubles (consecutive K entries) into NS bf16 planes; plane s of
 // the pair is returned packed in one dword (element 0 in the low half).
 template <int NS>
-__device__ __forceinline__ void split_pair(double x0, double x1, unsigned* out) {
-  float f0 = (float)x0, f1 = (float)x1;
-  // Keep the f64->f32 rounding separate from the bf16 rounding below.
-  asm volatile("" : "+v"(f0), "+v"(f1));
+__device__ __forceinline__ void split_pair_f(float f0, float f1, unsigned* out) {
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     const __bf16 b0 = (__bf16)f0, b1 = (__bf16)f1;
@@ -79,6 +76,13 @@ __device__ __forceinline__ void split_pair(double x0, double x1, unsigned* out) 
       f1 -= (float)b1;
     }
   }
+}
+template <int NS>
+__device__ __forceinline__ void split_pair(double x0, double x1, unsigned* out) {
+  float f0 = (float)x0, f1 = (float)x1;
+  // Keep the f64->f32 rounding separate from the bf16 rounding below.
+  asm volatile("" : "+v"(f0), "+v"(f1));
+  split_pair_f<NS>(f0, f1, out);
 }
 
 __device__ __forceinline__ bf16x8 as_bf8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
@@ -188,48 +192,6 @@ __global__ __launch_bounds__(512) void coef_split(LegParams p) {
     const double x1 = win[(lo + 2 * e2 + 1) * kSplitRow + 16 * wave + c];
     unsigned o[NS];
     split_pair<NS>(x0, x1, o);
-#pragma unroll
-    for (int s = 0; s < NS; s++) planes[s][e2] = o[s];
-  }
-#pragma unroll
-  for (int s = 0; s < NS; s++)
-    p.bs_out[(b * NS + s) * 64 + lane] =
-        make_uint4(planes[s][0], planes[s][1], planes[s][2], planes[s][3]);
-}
-
-// ---------------------------------------------------------------------------
-// Bin split: analysis B operands.  Block ((slot * nkc + kchunk) * ntile +
-// tile, s) holds, for lane = 16*q + c, the bf16 plane s of the weighted bins
-// X'(theta, col) for theta = 32*kchunk + 8*q + e (e = 0..7) and col =
-// 16*tile + c, read from the FP64 layout FA; colatitudes past ntheta and
-// columns past ncol are zero.  For a fixed e the 64 lanes read four whole
-// 128-byte rows of FA.
-// ---------------------------------------------------------------------------
-template <int NS>
-__global__ __launch_bounds__(256) void bin_split(LegParams p, long nblocks) {
-  const long b = (long)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (b >= nblocks) return;
-  const int lane = threadIdx.x & 63;
-  const int tile = (int)(b % p.ntile);
-  const long sk = b / p.ntile;
-  const int kc = (int)(sk % p.nkc);
-  const int slot = (int)(sk / p.nkc);
-  const int col = tile * 16 + (lane & 15);
-  const int t0 = 32 * kc + 8 * (lane >> 4);
-  const bool cv = col < p.ncol;
-  unsigned planes[NS][4];
-#pragma unroll
-  for (int e2 = 0; e2 < 4; e2++) {
-    double x[2];
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int t = t0 + 2 * e2 + u;
-      const bool ok = cv && t < p.ntheta;
-      const double v = p.F[fa_index(slot, col, ok ? t : 0, p.n_cg, p.nquad)];
-      x[u] = ok ? v : 0.0;
-    }
-    unsigned o[NS];
-    split_pair<NS>(x[0], x[1], o);
 #pragma unroll
     for (int s = 0; s < NS; s++) planes[s][e2] = o[s];
   }
@@ -553,69 +515,44 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   // Consumer role.
   // Wave w multiplies all 4 degree tiles with column tile w, so every B
   // operand block is loaded by one wave of the workgroup (the A operands
-  // come from LDS).  Lane (q4, c) = (lane / 16, lane % 16) of K chunk kc
-  // takes colatitudes 32 kc + 8 q4 + e (e = 0..7) of column 16 tile + c.
-  // Callers pass chunks below nkc only.
-  //  - bf16x6 (NS = 3): straight from the FP64 bins FA, i.e. FA entries 16 e
-  //    past the lane's base in quad 8 kc + 2 q4 (each e a whole 128-byte row
-  //    per 16 lanes), split into bf16 planes just before the chunk's MFMAs;
-  //    the six products per operand hide the longer loads.  A tile past the
-  //    last column group reads the last one: its outputs are never stored,
-  //    and a column only meets its own B values, so nothing else needs those
-  //    columns zero.  Reads of the last block run up to 8 quads past it (the
-  //    FA buffer is padded for that); colatitudes from ntheta on are zeroed
-  //    at the split of the last group's chunks.
-  //  - bf16x3 (NS = 2): from the split blocks of bin_split (p.bs); a tile
-  //    past ntile reads the zero block p.bzero at every chunk (a base address
-  //    and a zero chunk stride, chosen once).
-  constexpr bool kFA = NS == 3;
+  // come from LDS).  B comes straight from the FP32 bins FA32: lane (q4, c) =
+  // (lane / 16, lane % 16) of K chunk kc takes colatitudes 32 kc + 8 q4 + e
+  // (e = 0..7) of column 16 tile + c, i.e. FA32 entries 16 e past its base
+  // in quad 8 kc + 2 q4 (each e 64 contiguous bytes per 16 lanes), split
+  // into bf16 planes just before the chunk's MFMAs.  A tile past the last
+  // column group reads the last one: its outputs are never stored, and a
+  // column only meets its own B values, so nothing else needs those columns
+  // zero.  Reads of the last block run up to 8 quads past it (the FA buffer
+  // is padded for that); colatitudes from ntheta on are zeroed at the split
+  // of the last group's chunks.  Callers pass chunks below nkc only.
   struct RawB {
-    double x[kFA ? 8 : 1];
-    uint4 b[kFA ? 1 : NS];
+    float x[8];
   };
   const int tile = csg * 8 + wave;
-  const double* fb =
-      p.F + (((long)slot * p.n_cg + min(tile, p.n_cg - 1)) * p.nquad + 2 * (lane >> 4)) * 64 +
+  const float* fb =
+      p.FA32 + (((long)slot * p.n_cg + min(tile, p.n_cg - 1)) * p.nquad + 2 * (lane >> 4)) * 64 +
       (lane & 15);
-  const bool tvalid = tile < p.ntile;
-  const uint4* bt = kFA ? nullptr
-                        : tvalid ? p.bs + (long)slot * p.nkc * p.ntile * NS * 64 + lane +
-                                       (long)tile * NS * 64
-                                 : p.bzero + lane;
-  const long bstr = tvalid ? (long)p.ntile * NS * 64 : 0;
   auto loadB = [&](int kc, RawB& X) {
-    if constexpr (kFA) {
 #pragma unroll
-      for (int e = 0; e < 8; e++) X.x[e] = fb[(long)kc * 512 + 16 * e];
-    } else {
-#pragma unroll
-      for (int s = 0; s < NS; s++) X.b[s] = bt[kc * bstr + s * 64];
-    }
+    for (int e = 0; e < 8; e++) X.x[e] = fb[(long)kc * 512 + 16 * e];
   };
   auto splitB = [&](const RawB& X, uint4 (&B)[1][NS], int kc, bool last) {
-    if constexpr (kFA) {
-      unsigned pl[NS][4];
-      const int th = 32 * kc + 8 * (lane >> 4);  // colatitude of e = 0
+    unsigned pl[NS][4];
+    const int th = 32 * kc + 8 * (lane >> 4);  // colatitude of e = 0
 #pragma unroll
-      for (int e2 = 0; e2 < 4; e2++) {
-        double x0 = X.x[2 * e2], x1 = X.x[2 * e2 + 1];
-        if (last) {
-          x0 = th + 2 * e2 < nt ? x0 : 0.0;
-          x1 = th + 2 * e2 + 1 < nt ? x1 : 0.0;
-        }
-        unsigned o[NS];
-        split_pair<NS>(x0, x1, o);
-#pragma unroll
-        for (int s = 0; s < NS; s++) pl[s][e2] = o[s];
+    for (int e2 = 0; e2 < 4; e2++) {
+      float x0 = X.x[2 * e2], x1 = X.x[2 * e2 + 1];
+      if (last) {
+        x0 = th + 2 * e2 < nt ? x0 : 0.0f;
+        x1 = th + 2 * e2 + 1 < nt ? x1 : 0.0f;
       }
+      unsigned o[NS];
+      split_pair_f<NS>(x0, x1, o);
 #pragma unroll
-      for (int s = 0; s < NS; s++) B[0][s] = make_uint4(pl[s][0], pl[s][1], pl[s][2], pl[s][3]);
-    } else {
-      (void)kc;
-      (void)last;
-#pragma unroll
-      for (int s = 0; s < NS; s++) B[0][s] = X.b[s];
+      for (int s = 0; s < NS; s++) pl[s][e2] = o[s];
     }
+#pragma unroll
+    for (int s = 0; s < NS; s++) B[0][s] = make_uint4(pl[s][0], pl[s][1], pl[s][2], pl[s][3]);
   };
 
   d4 acc[4][1];
@@ -731,18 +668,6 @@ hipError_t launch_coef_split(const LegParams& p, int ns, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_bin_split(const LegParams& p0, int ns, long nblocks, hipStream_t s) {
-  LegParams p = p0;
-  p.n_cg = (p.ncol + 15) / 16;
-  p.nquad = (p.ntheta + 3) / 4;
-  const dim3 grid((nblocks + 3) / 4);
-  if (ns == 3)
-    hipLaunchKernelGGL((bin_split<3>), grid, dim3(256), 0, s, p, nblocks);
-  else
-    hipLaunchKernelGGL((bin_split<2>), grid, dim3(256), 0, s, p, nblocks);
-  return hipGetLastError();
-}
-
 hipError_t launch_leg_synth_bf16(const LegParams& p0, int ns, hipStream_t s) {
   if (!p0.ck || !p0.ckoff) return hipErrorInvalidValue;  // checkpoint table required
   LegParams p = p0;
@@ -766,7 +691,6 @@ hipError_t launch_anal_bf16_t(const LegParams& p0, hipStream_t s) {
   p.n_cg = (p.ncol + 15) / 16;  // the layout FA
   p.nquad = (p.ntheta + 3) / 4;
   p.nkc = (p.ntheta + 31) / 32;
-  p.ntile = (p.ncol + 15) / 16;
   const dim3 grid(8 * p.items_per);  // work list: (order, pass, column super-group)
   if (p.real)
     hipLaunchKernelGGL((leg_anal_bf16<NS, 1>), grid, dim3(512), 0, s, p);
@@ -777,6 +701,7 @@ hipError_t launch_anal_bf16_t(const LegParams& p0, hipStream_t s) {
 
 hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s) {
   if (!p.ck || !p.ckoff) return hipErrorInvalidValue;  // checkpoint table required
+  if (!p.FA32) return hipErrorInvalidValue;             // FP32 bins required
   return ns == 3 ? launch_anal_bf16_t<3>(p, s) : launch_anal_bf16_t<2>(p, s);
 }
 

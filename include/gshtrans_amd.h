@@ -161,7 +161,8 @@ enum gsh_kernel {
   GSH_KERNEL_FFT_ANALYSIS = 2,
   GSH_KERNEL_LEGENDRE_ANALYSIS = 3,
   GSH_KERNEL_COEF_SPLIT = 4,      /* mixed precision: coefficient split   */
-  GSH_KERNEL_BIN_SPLIT = 5,       /* mixed precision: analysis bin split  */
+  GSH_KERNEL_BIN_SPLIT = 5,       /* not launched (always 0): the mixed
+                                     analysis reads FP32 bins directly     */
   GSH_KERNEL_COUNT = 6
 };
 /* enable != 0 records a HIP event pair around every kernel launch of this
