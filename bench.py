@@ -24,6 +24,7 @@ serial transform with its precomputed Wigner table) on a bounded sample.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -43,14 +44,19 @@ HBM_GBPS = 8000.0
 # Per-dispatch HBM bytes of each kernel, from FETCH_SIZE / WRITE_SIZE counter
 # runs scaled by a plain copy's reading (rocprofv3 summary, text).
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "transform_kernels_lmax1024_b64_counters.md")
-# Kernel instantiations of each work family per mode (real and complex batch).
+# Kernel instantiations of each work family per mode (real and complex
+# batch), as regular expressions over the summary's kernel names.
 FAMILY_KERNELS = {
-    "legendre_analysis": {"fp64": "leg_anal_f64<4, false>", "fp32": "leg_anal_f64<4, true>",
-                          "bf16x3": "leg_anal_bf16<2,", "bf16x6": "leg_anal_bf16<3,"},
-    "legendre_synthesis": {"fp64": "leg_synth_f64<2, ", "fp32": "leg_synth_f64<2, ",
-                           "bf16x3": "leg_synth_bf16<2,", "bf16x6": "leg_synth_bf16<3,"},
-    "fft_analysis": "fft_anal_w<32, ",
-    "fft_synthesis": "fft_synth_w<32, ",
+    "legendre_analysis": {"fp64": r"leg_anal_f64<4, false>", "fp32": r"leg_anal_f64<4, true>",
+                          "bf16x3": r"leg_anal_bf16<2, \d>", "bf16x6": r"leg_anal_bf16<3, \d>"},
+    "legendre_synthesis": {"fp64": r"leg_synth_f64<2, \d, false>",
+                           "fp32": r"leg_synth_f64<2, \d, true>",
+                           "bf16x3": r"leg_synth_bf16<2, \d>", "bf16x6": r"leg_synth_bf16<3, \d>"},
+    # FA in FP64 (fp64, fp32) or FP32 (split-BF16 modes): last template argument
+    "fft_analysis": {"fp64": r"fft_anal_w<32, \d, 0, 0>", "fp32": r"fft_anal_w<32, \d, 0, 0>",
+                     "bf16x3": r"fft_anal_w<32, \d, 0, 1>", "bf16x6": r"fft_anal_w<32, \d, 0, 1>"},
+    "fft_synthesis": r"fft_synth_w<32, \d, 0>",
+    "coef_split": {"bf16x3": r"coef_split<2, \d>", "bf16x6": r"coef_split<3, \d>"},
 }
 HEADLINE = "bf16x6"            # <= 1e-7 relative L2 against FP64
 REPORTED = ["bf16x3", "fp32"]  # reported beside it (throughput and errors)
@@ -130,14 +136,16 @@ def run_gpu(args):
                       "kernels": kt}
         log(f"[bench] {m}: {tps:.1f} transforms/s, {1e3 * dt / args.steps:.2f} ms/step")
 
-    # The real and complex batches are independent jobs: the same step with
-    # each batch on its own HIP stream (default priority) and grid handle
-    # (its own work buffers), so one batch's HBM-bound ring FFTs overlap the
-    # other's Legendre kernels.  Reported beside the sequential figures,
-    # which carry the per-kernel timings.
-    gc = G.GaussLegendreGrid(L, NMAX)
-    gc.prepare(2)
-    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    # --two-streams: the real and complex batches are independent jobs; the
+    # same step with each batch on its own HIP stream (default priority) and
+    # grid handle (its own work buffers), so one batch's HBM-bound ring FFTs
+    # overlap the other's Legendre kernels.  Reported beside the sequential
+    # figures, which carry the per-kernel timings (off by default so that a
+    # profile of the default run holds one-stream kernel durations only).
+    if args.two_streams:
+        gc = G.GaussLegendreGrid(L, NMAX)
+        gc.prepare(2)
+        s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
 
     def step2(prec, o):
         cur = torch.cuda.current_stream()
@@ -151,7 +159,7 @@ def run_gpu(args):
         cur.wait_stream(s2)
 
     two = {}
-    for m in dict.fromkeys(["fp64", headline]):
+    for m in (dict.fromkeys(["fp64", headline]) if args.two_streams else []):
         o = outs[m]
         for _ in range(args.warmup):
             step2(m, o)
@@ -164,7 +172,7 @@ def run_gpu(args):
         two[m] = {"transforms_per_s": args.steps * 4 * BATCH / dt,
                   "ms_per_step": 1e3 * dt / args.steps}
         log(f"[bench] two streams, {m}: {two[m]['transforms_per_s']:.1f} transforms/s")
-    results["two_streams"] = two
+    results["two_streams"] = two or None
 
     # Accuracy of each mode against FP64 per field (relative L2).
     def per_field(a, b):
@@ -242,9 +250,11 @@ def traffic_per_launch(family, mode):
     t = load_table(TRAFFIC_FILE)
     if not t:
         return None
-    pre = FAMILY_KERNELS[family]
-    pre = pre[mode] if isinstance(pre, dict) else pre
-    rows = [v for k, v in t["kernels"].items() if k.startswith(pre)]
+    pat = FAMILY_KERNELS.get(family)
+    pat = pat.get(mode) if isinstance(pat, dict) else pat
+    if pat is None:
+        return None
+    rows = [v for k, v in t["kernels"].items() if re.fullmatch(pat, k)]
     if not rows:
         return None
     return sum(r["fetch_bytes"] + r["write_bytes"] for r in rows) / len(rows)
@@ -302,6 +312,9 @@ def main():
                     help="comma-separated reduced-precision modes reported beside the "
                          "headline (default bf16x3,fp32)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--two-streams", action="store_true",
+                    help="also time FP64 and the headline mode with the real and complex "
+                         "batches on two HIP streams")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (the "
                          "known-size reference for HBM byte counters) and exit")
@@ -345,7 +358,8 @@ def main():
                   for m, r in results.items() if m != "two_streams"},
         "two_streams": dict(results["two_streams"], note=(
             "the real and complex batches of each step on two HIP streams, each with its "
-            "own grid handle; value and the per-kernel figures are the one-stream run")),
+            "own grid handle; value and the per-kernel figures are the one-stream run"))
+        if results["two_streams"] else None,
         "rel_l2": errors,
         "roofline": rl,
         "cpu_baseline": cpu,
