@@ -37,7 +37,8 @@ EXPORTS = [
     "gsh_grid_colatitudes", "gsh_grid_prepare", "gsh_inverse", "gsh_forward",
     "gsh_adjoint_inverse", "gsh_inverse_host", "gsh_forward_host",
     "gsh_timing_enable", "gsh_timing_read", "gsh_plan_create", "gsh_plan_destroy",
-    "gsh_plan_inverse", "gsh_plan_forward", "gsh_plan_info",
+    "gsh_plan_inverse", "gsh_plan_forward", "gsh_plan_info", "gsh_device_alloc",
+    "gsh_device_free", "gsh_copy", "gsh_synchronize",
 ]
 KERNELS = ["legendre_synthesis", "fft_synthesis", "fft_analysis", "legendre_analysis",
            "coef_split", "bin_split"]

@@ -1479,4 +1479,45 @@ int gsh_plan_info(gsh_plan_t P, int* ngroups, int cap, int* n, int* lmax, int* c
   return GSH_OK;
 }
 
+// Device storage of the facade's Device* containers (the layouts of
+// CanonicalComponentField.h:74-81, 119-124 and CanonicalComponentExpansion.h:
+// 69-78 in device memory).
+int gsh_device_alloc(gsh_grid_t g, size_t bytes, void** ptr) {
+  if (!g || !ptr) return GSH_ERR_INVALID_ARGUMENT;
+  *ptr = nullptr;
+  if (bytes == 0) return GSH_OK;
+  GSH_HIP(hipSetDevice(g->device));
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) return GSH_ERR_OUT_OF_MEMORY;
+  if (hipMemset(p, 0, bytes) != hipSuccess) {
+    (void)hipFree(p);
+    return GSH_ERR_DEVICE;
+  }
+  *ptr = p;
+  return GSH_OK;
+}
+
+int gsh_device_free(gsh_grid_t g, void* ptr) {
+  if (!g) return GSH_ERR_INVALID_ARGUMENT;
+  if (!ptr) return GSH_OK;
+  GSH_HIP(hipSetDevice(g->device));
+  GSH_HIP(hipFree(ptr));
+  return GSH_OK;
+}
+
+int gsh_copy(gsh_grid_t g, void* dst, const void* src, size_t bytes) {
+  if (!g || (bytes && (!dst || !src))) return GSH_ERR_INVALID_ARGUMENT;
+  if (bytes == 0) return GSH_OK;
+  GSH_HIP(hipSetDevice(g->device));
+  GSH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
+  return GSH_OK;
+}
+
+int gsh_synchronize(gsh_grid_t g, void* stream) {
+  if (!g) return GSH_ERR_INVALID_ARGUMENT;
+  GSH_HIP(hipSetDevice(g->device));
+  GSH_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return GSH_OK;
+}
+
 }  // extern "C"

@@ -96,6 +96,19 @@ class GSHIndices {
   Int _lMax, _n;
 };
 
+// A field or coefficient array held in device memory
+// (DeviceCanonicalComponentField / DeviceCanonicalComponentExpansion in
+// DeviceCanonicalComponentField.h): the transforms take its device pointer
+// without host staging.
+template <typename T>
+concept DeviceResident = requires(T& t, const T& ct) {
+  requires T::on_device;
+  typename T::Scalar;
+  { t.DeviceData() } -> std::same_as<typename T::Scalar*>;
+  { ct.DeviceData() } -> std::same_as<const typename T::Scalar*>;
+  { ct.Size() } -> std::convertible_to<std::ptrdiff_t>;
+};
+
 template <std::floating_point Real_ = double, OrderIndexRange MRange_ = All,
           IndexRange NRange_ = All>
 class GaussLegendreGrid {
@@ -198,6 +211,44 @@ class GaussLegendreGrid {
                            reinterpret_cast<const double*>(std::ranges::data(in)),
                            reinterpret_cast<double*>(std::ranges::data(out))),
           "InverseTransformation");
+  }
+
+  // The same two transforms on device-resident arrays: no host staging, the
+  // work queued on `stream` (nullptr: the default stream); `precision` is a
+  // gsh_precision (GSH_FP64, the reference's arithmetic, by default).  The
+  // analysis accumulates into `out` as the reference does
+  // (GaussLegendreGrid.h:192).  Call Synchronize(stream) before reading the
+  // results on the host.
+  template <DeviceResident In, DeviceResident Out>
+    requires std::same_as<typename Out::Scalar, Complex> &&
+             (std::same_as<typename In::Scalar, Real> || std::same_as<typename In::Scalar, Complex>)
+  void ForwardTransformation(Int lMax, Int n, const In& in, Out& out, int precision = GSH_FP64,
+                             void* stream = nullptr) const {
+    constexpr bool real = std::same_as<typename In::Scalar, Real>;
+    if (in.Size() != FieldSize() ||
+        out.Size() != (real ? RealCoefficientSize(lMax, n) : ComplexCoefficientSize(lMax, n)))
+      throw Error(GSH_ERR_INVALID_ARGUMENT, "ForwardTransformation");
+    Check(gsh_forward(_h.get(), static_cast<int>(lMax), static_cast<int>(n),
+                      real ? GSH_REAL : GSH_COMPLEX, precision, GSH_STORE_FP64, 1,
+                      in.DeviceData(), 0, out.DeviceData(), 0, 1, stream),
+          "ForwardTransformation");
+  }
+  template <DeviceResident In, DeviceResident Out>
+    requires std::same_as<typename In::Scalar, Complex> &&
+             (std::same_as<typename Out::Scalar, Real> || std::same_as<typename Out::Scalar, Complex>)
+  void InverseTransformation(Int lMax, Int n, const In& in, Out& out, int precision = GSH_FP64,
+                             void* stream = nullptr) const {
+    constexpr bool real = std::same_as<typename Out::Scalar, Real>;
+    if (out.Size() != FieldSize() ||
+        in.Size() != (real ? RealCoefficientSize(lMax, n) : ComplexCoefficientSize(lMax, n)))
+      throw Error(GSH_ERR_INVALID_ARGUMENT, "InverseTransformation");
+    Check(gsh_inverse(_h.get(), static_cast<int>(lMax), static_cast<int>(n),
+                      real ? GSH_REAL : GSH_COMPLEX, precision, GSH_STORE_FP64, 1,
+                      in.DeviceData(), 0, out.DeviceData(), 0, stream),
+          "InverseTransformation");
+  }
+  void Synchronize(void* stream = nullptr) const {
+    Check(gsh_synchronize(_h.get(), stream), "Synchronize");
   }
 
   // GridBase.h:115-172, with a seeded splitmix64 -> Box-Muller generator (the

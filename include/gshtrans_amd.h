@@ -32,6 +32,11 @@
  *   gsh_timing_enable /    no reference counterpart (the reference has no
  *   gsh_timing_read        timing code): HIP events around each kernel
  *                          launch, on the launch stream, for benchmarks
+ *   gsh_device_alloc /     the storage of CanonicalComponentField.h:74-81,
+ *   gsh_device_free /      119-124 and CanonicalComponentExpansion.h:69-78
+ *   gsh_copy /             in device memory (the C++ facade's Device*
+ *   gsh_synchronize        containers), so transforms take no per-call
+ *                          host staging; no reference counterpart
  *
  * Layouts are the reference's:
  *   coefficients  GSHIndices<All>(lMax, lMax, n) for complex fields and
@@ -49,6 +54,8 @@
  */
 #ifndef GSHTRANS_AMD_H
 #define GSHTRANS_AMD_H
+
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -171,6 +178,15 @@ int gsh_timing_enable(gsh_grid_t grid, int enable);
 /* Waits for the recorded events and returns, per kernel id, the summed
  * duration in milliseconds and the number of launches since enabling. */
 int gsh_timing_read(gsh_grid_t grid, double* total_ms, long* launches, int nkernels);
+
+/* ---- device memory on the grid's GPU ---------------------------------- */
+/* Zero-filled device allocation of `bytes` (0 gives NULL). */
+int gsh_device_alloc(gsh_grid_t grid, size_t bytes, void** ptr);
+int gsh_device_free(gsh_grid_t grid, void* ptr);
+/* Synchronous copy of `bytes` between any two of host and device memory. */
+int gsh_copy(gsh_grid_t grid, void* dst, const void* src, size_t bytes);
+/* Waits for the work queued on `stream` (NULL: the default stream). */
+int gsh_synchronize(gsh_grid_t grid, void* stream);
 
 /* ---- transforms on host memory (staged through device buffers) -------- */
 int gsh_inverse_host(gsh_grid_t grid, int lmax, int n, int value_kind,

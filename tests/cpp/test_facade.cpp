@@ -4,7 +4,10 @@
 //   * accessors of the grid (GaussLegendreGrid.h:86-103, GridBase.h:17-113);
 //   * the coefficient round trip of tests/CheckCoeff2Coeff.h:74-83 for every
 //     upper index, complex and real fields, bar 5e4 eps;
-//   * the typed containers feeding the transforms through Data().
+//   * the typed containers feeding the transforms through Data();
+//   * the device-resident containers (Device*CanonicalComponent*): the same
+//     transforms without host staging, bit-identical to the host-staged
+//     calls, in FP64 and in the split-BF16 mode.
 // Prints one line per check and "OK" at the end; exits non-zero on failure.
 #include <GSHTrans/Core>
 
@@ -94,6 +97,73 @@ int main() {
   for (int l = 1; l <= L; l++)
     for (int m = -l; m <= l; m++) e = std::max(e, std::abs(v(l, m) - u(l, m)));
   expect(e < 5e4 * eps, "CanonicalComponentExpansion / Field round trip");
+  // Device-resident containers: no host staging, same kernels, same bits.
+  {
+    DeviceComplexCanonicalComponentExpansion<1, Grid> du(u);
+    DeviceComplexCanonicalComponentField<1, Grid> df(grid);
+    grid.InverseTransformation(L, 1, du, df);
+    DeviceComplexCanonicalComponentExpansion<1, Grid> dv(grid);
+    grid.ForwardTransformation(L, 1, df, dv);
+    grid.Synchronize();
+    const auto fh = df.ToHost();
+    const auto vh = dv.ToHost();
+    bool same = true;
+    for (int i = 0; i < grid.NumberOfCoLatitudes(); i++)
+      for (int j = 0; j < grid.NumberOfLongitudes(); j++) same = same && fh(i, j) == f(i, j);
+    expect(same, "device synthesis equals the host-staged synthesis bit for bit");
+    same = true;
+    for (int l = 1; l <= L; l++)
+      for (int m = -l; m <= l; m++) same = same && vh(l, m) == v(l, m);
+    expect(same, "device analysis equals the host-staged analysis bit for bit");
+    // Split-BF16 synthesis on the device against the FP64 field.
+    DeviceComplexCanonicalComponentField<1, Grid> dm(grid);
+    grid.InverseTransformation(L, 1, du, dm, GSH_BF16X6);
+    grid.Synchronize();
+    const auto mh = dm.ToHost();
+    double num = 0, den = 0;
+    for (int i = 0; i < grid.NumberOfCoLatitudes(); i++)
+      for (int j = 0; j < grid.NumberOfLongitudes(); j++) {
+        num += std::norm(mh(i, j) - f(i, j));
+        den += std::norm(f(i, j));
+      }
+    std::printf("  bf16x6 device synthesis relative L2 %.2e\n", std::sqrt(num / den));
+    expect(std::sqrt(num / den) < 1e-7, "bf16x6 device synthesis within 1e-7 of FP64");
+    // Split-BF16 analysis of the FP64 field (FP32 bins) against the FP64 one.
+    DeviceComplexCanonicalComponentExpansion<1, Grid> dw(grid);
+    grid.ForwardTransformation(L, 1, df, dw, GSH_BF16X6);
+    grid.Synchronize();
+    const auto wh = dw.ToHost();
+    num = den = 0;
+    for (int l = 1; l <= L; l++)
+      for (int m = -l; m <= l; m++) {
+        num += std::norm(wh(l, m) - v(l, m));
+        den += std::norm(v(l, m));
+      }
+    std::printf("  bf16x6 device analysis relative L2 %.2e\n", std::sqrt(num / den));
+    expect(std::sqrt(num / den) < 1e-7, "bf16x6 device analysis within 1e-7 of FP64");
+    // A real field on a NonNegative grid.
+    RealCanonicalComponentExpansion<0, RealGrid> ru(rgrid);
+    rgrid.RandomRealCoefficient(L, 0, ru.Data(), 41);
+    DeviceRealCanonicalComponentExpansion<0, RealGrid> dru(ru);
+    DeviceRealCanonicalComponentField<0, RealGrid> drf(rgrid);
+    rgrid.InverseTransformation(L, 0, dru, drf);
+    DeviceRealCanonicalComponentExpansion<0, RealGrid> drv(rgrid);
+    rgrid.ForwardTransformation(L, 0, drf, drv);
+    rgrid.Synchronize();
+    const auto rv = drv.ToHost();
+    double er = 0;
+    for (int l = 0; l <= L; l++)
+      for (int m = 0; m <= l; m++) er = std::max(er, std::abs(rv(l, m) - ru(l, m)));
+    expect(er < 5e4 * eps, "device real field round trip");
+    // Size mismatches raise instead of reading past the arrays.
+    bool raised = false;
+    try {
+      grid.InverseTransformation(L - 1, 1, du, df);
+    } catch (const Error& err) {
+      raised = err.status == GSH_ERR_INVALID_ARGUMENT;
+    }
+    expect(raised, "device arrays of the wrong size raise");
+  }
   // A single mode: Y_{3,2} (n = 0) synthesised then read back.
   ComplexCanonicalComponentExpansion<0, Grid> y(grid);
   y(3, 2) = 1.0;
