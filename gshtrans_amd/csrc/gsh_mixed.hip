@@ -558,12 +558,16 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   d4 acc[4][1];
 #pragma unroll
   for (int i = 0; i < 4; i++) acc[i][0] = d4{0.0, 0.0, 0.0, 0.0};
-  // One K chunk kc of buffer buf; the chains of the previous chunk (cp, zero
-  // before the first) are flushed to FP64 after this chunk's MFMAs are issued.
-  f4 cp[4][1];
+  // One K chunk kc of buffer buf.  After this chunk's MFMAs are issued, the
+  // chains of the previous chunk (cp, complete by then; zero before the
+  // first) join a pair: an odd chunk kc keeps them in pp, an even one adds
+  // them to pp in FP32 and flushes the pair sum to FP64 (one FP32 rounding
+  // per pair of 32-term chunk sums, half the FP64 work of flushing every
+  // chunk).  Chunk indices of a group are 4 g + j, so the parity is j's.
+  f4 cp[4][1], pp[4][1];
 #pragma unroll
-  for (int i = 0; i < 4; i++) cp[i][0] = f4{0.f, 0.f, 0.f, 0.f};
-  auto step = [&](int buf, int kc, const RawB& X, bool last) {
+  for (int i = 0; i < 4; i++) cp[i][0] = pp[i][0] = f4{0.f, 0.f, 0.f, 0.f};
+  auto step = [&](int buf, int kc, const RawB& X, bool last, bool odd) {
     uint4 B[1][NS];
     splitB(X, B, kc, last);
     uint4 A[4][NS];
@@ -577,7 +581,10 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
     split_mma<NS, 4, 1>(A, B, c);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      flush(acc[i][0], cp[i][0]);
+      if (odd)
+        pp[i][0] = cp[i][0];
+      else
+        flush(acc[i][0], pp[i][0] + cp[i][0]);
       cp[i][0] = c[i][0];
     }
   };
@@ -610,7 +617,7 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
     for (int j = 0; j < 4; j++) {
       loadB(min(kc0 + j + kAhead, kcl), X[(j + kAhead) & 3]);
       produce4(buf ^ 1, j);
-      step(buf, kc0 + j, X[j], false);
+      step(buf, kc0 + j, X[j], false, j & 1);
       __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
@@ -619,19 +626,22 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
     // Last group: chunks 4 (ngrp-1) .. nkc-1.
     const int buf = (ngrp - 1) & 1;
     const int kc0 = 4 * (ngrp - 1);
-    step(buf, kc0, X[0], true);
+    step(buf, kc0, X[0], true, false);
     if (kc0 + 1 < p.nkc) {
-      step(buf, kc0 + 1, X[1], true);
+      step(buf, kc0 + 1, X[1], true, true);
       if (kc0 + 2 < p.nkc) {
-        step(buf, kc0 + 2, X[2], true);
+        step(buf, kc0 + 2, X[2], true, false);
         if (kc0 + 3 < p.nkc) {
           loadB(kc0 + 3, X[3]);
-          step(buf, kc0 + 3, X[3], true);
+          step(buf, kc0 + 3, X[3], true, true);
         }
       }
     }
+    // The last chunk nkc - 1: an odd one completes the pair in pp, an even
+    // one (its predecessor's pair already flushed) stands alone.
+    const bool lodd = (p.nkc - 1) & 1;
 #pragma unroll
-    for (int i = 0; i < 4; i++) flush(acc[i][0], cp[i][0]);
+    for (int i = 0; i < 4; i++) flush(acc[i][0], lodd ? pp[i][0] + cp[i][0] : cp[i][0]);
   }
 
   // Rows l = lp + 16*lt + 4*(lane/16) + r, column 16*tile + lane%16.
