@@ -38,7 +38,8 @@ EXPORTS = [
     "gsh_adjoint_inverse", "gsh_inverse_host", "gsh_forward_host",
     "gsh_timing_enable", "gsh_timing_read", "gsh_plan_create", "gsh_plan_destroy",
     "gsh_plan_inverse", "gsh_plan_forward", "gsh_plan_info", "gsh_device_alloc",
-    "gsh_device_free", "gsh_copy", "gsh_synchronize",
+    "gsh_device_free", "gsh_copy", "gsh_synchronize", "gsh_wigner_size", "gsh_wigner_offset",
+    "gsh_wigner_table", "gsh_wigner_table_host",
 ]
 KERNELS = ["legendre_synthesis", "fft_synthesis", "fft_analysis", "legendre_analysis",
            "coef_split", "bin_split"]
@@ -60,6 +61,11 @@ def _lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `python -m gshtrans_amd.build`")
+        # torch's wheel carries its own HIP runtime (SONAME libamdhip64.so.7,
+        # requested by name as libamdhip64.so).  Loading torch first lets this
+        # library bind to that runtime by SONAME, so the process holds one
+        # HIP runtime and torch's streams and allocations are this library's.
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         i, l, d, vp = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p
         dp = ctypes.POINTER(ctypes.c_double)
@@ -83,11 +89,17 @@ def _lib():
         L.gsh_forward_host.argtypes = [vp, i, i, i, dp, dp, i]
         L.gsh_timing_enable.argtypes = [vp, i]
         L.gsh_timing_read.argtypes = [vp, dp, ctypes.POINTER(l), i]
+        L.gsh_wigner_size.restype = l
+        L.gsh_wigner_size.argtypes = [i, i, i, i, i, i]
+        L.gsh_wigner_offset.restype = l
+        L.gsh_wigner_offset.argtypes = [i, i, i, i, i, i, i, i, i]
+        L.gsh_wigner_table.argtypes = [i, i, i, i, i, dp, i, i, i, vp, vp]
+        L.gsh_wigner_table_host.argtypes = [i, i, i, i, i, dp, i, i, i, dp]
         for name in ("gsh_gl_nodes", "gsh_wigner_seed", "gsh_grid_create", "gsh_grid_destroy",
                      "gsh_grid_info", "gsh_grid_colatitudes", "gsh_grid_prepare",
                      "gsh_inverse", "gsh_forward", "gsh_adjoint_inverse",
                      "gsh_inverse_host", "gsh_forward_host", "gsh_timing_enable",
-                     "gsh_timing_read"):
+                     "gsh_timing_read", "gsh_wigner_table", "gsh_wigner_table_host"):
             getattr(L, name).restype = i
         _LIB = L
     return _LIB
@@ -205,6 +217,89 @@ def _stream(stream):
     import torch
     s = stream if stream is not None else torch.cuda.current_stream()
     return ctypes.c_void_p(s.cuda_stream)
+
+
+WIGNER_NORM = {"Ortho": 0, "FourPi": 1}
+WIGNER_STORAGE = {"ColumnMajor": 0, "RowMajor": 1}
+_MRANGE = {"All": 0, "NonNegative": 1}
+
+
+class Wigner:
+    """Wigner d tables, Wigner<double, Norm, MRange, NRange, AngleRange,
+    Storage>(lMax, mMax, nMax, theta) (Wigner.h:99-286), computed on the GPU.
+
+    `data` is one float64 device tensor in the reference's layout: a block of
+    GSHIndices<MRange>(lmax, mmax, n) (Indexing.h:72-150) per (n, theta_i),
+    n over UpperIndices() (NRange All: -nmax..nmax, NonNegative: 0..nmax,
+    Single: nmax), blocks ColumnMajor (n outer) or RowMajor (theta outer).
+    `self[n, i]` is the block of (n, theta_i) and `self(n, i, l, m)` one
+    value d^l_{mn}(theta_i) (Ortho: times sqrt(2l+1) / (2 sqrt(pi))).
+    """
+
+    def __init__(self, lmax, mmax, nmax, theta, norm="Ortho", mrange="All", nrange="Single",
+                 storage="ColumnMajor", stream=None):
+        import torch
+        theta = np.ascontiguousarray(np.atleast_1d(np.asarray(theta, dtype=np.float64)))
+        self.lmax, self.mmax, self.nmax = int(lmax), int(mmax), int(nmax)
+        self.norm, self.mrange, self.nrange, self.storage = norm, mrange, nrange, storage
+        self.nmin = {"All": -self.nmax, "NonNegative": 0, "Single": self.nmax}[nrange]
+        self.theta = theta
+        self._args = (self.lmax, self.mmax, self.nmin, self.nmax, len(theta), _MRANGE[mrange])
+        size = _lib().gsh_wigner_size(*self._args)
+        if size < 0:
+            raise GSHError(1, "Wigner")
+        self.data = torch.empty(size, dtype=torch.float64, device="cuda")
+        self.recompute(theta, stream)
+
+    def recompute(self, theta, stream=None):
+        """Wigner::ReCompute (Wigner.h:142-152): the same table for new angles."""
+        theta = np.ascontiguousarray(np.atleast_1d(np.asarray(theta, dtype=np.float64)))
+        assert len(theta) == len(self.theta)
+        self.theta = theta
+        _check(_lib().gsh_wigner_table(*self._args[:5], _dp(theta), self._args[5],
+                                       WIGNER_NORM[self.norm], WIGNER_STORAGE[self.storage],
+                                       _ptr(self.data), _stream(stream)), "gsh_wigner_table")
+
+    def upper_indices(self):
+        return range(self.nmin, self.nmax + 1)
+
+    def offset(self, n, itheta):
+        o = _lib().gsh_wigner_offset(*self._args, WIGNER_STORAGE[self.storage], n, itheta)
+        if o < 0:
+            raise GSHError(1, "Wigner.offset")
+        return o
+
+    def index(self, n, l, m):
+        """Index(l, m) of GSHIndices<MRange>(lmax, mmax, n) (Indexing.h:54-65, 104-146)."""
+        mm = min(self.lmax, self.mmax)
+        nabs = abs(n)
+        assert nabs <= l <= self.lmax and abs(m) <= min(l, mm)
+        if self.mrange == "All":
+            if mm >= nabs:
+                off = (l * l - nabs * nabs if l <= mm else
+                       (mm + 1) ** 2 - nabs * nabs + (l - 1 - mm) * (2 * mm + 1))
+            else:
+                off = (l - nabs) * (2 * mm + 1)
+            return off + m + min(l, mm)
+        assert m >= 0
+        if mm >= nabs:
+            off = (l * (l + 1) // 2 - nabs * (nabs + 1) // 2 if l <= mm else
+                   (mm + 1) * (mm + 2) // 2 - nabs * (nabs + 1) // 2 + (l - 1 - mm) * (mm + 1))
+        else:
+            off = (l - nabs) * (mm + 1)
+        return off + m
+
+    def block_size(self, n):
+        mm = min(self.lmax, self.mmax)
+        return self.index(n, self.lmax, mm) + 1
+
+    def __getitem__(self, key):
+        n, itheta = key
+        o = self.offset(n, itheta)
+        return self.data[o:o + self.block_size(n)]
+
+    def __call__(self, n, itheta, l, m):
+        return float(self.data[self.offset(n, itheta) + self.index(n, l, m)])
 
 
 class GaussLegendreGrid:

@@ -127,6 +127,31 @@ double ortho_scale(long l, double v) {
   return factor * std::sqrt(static_cast<double>(2 * l + 1)) * v;
 }
 
+// Recurrence coefficients {a, b, c} of the Ortho-normalised values of order
+// m at degree l > l0 = max(|m|, |n|), in long double:
+//   d_l = [(2l-1)(l(l-1)cos - m n) d_{l-1} - l sqrt(((l-1)^2-n^2)((l-1)^2-m^2)) d_{l-2}]
+//         / ((l-1) sqrt((l^2-n^2)(l^2-m^2)))            (Wigner.h:407-445)
+// with the l = |n|+1 one-term form (Wigner.h:368-376) as its n m / (l-1) -> sign(n) m
+// limit, and the Ortho ratio sqrt(2l+1) folded in:
+//   e_l = (a cos - b) e_{l-1} - c e_{l-2}.
+double4 rec_coef(long l, long m, long n) {
+  const long l0 = std::max(std::labs(m), std::labs(n));
+  const long double ll = l, mm = m, nn = n;
+  const long double den = std::sqrt((ll * ll - nn * nn) * (ll * ll - mm * mm));
+  const long double r1 = std::sqrt((2 * ll + 1) / (2 * ll - 1));
+  const long double a = (2 * ll - 1) * ll / den;
+  long double b = 0;
+  if (m * n != 0) b = (2 * ll - 1) * mm * nn / ((ll - 1) * den);
+  long double c = 0;
+  if (l >= l0 + 2) {
+    const long double r2 = std::sqrt((2 * ll + 1) / (2 * ll - 3));
+    c = ll * std::sqrt(((ll - 1) * (ll - 1) - nn * nn) * ((ll - 1) * (ll - 1) - mm * mm)) /
+        ((ll - 1) * den) * r2;
+  }
+  return make_double4(static_cast<double>(a * r1), static_cast<double>(b * r1),
+                      static_cast<double>(c), 0.0);
+}
+
 template <typename F>
 void parallel_for(long n, F f) {
   unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
@@ -223,11 +248,7 @@ int ensure_ndata(gsh_grid_s* g, int n, NData** out) {
       seeds[(m + L) * nt + t] = ortho_scale(l0, seed_value(m, n, a));
     }
   });
-  // Recurrence coefficients per (m, l), l0 < l <= L, in long double:
-  //   d_l = [(2l-1)(l(l-1)cos - m n) d_{l-1} - l sqrt(((l-1)^2-n^2)((l-1)^2-m^2)) d_{l-2}]
-  //         / ((l-1) sqrt((l^2-n^2)(l^2-m^2)))            (Wigner.h:407-445)
-  // with the l = |n|+1 one-term form (Wigner.h:368-376) as its n m / (l-1) -> sign(n) m
-  // limit, and the Ortho ratio sqrt(2l+1) folded in.
+  // Recurrence coefficients per (m, l), l0 < l <= L (rec_coef).
   std::vector<long> roff(norders);
   long total = 0;
   for (long m = -L; m <= L; m++) {
@@ -238,23 +259,7 @@ int ensure_ndata(gsh_grid_s* g, int n, NData** out) {
   parallel_for(norders, [&](long mi) {
     const long m = mi - L;
     const long l0 = std::max(std::labs(m), nabs);
-    for (long l = l0 + 1; l <= L; l++) {
-      const long double ll = l, mm = m, nn = n;
-      const long double den = std::sqrt((ll * ll - nn * nn) * (ll * ll - mm * mm));
-      const long double r1 = std::sqrt((2 * ll + 1) / (2 * ll - 1));
-      const long double a = (2 * ll - 1) * ll / den;
-      long double b = 0;
-      if (m * n != 0) b = (2 * ll - 1) * mm * nn / ((ll - 1) * den);
-      long double c = 0;
-      if (l >= l0 + 2) {
-        const long double r2 = std::sqrt((2 * ll + 1) / (2 * ll - 3));
-        c = ll * std::sqrt(((ll - 1) * (ll - 1) - nn * nn) * ((ll - 1) * (ll - 1) - mm * mm)) /
-            ((ll - 1) * den) * r2;
-      }
-      const long idx = roff[mi] + (l - l0);
-      rec[idx] = make_double4(static_cast<double>(a * r1), static_cast<double>(b * r1),
-                              static_cast<double>(c), 0.0);
-    }
+    for (long l = l0 + 1; l <= L; l++) rec[roff[mi] + (l - l0)] = rec_coef(l, m, n);
   });
   NData d;
   GSH_HIP(hipSetDevice(g->device));
@@ -1518,6 +1523,171 @@ int gsh_synchronize(gsh_grid_t g, void* stream) {
   GSH_HIP(hipSetDevice(g->device));
   GSH_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
   return GSH_OK;
+}
+
+// ---- Wigner d tables (Wigner.h:99-286) ----------------------------------
+
+namespace {
+
+// GSHIndices<MRange>(lmax, mmax, n).Size() (Indexing.h:104-150).
+long gsh_block_size(long lmax, long mmax, long n, bool all) {
+  const long mm = std::min(lmax, mmax), nabs = std::labs(n);
+  auto offset = [&](long l) {
+    if (all) {
+      if (mm >= nabs)
+        return l <= mm ? l * l - nabs * nabs
+                       : (mm + 1) * (mm + 1) - nabs * nabs + (l - 1 - mm) * (2 * mm + 1);
+      return (l - nabs) * (2 * mm + 1);
+    }
+    if (mm >= nabs)
+      return l <= mm ? (l * (l + 1)) / 2 - (nabs * (nabs + 1)) / 2
+                     : ((mm + 1) * (mm + 2)) / 2 - (nabs * (nabs + 1)) / 2 + (l - 1 - mm) * (mm + 1);
+    return (l - nabs) * (mm + 1);
+  };
+  return offset(lmax) + (all ? 2 * std::min(lmax, mm) + 1 : std::min(lmax, mm) + 1);
+}
+
+int wigner_check(int lmax, int mmax, int nmin, int nmax, int ntheta, int mrange) {
+  if (lmax < 0 || mmax < 0 || ntheta < 1 || nmin > nmax) return GSH_ERR_INVALID_ARGUMENT;
+  if (mrange != GSH_ALL && mrange != GSH_NONNEGATIVE) return GSH_ERR_INVALID_ARGUMENT;
+  if (std::max(std::abs(nmin), std::abs(nmax)) > lmax) return GSH_ERR_UPPER_INDEX;
+  return GSH_OK;
+}
+
+// Offsets of the (n, theta_i) blocks in the order of Wigner::Indices
+// (Wigner.h:240-250): ColumnMajor n outer, RowMajor theta outer.
+std::vector<long> wigner_offsets(int lmax, int mmax, int nmin, int nmax, int ntheta, bool all,
+                                 int storage, long* total) {
+  const int nn = nmax - nmin + 1;
+  std::vector<long> size(nn), off((size_t)nn * ntheta);
+  for (int ni = 0; ni < nn; ni++) size[ni] = gsh_block_size(lmax, mmax, nmin + ni, all);
+  long acc = 0;
+  if (storage == GSH_COLUMN_MAJOR) {
+    for (int ni = 0; ni < nn; ni++)
+      for (int i = 0; i < ntheta; i++) {
+        off[(size_t)ni * ntheta + i] = acc;
+        acc += size[ni];
+      }
+  } else {
+    for (int i = 0; i < ntheta; i++)
+      for (int ni = 0; ni < nn; ni++) {
+        off[(size_t)ni * ntheta + i] = acc;
+        acc += size[ni];
+      }
+  }
+  *total = acc;
+  return off;
+}
+
+}  // namespace
+
+long gsh_wigner_size(int lmax, int mmax, int nmin, int nmax, int ntheta, int mrange) {
+  if (wigner_check(lmax, mmax, nmin, nmax, ntheta, mrange)) return -1;
+  long total = 0;
+  for (int n = nmin; n <= nmax; n++) total += gsh_block_size(lmax, mmax, n, mrange == GSH_ALL);
+  return total * ntheta;
+}
+
+long gsh_wigner_offset(int lmax, int mmax, int nmin, int nmax, int ntheta, int mrange,
+                       int storage, int n, int itheta) {
+  if (wigner_check(lmax, mmax, nmin, nmax, ntheta, mrange)) return -1;
+  if (storage != GSH_COLUMN_MAJOR && storage != GSH_ROW_MAJOR) return -1;
+  if (n < nmin || n > nmax || itheta < 0 || itheta >= ntheta) return -1;
+  long total = 0;
+  const auto off = wigner_offsets(lmax, mmax, nmin, nmax, ntheta, mrange == GSH_ALL, storage,
+                                  &total);
+  return off[(size_t)(n - nmin) * ntheta + itheta];
+}
+
+int gsh_wigner_table(int lmax, int mmax, int nmin, int nmax, int ntheta, const double* theta,
+                     int mrange, int norm, int storage, double* table, void* stream) {
+  int st = wigner_check(lmax, mmax, nmin, nmax, ntheta, mrange);
+  if (st) return st;
+  if (!theta || !table || (norm != GSH_ORTHO && norm != GSH_FOURPI) ||
+      (storage != GSH_COLUMN_MAJOR && storage != GSH_ROW_MAJOR))
+    return GSH_ERR_INVALID_ARGUMENT;
+  int ndev = 0, dev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return GSH_ERR_NO_DEVICE;
+  if (hipGetDevice(&dev) != hipSuccess) return GSH_ERR_NO_DEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return GSH_ERR_NO_DEVICE;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return GSH_ERR_NO_DEVICE;
+  const bool all = mrange == GSH_ALL;
+  gsh::WignerParams p{};
+  p.lmax = lmax;
+  p.mmax = std::min(lmax, mmax);
+  p.mlo = all ? -p.mmax : 0;
+  p.nm = p.mmax - p.mlo + 1;
+  p.nmin = nmin;
+  p.nn = nmax - nmin + 1;
+  p.ntheta = ntheta;
+  p.all = all ? 1 : 0;
+  p.fourpi = norm == GSH_FOURPI ? 1 : 0;
+  long total = 0;
+  const auto off = wigner_offsets(lmax, mmax, nmin, nmax, ntheta, all, storage, &total);
+  // Seeds (Ortho, at l0 = max(|m|, |n|)) and recurrence coefficients, as for
+  // the grid's tables (ensure_ndata), for the caller's angles.
+  std::vector<double> seeds((size_t)p.nn * ntheta * p.nm), cosv(ntheta), inv(lmax + 1);
+  parallel_for((long)p.nn * ntheta, [&](long k) {
+    const int ni = static_cast<int>(k / ntheta), i = static_cast<int>(k % ntheta);
+    const long n = nmin + ni;
+    SeedArgs a(theta[i]);
+    for (int mi = 0; mi < p.nm; mi++) {
+      const long m = p.mlo + mi;
+      seeds[(size_t)k * p.nm + mi] = ortho_scale(std::max(std::labs(m), std::labs(n)),
+                                                 seed_value(m, n, a));
+    }
+  });
+  std::vector<double4> rec((size_t)p.nn * (lmax + 1) * p.nm, make_double4(0.0, 0.0, 0.0, 0.0));
+  parallel_for((long)p.nn * p.nm, [&](long k) {
+    const int ni = static_cast<int>(k / p.nm), mi = static_cast<int>(k % p.nm);
+    const long n = nmin + ni, m = p.mlo + mi;
+    const long l0 = std::max(std::labs(m), std::labs(n));
+    for (long l = l0 + 1; l <= lmax; l++)
+      rec[((size_t)ni * (lmax + 1) + l) * p.nm + mi] = rec_coef(l, m, n);
+  });
+  for (int i = 0; i < ntheta; i++) cosv[i] = std::cos(theta[i]);
+  for (int l = 0; l <= lmax; l++) inv[l] = 1.0 / ortho_scale(l, 1.0);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  void* buf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  const size_t bytes[5] = {cosv.size() * sizeof(double), seeds.size() * sizeof(double),
+                           rec.size() * sizeof(double4), inv.size() * sizeof(double),
+                           off.size() * sizeof(long)};
+  const void* src[5] = {cosv.data(), seeds.data(), rec.data(), inv.data(), off.data()};
+  st = GSH_OK;
+  for (int k = 0; k < 5 && st == GSH_OK; k++) {
+    if (hipMalloc(&buf[k], bytes[k]) != hipSuccess) st = GSH_ERR_OUT_OF_MEMORY;
+    else if (hipMemcpy(buf[k], src[k], bytes[k], hipMemcpyHostToDevice) != hipSuccess)
+      st = GSH_ERR_DEVICE;
+  }
+  if (st == GSH_OK) {
+    p.cosv = static_cast<const double*>(buf[0]);
+    p.seeds = static_cast<const double*>(buf[1]);
+    p.rec = static_cast<const double4*>(buf[2]);
+    p.inv_ortho = static_cast<const double*>(buf[3]);
+    p.blockoff = static_cast<const long*>(buf[4]);
+    p.out = table;
+    if (gsh::launch_wigner_table(p, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+      st = GSH_ERR_DEVICE;
+  }
+  for (auto b : buf)
+    if (b) (void)hipFree(b);
+  return st;
+}
+
+int gsh_wigner_table_host(int lmax, int mmax, int nmin, int nmax, int ntheta,
+                          const double* theta, int mrange, int norm, int storage, double* table) {
+  const long size = gsh_wigner_size(lmax, mmax, nmin, nmax, ntheta, mrange);
+  if (size < 0) return wigner_check(lmax, mmax, nmin, nmax, ntheta, mrange);
+  if (!table) return GSH_ERR_INVALID_ARGUMENT;
+  void* d = nullptr;
+  if (hipMalloc(&d, size * sizeof(double)) != hipSuccess) return GSH_ERR_OUT_OF_MEMORY;
+  int st = gsh_wigner_table(lmax, mmax, nmin, nmax, ntheta, theta, mrange, norm, storage,
+                            static_cast<double*>(d), nullptr);
+  if (st == GSH_OK && hipMemcpy(table, d, size * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    st = GSH_ERR_DEVICE;
+  (void)hipFree(d);
+  return st;
 }
 
 }  // extern "C"

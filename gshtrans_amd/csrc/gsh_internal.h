@@ -188,6 +188,22 @@ hipError_t launch_widen_c32(const void* in, double* out, long total, hipStream_t
 hipError_t launch_narrow_c64(const double* in, void* out, long stride, const long* csize,
                              long cmax, int count, hipStream_t s);
 
+// Wigner d tables (gsh_wigner.hip).  Orders m = mlo .. mlo + nm - 1 (mmax =
+// min(lmax, caller's mMax), mlo = -mmax for MRange All, 0 for NonNegative),
+// upper indices n = nmin .. nmin + nn - 1, ntheta angles.
+struct WignerParams {
+  int lmax, mmax, mlo, nm, nmin, nn, ntheta;
+  int all;     // MRange All (else NonNegative)
+  int fourpi;  // FourPi normalisation (else Ortho)
+  const double* cosv;      // cos(theta_i)
+  const double* seeds;     // Ortho value at l0 of (n, theta_i, m): [(ni * ntheta + i) * nm + mi]
+  const double4* rec;      // recurrence coefficients of (n, l, m): [(ni * (lmax + 1) + l) * nm + mi]
+  const double* inv_ortho; // 2 sqrt(pi) / sqrt(2l + 1), l <= lmax
+  const long* blockoff;    // first entry of block (n, theta_i): [ni * ntheta + i]
+  double* out;
+};
+hipError_t launch_wigner_table(const WignerParams& p, hipStream_t s);
+
 // lmax == 0: mode 1 synthesis, 0 analysis, 2 adjoint of the synthesis.
 hipError_t launch_lmax0(int mode, int real, int nfield, const double* in, long istride,
                         double* out, long ostride, hipStream_t s);

@@ -32,6 +32,9 @@
  *   gsh_timing_enable /    no reference counterpart (the reference has no
  *   gsh_timing_read        timing code): HIP events around each kernel
  *                          launch, on the launch stream, for benchmarks
+ *   gsh_wigner_table       Wigner<...>(lMax, mMax, nMax, theta) table build,
+ *   (_host, _size,         Wigner.h:99-286 (ComputeAll :305-314, offsets
+ *    _offset)              :116-132, 280-286), computed on the GPU
  *   gsh_device_alloc /     the storage of CanonicalComponentField.h:74-81,
  *   gsh_device_free /      119-124 and CanonicalComponentExpansion.h:69-78
  *   gsh_copy /             in device memory (the C++ facade's Device*
@@ -187,6 +190,30 @@ int gsh_device_free(gsh_grid_t grid, void* ptr);
 int gsh_copy(gsh_grid_t grid, void* dst, const void* src, size_t bytes);
 /* Waits for the work queued on `stream` (NULL: the default stream). */
 int gsh_synchronize(gsh_grid_t grid, void* stream);
+
+/* ---- Wigner d tables ---------------------------------------------------- */
+/* The table of Wigner<double, Norm, MRange, NRange, AngleRange, Storage>(
+ * lmax, mmax, nmax, theta) (Wigner.h:99-286) for upper indices nmin..nmax
+ * (NRange All: -nmax..nmax, NonNegative: 0..nmax, Single: nmax..nmax) and
+ * ntheta angles: one GSHIndices<MRange>(lmax, mmax, n) block of d^l_{mn}
+ * (Ortho: times sqrt(2l+1) / (2 sqrt(pi)), Wigner.h:475-485) per
+ * (n, theta_i), blocks in ColumnMajor (n outer) or RowMajor (theta outer)
+ * order (Wigner.h:240-250, 280-286).  mrange is GSH_ALL or GSH_NONNEGATIVE. */
+enum gsh_wigner_norm { GSH_ORTHO = 0, GSH_FOURPI = 1 };
+enum gsh_wigner_storage { GSH_COLUMN_MAJOR = 0, GSH_ROW_MAJOR = 1 };
+/* Number of doubles of the table (-1 for invalid arguments). */
+long gsh_wigner_size(int lmax, int mmax, int nmin, int nmax, int ntheta, int mrange);
+/* First entry of block (n, itheta) (-1 for invalid arguments). */
+long gsh_wigner_offset(int lmax, int mmax, int nmin, int nmax, int ntheta, int mrange,
+                       int storage, int n, int itheta);
+/* Fills `table` (device memory of the current device, gsh_wigner_size
+ * doubles; theta in host memory) and waits for the GPU. */
+int gsh_wigner_table(int lmax, int mmax, int nmin, int nmax, int ntheta, const double* theta,
+                     int mrange, int norm, int storage, double* table, void* stream);
+/* The same into host memory. */
+int gsh_wigner_table_host(int lmax, int mmax, int nmin, int nmax, int ntheta,
+                          const double* theta, int mrange, int norm, int storage,
+                          double* table);
 
 /* ---- transforms on host memory (staged through device buffers) -------- */
 int gsh_inverse_host(gsh_grid_t grid, int lmax, int n, int value_kind,
