@@ -77,23 +77,41 @@ template <OrderIndexRange MRange>
 class GSHIndices {
  public:
   using Int = std::ptrdiff_t;
-  GSHIndices(Int lMax, Int /*mMax*/, Int n) : _lMax{lMax}, _n{n} {}
+  GSHIndices(Int lMax, Int mMax, Int n) : _lMax{lMax}, _mMax{std::min(lMax, mMax)}, _n{n} {}
   Int MinDegree() const { return std::abs(_n); }
   Int MaxDegree() const { return _lMax; }
-  Int Size() const {
-    return gsh_coefficient_size(static_cast<int>(_lMax), static_cast<int>(_n),
-                                std::same_as<MRange, All> ? GSH_COMPLEX : GSH_REAL);
+  Int MaxOrder() const { return _mMax; }
+  // GSHIndices::OffsetForDegree (Indexing.h:104-131): degrees below
+  // min(lMax, mMax) hold every order, those above 2 mMax + 1 (All) or
+  // mMax + 1 (NonNegative).
+  Int OffsetForDegree(Int l) const {
+    const Int na = std::abs(_n), mm = _mMax;
+    if constexpr (std::same_as<MRange, All>) {
+      if (mm >= na)
+        return l <= mm ? l * l - na * na
+                       : (mm + 1) * (mm + 1) - na * na + (l - 1 - mm) * (2 * mm + 1);
+      return (l - na) * (2 * mm + 1);
+    } else {
+      if (mm >= na)
+        return l <= mm ? l * (l + 1) / 2 - na * (na + 1) / 2
+                       : (mm + 1) * (mm + 2) / 2 - na * (na + 1) / 2 + (l - 1 - mm) * (mm + 1);
+      return (l - na) * (mm + 1);
+    }
   }
+  Int Size() const {
+    const Int top = std::min(_lMax, _mMax);
+    return OffsetForDegree(_lMax) + (std::same_as<MRange, All> ? 2 * top + 1 : top + 1);
+  }
+  // Index(l, m) (Indexing.h:54-65, 143-146).
   Int Index(Int l, Int m) const {
-    const Int na = std::abs(_n);
     if constexpr (std::same_as<MRange, All>)
-      return l * l - na * na + m + l;
+      return OffsetForDegree(l) + m + std::min(l, _mMax);
     else
-      return l * (l + 1) / 2 - na * (na + 1) / 2 + m;
+      return OffsetForDegree(l) + m;
   }
 
  private:
-  Int _lMax, _n;
+  Int _lMax, _mMax, _n;
 };
 
 // A field or coefficient array held in device memory

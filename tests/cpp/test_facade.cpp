@@ -164,6 +164,48 @@ int main() {
     }
     expect(raised, "device arrays of the wrong size raise");
   }
+  // Wigner tables (Wigner.h:99-286) computed on the GPU: Gauss-Legendre
+  // orthonormality of the Ortho values, FourPi = Ortho / (sqrt(2l+1) /
+  // (2 sqrt(pi))), RowMajor blocks equal to ColumnMajor ones, NonNegative
+  // orders equal to the All table's, and a single-angle table.
+  {
+    std::vector<double> th(grid.CoLatitudes().begin(), grid.CoLatitudes().end());
+    std::vector<double> wt(grid.CoLatitudeWeights().begin(), grid.CoLatitudeWeights().end());
+    Wigner<double, Ortho, All, All, Multiple, ColumnMajor> w(L, L, 2, th);
+    Wigner<double, FourPi, All, All, Multiple, RowMajor> w4(L, L, 2, th);
+    Wigner<double, Ortho, NonNegative, NonNegative, Multiple, ColumnMajor> wp(L, L, 2, th);
+    double orth = 0, rel = 0;
+    bool same = true;
+    for (int n = -2; n <= 2; n++)
+      for (int m : {-5, 0, 3, 17})
+        for (int l1 = std::max(std::abs(m), std::abs(n)); l1 <= L; l1++)
+          for (int l2 = l1; l2 <= L; l2++) {
+            double s = 0;
+            for (int i = 0; i <= L; i++) s += wt[i] * w(n, i)(l1, m) * w(n, i)(l2, m);
+            orth = std::max(orth, std::abs(2 * M_PI * s - (l1 == l2 ? 1.0 : 0.0)));
+          }
+    for (int n = -2; n <= 2; n++)
+      for (int i = 0; i <= L; i += 7)
+        for (int l = std::abs(n); l <= L; l++)
+          for (int m = -l; m <= l; m++) {
+            const double f = std::sqrt(2.0 * l + 1) / (2 * std::sqrt(M_PI));
+            rel = std::max(rel, std::abs(w4(n, i)(l, m) * f - w(n, i)(l, m)));
+            if (n >= 0 && m >= 0) same = same && wp(n, i)(l, m) == w(n, i)(l, m);
+          }
+    std::printf("  Wigner orthonormality %.2e, FourPi vs Ortho %.2e\n", orth, rel);
+    expect(orth < 1e-13, "Wigner Ortho table orthonormal on the Gauss-Legendre nodes");
+    expect(rel < 1e-14, "Wigner FourPi = Ortho / sqrt(2l+1) / (2 sqrt(pi)), RowMajor blocks");
+    expect(same, "Wigner NonNegative orders and upper indices equal the All table's");
+    Wigner<double, Ortho, All, Single, Single, ColumnMajor> w1(L, 5, 1, th[4]);
+    double d1 = 0;
+    for (int l = 1; l <= L; l++)
+      for (int m = -std::min(l, 5); m <= std::min(l, 5); m++)
+        d1 = std::max(d1, std::abs(w1(1, 0)(l, m) - w(1, 4)(l, m)));
+    std::printf("  single-angle table vs full table %.2e; MaxOrder(3) %ld, MaxOrder(9) %ld\n", d1,
+                static_cast<long>(w1.MaxOrder(3)), static_cast<long>(w1.MaxOrder(9)));
+    expect(d1 == 0 && w1.MaxOrder(3) == 3 && w1.MaxOrder(9) == 5,
+           "single-angle Wigner with mMax < lMax equals the full table's orders");
+  }
   // A single mode: Y_{3,2} (n = 0) synthesised then read back.
   ComplexCanonicalComponentExpansion<0, Grid> y(grid);
   y(3, 2) = 1.0;
