@@ -39,7 +39,7 @@ EXPORTS = [
     "gsh_timing_enable", "gsh_timing_read", "gsh_plan_create", "gsh_plan_destroy",
     "gsh_plan_inverse", "gsh_plan_forward", "gsh_plan_info", "gsh_device_alloc",
     "gsh_device_free", "gsh_copy", "gsh_synchronize", "gsh_wigner_size", "gsh_wigner_offset",
-    "gsh_wigner_table", "gsh_wigner_table_host",
+    "gsh_wigner_table", "gsh_wigner_table_host", "gsh_forward_product",
 ]
 KERNELS = ["legendre_synthesis", "fft_synthesis", "fft_analysis", "legendre_analysis",
            "coef_split", "bin_split"]
@@ -89,6 +89,7 @@ def _lib():
         L.gsh_forward_host.argtypes = [vp, i, i, i, dp, dp, i]
         L.gsh_timing_enable.argtypes = [vp, i]
         L.gsh_timing_read.argtypes = [vp, dp, ctypes.POINTER(l), i]
+        L.gsh_forward_product.argtypes = [vp, i, i, i, i, i, vp, vp, l, vp, l, i, vp]
         L.gsh_wigner_size.restype = l
         L.gsh_wigner_size.argtypes = [i, i, i, i, i, i]
         L.gsh_wigner_offset.restype = l
@@ -99,7 +100,8 @@ def _lib():
                      "gsh_grid_info", "gsh_grid_colatitudes", "gsh_grid_prepare",
                      "gsh_inverse", "gsh_forward", "gsh_adjoint_inverse",
                      "gsh_inverse_host", "gsh_forward_host", "gsh_timing_enable",
-                     "gsh_timing_read", "gsh_wigner_table", "gsh_wigner_table_host"):
+                     "gsh_timing_read", "gsh_wigner_table", "gsh_wigner_table_host",
+                     "gsh_forward_product"):
             getattr(L, name).restype = i
         _LIB = L
     return _LIB
@@ -431,6 +433,35 @@ class GaussLegendreGrid:
                                   self.FieldSize(), _ptr(coeffs), C, 1 if accumulate else 0,
                                   _stream(stream)), "gsh_forward")
         return coeffs if field.dim() == 3 else coeffs[0]
+
+    def forward_product(self, lmax, n, field_a, field_b, coeffs=None, real=False,
+                        precision="fp64", accumulate=False, stream=None):
+        """Analysis of the pointwise product field_a * field_b (upper index n =
+        N0 + N1 of the two factors; CanonicalComponentFieldBinary.h:203-253),
+        the product formed inside the ring FFT's loads: no product field is
+        written to memory."""
+        import torch
+        fa = field_a if field_a.dim() == 3 else field_a.unsqueeze(0)
+        fb = field_b if field_b.dim() == 3 else field_b.unsqueeze(0)
+        assert fa.shape == fb.shape and fa.dtype == fb.dtype
+        B = fa.shape[0]
+        C = max(int(_lib().gsh_coefficient_size(lmax, n, REAL if real else COMPLEX)), 1)
+        if coeffs is None:
+            coeffs = torch.zeros((B, C), dtype=torch.complex128, device=fa.device)
+        _check(_lib().gsh_forward_product(self._h, lmax, n, REAL if real else COMPLEX,
+                                          PRECISION[precision], B, _ptr(fa), _ptr(fb),
+                                          self.FieldSize(), _ptr(coeffs), C,
+                                          1 if accumulate else 0, _stream(stream)),
+               "gsh_forward_product")
+        return coeffs if field_a.dim() == 3 else coeffs[0]
+
+    def product(self, lmax, u, n0, v, n1, real=False, precision="fp64", stream=None):
+        """Coefficients (upper index n0 + n1) of the product of the fields of
+        expansions u (n0) and v (n1): two syntheses and one fused analysis."""
+        fa = self.inverse(lmax, n0, u, real=real, precision=precision, stream=stream)
+        fb = self.inverse(lmax, n1, v, real=real, precision=precision, stream=stream)
+        return self.forward_product(lmax, n0 + n1, fa, fb, real=real, precision=precision,
+                                    stream=stream)
 
     def adjoint(self, lmax, n, field, coeffs=None, real=False, precision="fp64", stream=None):
         """Adjoint of the synthesis, coeffs = S_n^H(field), for a batch: the

@@ -831,6 +831,7 @@ struct RunOpts {
   int f32 = 0;                   // fields stored in FP32
   int synth_tiles = 0;           // FP64 synthesis: 16-column tiles per workgroup (0: auto)
   int fft = 0;                   // 0: auto, 1: wavefront-per-ring kernel, 2: LDS kernel
+  const void* field2 = nullptr;  // analysis of the pointwise product field * field2
 };
 
 bool use_wave_fft(const FFTParams& q, const RunOpts& o) {
@@ -923,6 +924,7 @@ int run_forward(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
   const int ncol = 2 * batch;
   FFTParams q = fft_params(g, lmax, real, batch);
   q.fin = field;
+  q.fin2 = o.field2;
   q.fstride = fstride;
   q.f32 = o.f32;
   q.fmap = o.fmap;
@@ -1044,6 +1046,33 @@ int gsh_forward(gsh_grid_t g, int lmax, int n, int value_kind, int precision, in
                 int accumulate, void* stream) {
   return forward_impl(g, lmax, n, value_kind, precision, storage, batch, field, field_stride,
                       coeffs, coeff_stride, accumulate, stream, 0);
+}
+
+// Analysis of the pointwise product of two fields (the product field of
+// CanonicalComponentFieldBinary.h:203-253 and CanonicalComponentFieldOverloads.h:
+// 207-233, upper index n = N0 + N1), the product formed as the ring FFT loads
+// its rings.
+int gsh_forward_product(gsh_grid_t g, int lmax, int n, int value_kind, int precision, int batch,
+                        const void* field_a, const void* field_b, long field_stride,
+                        void* coeffs, long coeff_stride, int accumulate, void* stream) {
+  int st = check_call(g, lmax, n, value_kind, precision, GSH_STORE_FP64, batch);
+  if (st) return st;
+  if (batch == 0) return GSH_OK;
+  if (!coeffs || !field_a || !field_b) return GSH_ERR_INVALID_ARGUMENT;
+  if (lmax == 0) return GSH_ERR_UNSUPPORTED;  // the reference's lMax = 0 branch reads one value
+  const int real = value_kind == GSH_REAL;
+  const long csize = gsh_coefficient_size(lmax, n, value_kind);
+  const long fsize = static_cast<long>(g->ntheta()) * g->nphi();
+  if (batch > 1 && (coeff_stride < csize || field_stride < fsize)) return GSH_ERR_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> lock(g->mu);
+  GSH_HIP(hipSetDevice(g->device));
+  NData* d;
+  if ((st = ensure_ndata(g, n, &d))) return st;
+  RunOpts o;
+  o.field2 = field_b;
+  return run_forward(g, d, lmax, n, real, precision, batch, field_a, field_stride,
+                     static_cast<double*>(coeffs), coeff_stride, accumulate, 0, s, o);
 }
 
 int gsh_timing_enable(gsh_grid_t g, int enable) {

@@ -239,6 +239,21 @@ __global__ __launch_bounds__(kFFTThreads) void fft_anal(FFTParams p) {
       } else {
         x = static_cast<const double2*>(p.fin)[o];
       }
+      if (p.fin2) {  // pointwise product with the second field
+        if (p.real) {
+          x.x *= p.f32 ? (double)static_cast<const float*>(p.fin2)[o]
+                       : static_cast<const double*>(p.fin2)[o];
+        } else {
+          double2 y;
+          if (p.f32) {
+            const float2 z = static_cast<const float2*>(p.fin2)[o];
+            y = make_double2(z.x, z.y);
+          } else {
+            y = static_cast<const double2*>(p.fin2)[o];
+          }
+          x = make_double2(x.x * y.x - x.y * y.y, x.x * y.y + x.y * y.x);
+        }
+      }
     }
     buf[idx] = x;
   }

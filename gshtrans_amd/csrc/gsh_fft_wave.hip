@@ -447,16 +447,29 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
   double2 v[R];
   if (fv) {
     const long ia = (long)field_index(p.fmap, f0) * p.fstride + (long)theta * N + lane;
-    if (!REAL) {
+    const long ib =
+        REAL && f1v ? (long)field_index(p.fmap, f0 + 1) * p.fstride + (long)theta * N + lane : ia;
+    if (!p.fin2) {
+      if (!REAL) {
 #pragma unroll
-      for (int n1 = 0; n1 < R; n1++) v[n1] = ld_cplx<F32>(p.fin, ia + 64 * n1);
+        for (int n1 = 0; n1 < R; n1++) v[n1] = ld_cplx<F32>(p.fin, ia + 64 * n1);
+      } else {
+#pragma unroll
+        for (int n1 = 0; n1 < R; n1++) {
+          const double xa = ld_real<F32>(p.fin, ia + 64 * n1);
+          const double xb = ld_real<F32>(p.fin, ib + 64 * n1);
+          v[n1] = make_double2(xa, f1v ? xb : 0.0);
+        }
+      }
+    } else if (!REAL) {  // pointwise product of two fields
+#pragma unroll
+      for (int n1 = 0; n1 < R; n1++)
+        v[n1] = cmul(ld_cplx<F32>(p.fin, ia + 64 * n1), ld_cplx<F32>(p.fin2, ia + 64 * n1));
     } else {
-      const long ib =
-          f1v ? (long)field_index(p.fmap, f0 + 1) * p.fstride + (long)theta * N + lane : ia;
 #pragma unroll
       for (int n1 = 0; n1 < R; n1++) {
-        const double xa = ld_real<F32>(p.fin, ia + 64 * n1);
-        const double xb = ld_real<F32>(p.fin, ib + 64 * n1);
+        const double xa = ld_real<F32>(p.fin, ia + 64 * n1) * ld_real<F32>(p.fin2, ia + 64 * n1);
+        const double xb = ld_real<F32>(p.fin, ib + 64 * n1) * ld_real<F32>(p.fin2, ib + 64 * n1);
         v[n1] = make_double2(xa, f1v ? xb : 0.0);
       }
     }

@@ -156,6 +156,8 @@ struct FFTParams {
   double* F;          // bins: FA (analysis; float elements when fa32) or F
   int fa32;           // analysis: FA in FP32 (split-BF16 modes)
   const void* fin;    // analysis input field
+  const void* fin2;   // analysis of a pointwise product: the second factor
+                      // (same layout as fin), multiplied in as the rings load; or null
   void* fout;         // synthesis output field
   long fstride;       // elements between fields
   int f32;            // fields stored in FP32 (float / complex float)

@@ -32,6 +32,9 @@
  *   gsh_timing_enable /    no reference counterpart (the reference has no
  *   gsh_timing_read        timing code): HIP events around each kernel
  *                          launch, on the launch stream, for benchmarks
+ *   gsh_forward_product    ForwardTransformation of the product field of
+ *                          CanonicalComponentFieldBinary.h:203-253
+ *                          (CanonicalComponentFieldOverloads.h:207-233)
  *   gsh_wigner_table       Wigner<...>(lMax, mMax, nMax, theta) table build,
  *   (_host, _size,         Wigner.h:99-286 (ComputeAll :305-314, offsets
  *    _offset)              :116-132, 280-286), computed on the GPU
@@ -125,6 +128,13 @@ int gsh_inverse(gsh_grid_t grid, int lmax, int n, int value_kind, int precision,
 int gsh_forward(gsh_grid_t grid, int lmax, int n, int value_kind, int precision,
                 int storage, int batch, const void* field, long field_stride,
                 void* coeffs, long coeff_stride, int accumulate, void* stream);
+/* Analysis of the pointwise product field_a * field_b (coeffs (+)= A_n(a b)),
+ * the product formed as the ring FFT loads its rings; n is the product's
+ * upper index N0 + N1 (CanonicalComponentFieldBinary.h:203-253).  FP64
+ * storage; lmax >= 1. */
+int gsh_forward_product(gsh_grid_t grid, int lmax, int n, int value_kind, int precision,
+                        int batch, const void* field_a, const void* field_b, long field_stride,
+                        void* coeffs, long coeff_stride, int accumulate, void* stream);
 /* Adjoint of the synthesis: coeffs = S_n^H(field) (no quadrature weights). */
 int gsh_adjoint_inverse(gsh_grid_t grid, int lmax, int n, int value_kind,
                         int precision, int storage, int batch, const void* field,
