@@ -39,7 +39,8 @@ EXPORTS = [
     "gsh_timing_enable", "gsh_timing_read", "gsh_plan_create", "gsh_plan_destroy",
     "gsh_plan_inverse", "gsh_plan_forward", "gsh_plan_info", "gsh_device_alloc",
     "gsh_device_free", "gsh_copy", "gsh_synchronize", "gsh_wigner_size", "gsh_wigner_offset",
-    "gsh_wigner_table", "gsh_wigner_table_host", "gsh_forward_product",
+    "gsh_wigner_table", "gsh_wigner_table_host", "gsh_forward_product", "gsh_integrate",
+    "gsh_random_coefficients",
 ]
 KERNELS = ["legendre_synthesis", "fft_synthesis", "fft_analysis", "legendre_analysis",
            "coef_split", "bin_split"]
@@ -90,6 +91,8 @@ def _lib():
         L.gsh_timing_enable.argtypes = [vp, i]
         L.gsh_timing_read.argtypes = [vp, dp, ctypes.POINTER(l), i]
         L.gsh_forward_product.argtypes = [vp, i, i, i, i, i, vp, vp, l, vp, l, i, vp]
+        L.gsh_integrate.argtypes = [vp, i, i, vp, vp, l, vp, vp]
+        L.gsh_random_coefficients.argtypes = [vp, i, i, i, ctypes.c_ulonglong, d, i, vp, l, vp]
         L.gsh_wigner_size.restype = l
         L.gsh_wigner_size.argtypes = [i, i, i, i, i, i]
         L.gsh_wigner_offset.restype = l
@@ -101,7 +104,7 @@ def _lib():
                      "gsh_inverse", "gsh_forward", "gsh_adjoint_inverse",
                      "gsh_inverse_host", "gsh_forward_host", "gsh_timing_enable",
                      "gsh_timing_read", "gsh_wigner_table", "gsh_wigner_table_host",
-                     "gsh_forward_product"):
+                     "gsh_forward_product", "gsh_integrate", "gsh_random_coefficients"):
             getattr(L, name).restype = i
         _LIB = L
     return _LIB
@@ -454,6 +457,52 @@ class GaussLegendreGrid:
                                           1 if accumulate else 0, _stream(stream)),
                "gsh_forward_product")
         return coeffs if field_a.dim() == 3 else coeffs[0]
+
+    def weights(self, device="cuda"):
+        """GridBase::Weights (GridBase.h:60-66) as a (ntheta, nphi) tensor:
+        w_i dphi at every point."""
+        import torch
+        w = torch.from_numpy(self.CoLatitudeWeights() * (2 * np.pi / self.NumberOfLongitudes()))
+        return w.to(device)[:, None].expand(self.NumberOfCoLatitudes(),
+                                            self.NumberOfLongitudes()).contiguous()
+
+    def project(self, f, device="cuda", dtype=None):
+        """GridBase::ProjectFunction (GridBase.h:68-73): f(theta, phi) at every
+        point, for f acting elementwise on (ntheta, nphi) tensors."""
+        import torch
+        th = torch.from_numpy(np.asarray(self.CoLatitudes())).to(device)
+        ph = torch.arange(self.NumberOfLongitudes(), dtype=torch.float64,
+                          device=device) * (2 * np.pi / self.NumberOfLongitudes())
+        out = f(th[:, None].expand(-1, ph.numel()), ph[None, :].expand(th.numel(), -1))
+        return out.to(dtype) if dtype is not None else out
+
+    def integrate(self, field, other=None, stream=None):
+        """Integrate(u) (ScalarFieldOverloads.h:288-301) of a field or batch,
+        or with `other` the L2 inner product Integrate(conj(u) v) (:303-316),
+        on the device; returns complex128 (real fields: imaginary part 0)."""
+        import torch
+        fa = field if field.dim() == 3 else field.unsqueeze(0)
+        fb = None if other is None else (other if other.dim() == 3 else other.unsqueeze(0))
+        assert fb is None or (fb.shape == fa.shape and fb.dtype == fa.dtype)
+        real = not fa.is_complex()
+        out = torch.zeros(fa.shape[0], dtype=torch.complex128, device=fa.device)
+        _check(_lib().gsh_integrate(self._h, REAL if real else COMPLEX, fa.shape[0], _ptr(fa),
+                                    None if fb is None else _ptr(fb), self.FieldSize(),
+                                    _ptr(out), _stream(stream)), "gsh_integrate")
+        return out if field.dim() == 3 else out[0]
+
+    def random_coefficients(self, lmax, n, real=False, seed=0, decay=1.0, batch=None,
+                            stream=None):
+        """The coefficients of gshtrans_amd.random_coefficients(lmax, n, real,
+        seed, decay, grid_lmax=L, batch) generated on the device."""
+        import torch
+        C = int(_lib().gsh_coefficient_size(lmax, n, REAL if real else COMPLEX))
+        out = torch.empty((1 if batch is None else batch, C), dtype=torch.complex128,
+                          device="cuda")
+        _check(_lib().gsh_random_coefficients(self._h, lmax, n, REAL if real else COMPLEX,
+                                              seed, float(decay), out.shape[0], _ptr(out), C,
+                                              _stream(stream)), "gsh_random_coefficients")
+        return out[0] if batch is None else out
 
     def product(self, lmax, u, n0, v, n1, real=False, precision="fp64", stream=None):
         """Coefficients (upper index n0 + n1) of the product of the fields of

@@ -11,7 +11,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgshtrans_hip.so")
 
 SOURCES = ["gsh_capi.cpp", "gsh_legendre.hip", "gsh_fft.hip", "gsh_fft_wave.hip", "gsh_mixed.hip",
-           "gsh_batch.hip", "gsh_wigner.hip"]
+           "gsh_batch.hip", "gsh_wigner.hip",
+           "gsh_grid_ops.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -1048,6 +1048,68 @@ int gsh_forward(gsh_grid_t g, int lmax, int n, int value_kind, int precision, in
                       coeffs, coeff_stride, accumulate, stream, 0);
 }
 
+// Integrate (ScalarFieldOverloads.h:288-301) and the L2 inner product
+// (:303-316) with the grid's weights (GridBase.h:60-66): ring sums on the
+// device, then each field's sum over the colatitudes in order.
+int gsh_integrate(gsh_grid_t g, int value_kind, int batch, const void* field_a,
+                  const void* field_b, long field_stride, double* out, void* stream) {
+  if (!g || batch < 0 || (value_kind != GSH_REAL && value_kind != GSH_COMPLEX))
+    return GSH_ERR_INVALID_ARGUMENT;
+  if (value_kind == GSH_COMPLEX && g->mrange != GSH_ALL) return GSH_ERR_VALUE_KIND;
+  if (batch == 0) return GSH_OK;
+  const long fsize = static_cast<long>(g->ntheta()) * g->nphi();
+  if (!field_a || !out || (batch > 1 && field_stride < fsize)) return GSH_ERR_INVALID_ARGUMENT;
+  if (g->L == 0) return GSH_ERR_UNSUPPORTED;  // no longitudes
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> lock(g->mu);
+  GSH_HIP(hipSetDevice(g->device));
+  int st;
+  if ((st = ensure_work(g, sizeof(double2) * (size_t)batch * g->ntheta()))) return st;
+  gsh::GridOpParams p{};
+  p.ntheta = g->ntheta();
+  p.nphi = g->nphi();
+  p.batch = batch;
+  p.real = value_kind == GSH_REAL;
+  p.fstride = batch > 1 ? field_stride : fsize;
+  p.fa = field_a;
+  p.fb = field_b;
+  p.wq = g->d_wq;  // w_i 2 pi / nphi: the colatitude weight times the longitude weight
+  p.dphi = 1.0;
+  p.part = reinterpret_cast<double2*>(g->work);
+  p.out = out;
+  GSH_HIP(gsh::launch_integrate(p, s));
+  return GSH_OK;
+}
+
+// Seeded coefficients generated on the device: the counter-based stream of
+// gshtrans_amd.random_coefficients (the reference's generator, GridBase.h:
+// 115-172, draws from std::random_device), with its masks for the grid's L.
+int gsh_random_coefficients(gsh_grid_t g, int lmax, int n, int value_kind,
+                            unsigned long long seed, double decay, int batch, void* coeffs,
+                            long coeff_stride, void* stream) {
+  int st = check_call(g, lmax, n, value_kind, GSH_FP64, GSH_STORE_FP64, batch);
+  if (st) return st;
+  if (batch == 0) return GSH_OK;
+  const long csize = gsh_coefficient_size(lmax, n, value_kind);
+  if (!coeffs || (batch > 1 && coeff_stride < csize)) return GSH_ERR_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> lock(g->mu);
+  GSH_HIP(hipSetDevice(g->device));
+  gsh::RandomParams p{};
+  p.lmax = lmax;
+  p.grid_lmax = g->L;
+  p.nabs = std::abs(n);
+  p.real = value_kind == GSH_REAL;
+  p.batch = batch;
+  p.size = csize;
+  p.cstride = batch > 1 ? coeff_stride : csize;
+  p.seed = seed;
+  p.decay = decay;
+  p.out = static_cast<double*>(coeffs);
+  GSH_HIP(gsh::launch_random_coeffs(p, s));
+  return GSH_OK;
+}
+
 // Analysis of the pointwise product of two fields (the product field of
 // CanonicalComponentFieldBinary.h:203-253 and CanonicalComponentFieldOverloads.h:
 // 207-233, upper index n = N0 + N1), the product formed as the ring FFT loads

@@ -206,6 +206,27 @@ struct WignerParams {
 };
 hipError_t launch_wigner_table(const WignerParams& p, hipStream_t s);
 
+// Grid helpers (gsh_grid_ops.hip).
+struct GridOpParams {
+  int ntheta, nphi, batch, real;
+  long fstride;             // elements between fields
+  const void* fa;           // fields (double or complex double)
+  const void* fb;           // second factor of the inner product, or null
+  const double* wq;         // quadrature weights w_i
+  double dphi;              // longitude weight 2 pi / nphi
+  double2* part;            // ring sums [f * ntheta + i]
+  double* out;              // complex results, interleaved
+};
+hipError_t launch_integrate(const GridOpParams& p, hipStream_t s);
+struct RandomParams {
+  int lmax, grid_lmax, nabs, real, batch;
+  long size, cstride;       // coefficients per field, complex elements between fields
+  unsigned long long seed;
+  double decay;
+  double* out;
+};
+hipError_t launch_random_coeffs(const RandomParams& p, hipStream_t s);
+
 // lmax == 0: mode 1 synthesis, 0 analysis, 2 adjoint of the synthesis.
 hipError_t launch_lmax0(int mode, int real, int nfield, const double* in, long istride,
                         double* out, long ostride, hipStream_t s);

@@ -32,6 +32,10 @@
  *   gsh_timing_enable /    no reference counterpart (the reference has no
  *   gsh_timing_read        timing code): HIP events around each kernel
  *                          launch, on the launch stream, for benchmarks
+ *   gsh_integrate          Integrate / L2InnerProduct, ScalarFieldOverloads.h:
+ *                          288-316, with GridBase::Weights (GridBase.h:60-66)
+ *   gsh_random_coefficients GridBase::Random*Coefficient (GridBase.h:115-172),
+ *                          seeded, on the device
  *   gsh_forward_product    ForwardTransformation of the product field of
  *                          CanonicalComponentFieldBinary.h:203-253
  *                          (CanonicalComponentFieldOverloads.h:207-233)
@@ -135,6 +139,18 @@ int gsh_forward(gsh_grid_t grid, int lmax, int n, int value_kind, int precision,
 int gsh_forward_product(gsh_grid_t grid, int lmax, int n, int value_kind, int precision,
                         int batch, const void* field_a, const void* field_b, long field_stride,
                         void* coeffs, long coeff_stride, int accumulate, void* stream);
+/* Integrate (ScalarFieldOverloads.h:288-301) of `batch` fields, or with
+ * field_b the L2 inner product Integrate(conj(a) b) (:303-316), with the
+ * grid's quadrature weights w_i 2 pi / nphi (GridBase.h:60-66): out[f] is
+ * complex (interleaved; imaginary part 0 for real fields), device memory. */
+int gsh_integrate(gsh_grid_t grid, int value_kind, int batch, const void* field_a,
+                  const void* field_b, long field_stride, double* out, void* stream);
+/* Seeded coefficients generated on the device (device memory): the stream
+ * of gshtrans_amd.random_coefficients, scaled by (l+1)^-decay, with the
+ * masks of GridBase.h:115-172 for the grid's lMax. */
+int gsh_random_coefficients(gsh_grid_t grid, int lmax, int n, int value_kind,
+                            unsigned long long seed, double decay, int batch, void* coeffs,
+                            long coeff_stride, void* stream);
 /* Adjoint of the synthesis: coeffs = S_n^H(field) (no quadrature weights). */
 int gsh_adjoint_inverse(gsh_grid_t grid, int lmax, int n, int value_kind,
                         int precision, int storage, int batch, const void* field,
