@@ -112,8 +112,9 @@ __device__ __forceinline__ void wave_sync() {
 
 // Transform the ring held as v[n1] = x[64 n1 + lane]; on return v[t] =
 // X[lane + 64 t] = sum_n x[n] exp(i SIGN 2 pi n (lane + 64 t) / N).
-// twa[k1 * 64 + lane] = exp(-2 pi i lane k1 / N); tw[j] = exp(-2 pi i j / N).
-template <int R, int SIGN>
+// twa[k1 * 64 + lane] = exp(-2 pi i lane k1 / (TS N)); tw[j] = exp(-2 pi i
+// j / (TS N)): the tables of a TS times longer ring, read at stride TS.
+template <int R, int SIGN, int TS = 1>
 __device__ __forceinline__ void wave_fft(double2 (&v)[R], double2* lds,
                                          const double2* __restrict__ twa,
                                          const double2* __restrict__ tw, int lane) {
@@ -126,7 +127,7 @@ __device__ __forceinline__ void wave_fft(double2 (&v)[R], double2* lds,
   dft_reg<R, SIGN>(v);
 #pragma unroll
   for (int k1 = 1; k1 < R; k1++) {
-    double2 w = twa[k1 * 64 + lane];
+    double2 w = twa[(TS * k1) * 64 + lane];
     v[bitrev(k1, LR)] = SIGN < 0 ? cmul(v[bitrev(k1, LR)], w) : cmulc(v[bitrev(k1, LR)], w);
   }
 
@@ -153,7 +154,7 @@ __device__ __forceinline__ void wave_fft(double2 (&v)[R], double2* lds,
   for (int g = 0; g < G; g++) dft_reg<8, SIGN>(u[g]);
 #pragma unroll
   for (int c = 1; c < 8; c++) {
-    const double2 w = tw[b * c * R];  // exp(-2 pi i b c / 64)
+    const double2 w = tw[b * c * R * TS];  // exp(-2 pi i b c / 64)
 #pragma unroll
     for (int g = 0; g < G; g++)
       u[g][bitrev(c, 3)] = SIGN < 0 ? cmul(u[g][bitrev(c, 3)], w) : cmulc(u[g][bitrev(c, 3)], w);
@@ -222,6 +223,16 @@ __device__ __forceinline__ void st_real(void* b, long i, double v) {
     static_cast<float*>(b)[i] = (float)v;
   else
     static_cast<double*>(b)[i] = v;
+}
+
+// Two consecutive real samples (elements i, i + 1) of a real field.
+template <int F32>
+__device__ __forceinline__ double2 ld_pair(const void* b, long i) {
+  if (F32) {
+    const float2 v = *reinterpret_cast<const float2*>(static_cast<const float*>(b) + i);
+    return make_double2(v.x, v.y);
+  }
+  return *reinterpret_cast<const double2*>(static_cast<const double*>(b) + i);
 }
 
 __device__ __forceinline__ double2 ld2(const double* p) {
@@ -445,6 +456,45 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
   const bool f1v = REAL && f0 + 1 < p.nfield;
   const int lmax = p.lmax, m_first = p.m_first;
   double2 v[R];
+  double2* lw = lds_all + wave * LPW;
+  if constexpr (REAL && R >= 16) {
+    // Real pairs: each lane loads the even / odd samples x[2m], x[2m + 1]
+    // (m = 64 a + lane) of both rings with one 16-byte access per ring, the
+    // two half-length rings are transformed, and X[k] = E[k] + W_N^k O[k],
+    // X[k + N/2] = E[k] - W_N^k O[k] (k = lane + 64 t < N/2).
+    constexpr int R2 = R / 2;
+    double2 e[R2], o[R2];
+    if (fv) {
+      const long ia = (long)field_index(p.fmap, f0) * p.fstride + (long)theta * N + 2 * lane;
+      const long ib =
+          f1v ? (long)field_index(p.fmap, f0 + 1) * p.fstride + (long)theta * N + 2 * lane : ia;
+#pragma unroll
+      for (int a = 0; a < R2; a++) {
+        double2 pa = ld_pair<F32>(p.fin, ia + 128 * a);
+        double2 pb = ld_pair<F32>(p.fin, ib + 128 * a);
+        if (p.fin2) {  // pointwise product of two fields
+          const double2 qa = ld_pair<F32>(p.fin2, ia + 128 * a);
+          const double2 qb = ld_pair<F32>(p.fin2, ib + 128 * a);
+          pa = make_double2(pa.x * qa.x, pa.y * qa.y);
+          pb = make_double2(pb.x * qb.x, pb.y * qb.y);
+        }
+        if (!f1v) pb = make_double2(0.0, 0.0);
+        e[a] = make_double2(pa.x, pb.x);
+        o[a] = make_double2(pa.y, pb.y);
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < R2; a++) e[a] = o[a] = make_double2(0.0, 0.0);
+    }
+    wave_fft<R2, -1, 2>(e, lw, p.twa, p.tw, lane);
+    wave_fft<R2, -1, 2>(o, lw, p.twa, p.tw, lane);
+#pragma unroll
+    for (int t = 0; t < R2; t++) {
+      const double2 wo = cmul(o[t], p.tw[lane + 64 * t]);
+      v[t] = cadd(e[t], wo);
+      v[t + R2] = csub(e[t], wo);
+    }
+  } else {
   if (fv) {
     const long ia = (long)field_index(p.fmap, f0) * p.fstride + (long)theta * N + lane;
     const long ib =
@@ -477,8 +527,8 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
 #pragma unroll
     for (int n1 = 0; n1 < R; n1++) v[n1] = make_double2(0.0, 0.0);
   }
-  double2* lw = lds_all + wave * LPW;
   wave_fft<R, -1>(v, lw, p.twa, p.tw, lane);
+  }
   const double w = p.wq[theta];
   const int n_cg = p.n_cg, nquad = p.nquad;
   // This thread's share of the cooperative scatter: ring c, bins 64 i + bq.

@@ -430,3 +430,41 @@ def test_analysis_bin_scatter(R, real, part, nf):
     assert FA.keys() == expect.keys()
     for key, val in expect.items():
         assert FA[key] == val, key
+
+
+@pytest.mark.parametrize("R", [16, 32])
+def test_real_pair_radix2_split(R):
+    """fft_anal_w for real pairs at R >= 16: lane loads the even / odd samples
+    x[2m], x[2m + 1] (m = 64 a + lane) of both rings, runs two half-length
+    wavefront FFTs (twiddles of the length-N tables at stride 2, which are the
+    half-length ones), and combines X[k] = E[k] + W_N^k O[k],
+    X[k + N/2] = E[k] - W_N^k O[k] into v[t] = X[lane + 64 t]."""
+    N = 64 * R
+    rng = np.random.default_rng(300 + R)
+    xa, xb = rng.standard_normal(N), rng.standard_normal(N)
+    z = xa + 1j * xb
+    lane = np.arange(64)
+    R2 = R // 2
+    # Loads: pair a of the lane holds samples 128 a + 2 lane (+1) of each ring.
+    ein = np.zeros(64 * R2, complex)
+    oin = np.zeros(64 * R2, complex)
+    for a in range(R2):
+        idx = 128 * a + 2 * lane
+        ein[64 * a + lane] = xa[idx] + 1j * xb[idx]
+        oin[64 * a + lane] = xa[idx + 1] + 1j * xb[idx + 1]
+    E = wave_fft(ein, R2, -1)
+    O = wave_fft(oin, R2, -1)
+    tw = np.exp(-2j * np.pi * np.arange(N) / N)
+    v = np.zeros((R, 64), complex)
+    for t in range(R2):
+        k = lane + 64 * t
+        wo = O[k] * tw[k]
+        v[t] = E[k] + wo
+        v[t + R2] = E[k] - wo
+    X = np.fft.fft(z)
+    for t in range(R):
+        assert np.max(np.abs(v[t] - X[lane + 64 * t])) < 1e-12 * np.sqrt(N)
+    # Twiddle-table indices at stride 2 stay inside the length-N tables.
+    for k1 in range(1, R2):
+        assert 2 * k1 < R
+    assert 7 * 7 * R2 * 2 < N
