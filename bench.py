@@ -324,7 +324,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
-              "(the transform batch shards over GPUs without exchange; see DESIGN.md)",
+              "(the transform batch shards over GPUs without exchange: one GPU's rate is its share)",
               file=sys.stderr)
         sys.exit(2)
     results, headline, mixed, errors = run_gpu(args)
