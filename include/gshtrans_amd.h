@@ -58,6 +58,10 @@
  * the array's own type (complex elements for coefficients and complex
  * fields, doubles or floats for real fields).
  *
+ * Streams: the work of a call is queued on its stream.  Calls on one grid
+ * handle share the handle's work buffers; calls on different streams must be
+ * ordered by the caller (or use separate handles).
+ *
  * Errors: every entry returns a gsh_status; the reference only asserts
  * (GaussLegendreGrid.h:41-44, 126-134, 240-248), which the C++ facade
  * restores by asserting on non-zero status.
