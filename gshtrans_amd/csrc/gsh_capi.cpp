@@ -866,8 +866,13 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
       return gsh::launch_leg_synth_f64(p, s, T, precision == GSH_FP32);
     }));
   } else {
+    // Up to kSynthTailRows colatitudes past whole super-blocks go to the tail
+    // kernel rather than to items of their own.
+    const int nt = g->ntheta();
+    const bool tail = nt > 256 && nt % 256 != 0 && nt % 256 <= gsh::kSynthTailRows;
     WorkList w;
-    if ((st = ensure_synth_worklist(g, 20, lmax, real, ((p.ncol + 15) / 16 + 7) / 8, ntsb, &w)))
+    if ((st = ensure_synth_worklist(g, 20, lmax, real, ((p.ncol + 15) / 16 + 7) / 8,
+                                    tail ? nt / 256 : ntsb, &w)))
       return st;
     p.items = w.d;
     p.items_per = w.per;
@@ -884,9 +889,23 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
     p.ntile = ((p.ncol + 15) / 16 + 7) / 8 * 8;
     p.cso = cso;
     const long nblocks = nchunks * p.ntile;
-    if ((st = ensure_work2(g, (size_t)nblocks * ns * 64 * 16))) return st;
+    // The tail rows' d values (32 per chunk and row) and partial sums (16 per
+    // block and row) follow the split coefficients in the same buffer.
+    const int trows = tail ? nt % 256 : 0;
+    const size_t bs_bytes = (size_t)nblocks * ns * 64 * 16;
+    const size_t dv_bytes = (size_t)trows * nchunks * 32 * sizeof(double);
+    const size_t pt_bytes = (size_t)trows * nblocks * 16 * sizeof(double);
+    if ((st = ensure_work2(g, bs_bytes + dv_bytes + pt_bytes))) return st;
     p.bs = static_cast<const uint4*>(g->work2);
     p.bs_out = static_cast<uint4*>(g->work2);
+    if (trows) {
+      char* w2 = static_cast<char*>(g->work2);
+      p.tail_rows = trows;
+      p.tail_dv = reinterpret_cast<const double*>(w2 + bs_bytes);
+      p.tail_dstride = nchunks * 32;
+      p.tail_part = reinterpret_cast<double*>(w2 + bs_bytes + dv_bytes);
+      p.tail_pstride = nblocks * 16;
+    }
     {
       WorkList ws;
       if ((st = ensure_split_worklist(g, lmax, n, real, p.ntile, &ws))) return st;
@@ -894,11 +913,17 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
       p.items_per = ws.per;
     }
     GSH_HIP(timed(g, GSH_KERNEL_COEF_SPLIT, s,
-                  [&] { return gsh::launch_coef_split(p, ns, s); }));
+                  [&] {
+                    hipError_t e = trows ? gsh::launch_tail_dvals(p, s) : hipSuccess;
+                    return e == hipSuccess ? gsh::launch_coef_split(p, ns, s) : e;
+                  }));
     p.items = w.d;  // the synthesis work list again
     p.items_per = w.per;
-    GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_SYNTHESIS, s,
-                  [&] { return gsh::launch_leg_synth_bf16(p, ns, s); }));
+    GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_SYNTHESIS, s, [&] {
+      hipError_t e = gsh::launch_leg_synth_bf16(p, ns, s);
+      if (e == hipSuccess && trows) e = gsh::launch_tail_reduce(p, s);
+      return e;
+    }));
   }
   FFTParams q = fft_params(g, lmax, real, batch);
   q.fout = field;

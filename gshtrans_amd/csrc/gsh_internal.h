@@ -11,6 +11,9 @@ namespace gsh {
 // Zero recurrence entries kept after degree L for every order, so that
 // fixed-size chunks of the l-loop may step past L without bounds checks.
 constexpr int kRecPad = 64;
+// Colatitudes past the last whole 256-row super-block that the split-BF16
+// synthesis leaves to its tail kernel (more get a super-block item).
+constexpr int kSynthTailRows = 2;
 
 // Analysis passes: the degree range of an order is cut into passes of
 // kPass degrees that run in separate workgroups; the recurrence state at the
@@ -92,6 +95,15 @@ struct LegParams {
   // every l0 + kCk k <= L.
   const double2* ck;
   const long* ckoff;
+  // Split-BF16 synthesis of the last tail_rows colatitudes (0: none), from
+  // the d values tail_dv[r * tail_dstride + 32 (cso[slot] + chunk) + j] of
+  // degree l0 + 32 chunk + j and the per-block partial sums
+  // tail_part[r * tail_pstride + 16 block + column] that coef_split adds.
+  const double* tail_dv;
+  double* tail_part;
+  long tail_dstride;
+  long tail_pstride;
+  int tail_rows;
 };
 
 
@@ -182,6 +194,11 @@ hipError_t launch_fft_anal_wave(const FFTParams& p, hipStream_t s);
 // Mixed precision kernels (gsh_mixed.hip); ns = 2 (bf16x3) or 3 (bf16x6).
 hipError_t launch_coef_split(const LegParams& p, int ns, hipStream_t s);
 hipError_t launch_leg_synth_bf16(const LegParams& p, int ns, hipStream_t s);
+// Split-BF16 synthesis of the last p.tail_rows colatitudes (at most
+// kSynthTailRows): their d values before coef_split, which then adds the
+// partial sums, and the sums over chunks into F after it.
+hipError_t launch_tail_dvals(const LegParams& p, hipStream_t s);
+hipError_t launch_tail_reduce(const LegParams& p, hipStream_t s);
 hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s);
 
 // FP32 <-> FP64 coefficient staging of batch plans (gsh_batch.hip); strides

@@ -182,7 +182,8 @@ __global__ __launch_bounds__(512) void coef_split(LegParams p) {
   const int m = sl + p.m_first;
   const int l0 = max(abs(m), p.nabs);
   if (chunk >= (p.lmax - l0 + 32) / 32) return;  // no such block for this order
-  const long b = (ld_uniform(p.cso, sl) + chunk) * p.ntile + tile;  // output block
+  const long bc = ld_uniform(p.cso, sl) + chunk;  // chunk in the split layout
+  const long b = bc * p.ntile + tile;              // output block
   const int kr = lane >> 4, c = lane & 15;
   const int lo = l0 + 32 * chunk + 8 * kr - lw0;  // window offset of e = 0
   unsigned planes[NS][4];
@@ -199,6 +200,23 @@ __global__ __launch_bounds__(512) void coef_split(LegParams p) {
   for (int s = 0; s < NS; s++)
     p.bs_out[(b * NS + s) * 64 + lane] =
         make_uint4(planes[s][0], planes[s][1], planes[s][2], planes[s][3]);
+  // Tail colatitudes: this block's share of their sums, from the FP64
+  // coefficients, one partial per (row, block, column); tail_reduce adds the
+  // chunks up.
+  for (int r = 0; r < p.tail_rows; r++) {
+    const double2* dq =
+        reinterpret_cast<const double2*>(p.tail_dv + r * p.tail_dstride + bc * 32 + 8 * kr);
+    double acc = 0.0;
+#pragma unroll
+    for (int e2 = 0; e2 < 4; e2++) {
+      const double2 d = dq[e2];
+      acc = fma(d.x, win[(lo + 2 * e2) * kSplitRow + 16 * wave + c], acc);
+      acc = fma(d.y, win[(lo + 2 * e2 + 1) * kSplitRow + 16 * wave + c], acc);
+    }
+    acc += __shfl_xor(acc, 16);
+    acc += __shfl_xor(acc, 32);
+    if (lane < 16) p.tail_part[r * p.tail_pstride + b * 16 + c] = acc;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -663,6 +681,66 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Synthesis of the last colatitudes t0 .. ntheta - 1 (t0 = ntheta -
+// tail_rows) when ntheta is only a row or two past a multiple of 256.  A
+// leg_synth_bf16 item holding one row would still walk every degree chunk,
+// bound by the latency of its B operand loads while it occupies a whole CU.
+// Instead coef_split, which holds every FP64 coefficient in LDS anyway, adds
+// each block's partial sums for these rows, and tail_reduce sums them.
+//
+// tail_dvals: the FP64 values d_l(theta_t) of row t0 + blockIdx.y for every
+// order and degree of the split layout.  Lane s of the wave of a slot runs
+// the 16-degree segment s from checkpoint s (the seed for s = 0; segments
+// that start past L meet only zero coefficients and are written as zeros).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void tail_dvals(LegParams p, double* dv) {
+  const int slot = blockIdx.x;
+  const int t = p.ntheta - p.tail_rows + blockIdx.y;
+  const int m = slot + p.m_first;
+  const int l0 = max(abs(m), p.nabs);
+  const int nchunk = (p.lmax - l0 + 32) >> 5;
+  const int nck = (p.L - l0 + kCk) / kCk;  // checkpoints 1 .. nck-1 exist
+  const double cs = p.cosv[t];
+  const d4* rl = reinterpret_cast<const d4*>(p.rec + (ld_uniform(p.roff, m + p.L) - l0));
+  const double2* ckm = p.ck + ld_uniform(p.ckoff, m + p.L) * (long)p.ntheta + t;
+  double* out = dv + blockIdx.y * p.tail_dstride + ld_uniform(p.cso, slot) * 32;
+  for (int sg = threadIdx.x; sg < 2 * nchunk; sg += 64) {
+    double e0 = 0.0, e1 = 0.0;
+    if (sg == 0) {
+      e0 = p.seeds[(long)(m + p.L) * p.ntheta + t];
+    } else if (sg < nck) {
+      const double2 st = ckm[(long)(sg - 1) * p.ntheta];
+      e0 = st.x;
+      e1 = st.y;
+    }
+    const int ls = l0 + kCk * sg;  // entries up to ls + 15 <= lmax + 31 <= L + kRecPad
+#pragma unroll
+    for (int j = 0; j < kCk; j++) {
+      out[kCk * sg + j] = e0;
+      if (j + 1 < kCk) rec_step(rl[ls + j + 1], cs, e0, e1);
+    }
+  }
+}
+
+// tail_reduce: row t0 + blockIdx.y of order slot blockIdx.x, column by
+// thread, summed over the chunks in order.
+__global__ __launch_bounds__(256) void tail_reduce(LegParams p) {
+  const int slot = blockIdx.x;
+  const int t = p.ntheta - p.tail_rows + blockIdx.y;
+  const int m = slot + p.m_first;
+  const int l0 = max(abs(m), p.nabs);
+  const int nchunk = (p.lmax - l0 + 32) >> 5;
+  const long cstride = (long)p.ntile * 16;  // doubles per chunk
+  const double* pp = p.tail_part + blockIdx.y * p.tail_pstride + ld_uniform(p.cso, slot) * cstride;
+  const int ncg16 = (p.ncol + 15) >> 4;
+  for (int col = threadIdx.x; col < p.ncol; col += 256) {
+    double sum = 0.0;
+    for (int c = 0; c < nchunk; c++) sum += pp[c * cstride + col];
+    p.F[f_index(t, slot, col, p.nslot, ncg16)] = sum;
+  }
+}
+
 }  // namespace
 
 hipError_t launch_coef_split(const LegParams& p, int ns, hipStream_t s) {
@@ -692,6 +770,20 @@ hipError_t launch_leg_synth_bf16(const LegParams& p0, int ns, hipStream_t s) {
     hipLaunchKernelGGL((leg_synth_bf16<2, 1>), grid, dim3(512), 0, s, p);
   else
     hipLaunchKernelGGL((leg_synth_bf16<2, 0>), grid, dim3(512), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_tail_dvals(const LegParams& p, hipStream_t s) {
+  if (p.tail_rows < 1 || p.tail_rows > kSynthTailRows || !p.tail_dv || !p.ck || !p.ckoff)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(tail_dvals, dim3(p.nslot, p.tail_rows), dim3(64), 0, s, p,
+                     const_cast<double*>(p.tail_dv));
+  return hipGetLastError();
+}
+
+hipError_t launch_tail_reduce(const LegParams& p, hipStream_t s) {
+  if (p.tail_rows < 1 || p.tail_rows > kSynthTailRows || !p.tail_part) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(tail_reduce, dim3(p.nslot, p.tail_rows), dim3(256), 0, s, p);
   return hipGetLastError();
 }
 

@@ -61,6 +61,37 @@ def test_mixed_synthesis_error(torch, prec, L, n, real, batch):
     assert e.min() > 0 or prec == "fp64"
 
 
+@pytest.mark.parametrize("prec", ["bf16x6", "bf16x3"])
+@pytest.mark.parametrize("L,n,real,batch", [(256, 1, False, 3), (257, 0, True, 5),
+                                            (512, 2, False, 4), (1024, 0, True, 6)])
+def test_mixed_synthesis_last_rows(torch, prec, L, n, real, batch):
+    """With ntheta = L + 1 one or two rows past a multiple of 256 (257, 258,
+    513, 1025), the split-BF16 synthesis takes those last colatitudes out of
+    its MFMA items: the coefficient split adds their partial sums per block
+    from the FP64 coefficients and the FP64 recurrence values, and a last
+    kernel sums the blocks.  The whole field meets the mode's bar, and those
+    rows equal the FP64 transform to 1e-12 in either mode (a wrong degree,
+    order, column or block shows far above that)."""
+    say(f"[mixed synthesis, last rows] {prec} L={L} n={n} real={real} batch={batch}")
+    g = grid(L)
+    c = torch.from_numpy(G.random_coefficients(L, n, real=real, seed=SEED0 + 7 * L + n,
+                                               batch=batch)).cuda()
+    f64 = g.inverse(L, n, c, real=real)
+    fmx = g.inverse(L, n, c, real=real, precision=prec)
+    e = per_field_rel(torch, fmx, f64)
+    say(f"    field: max {e.max():.2e}")
+    assert e.max() <= BARS[prec]
+    nt = L + 1
+    assert 1 <= nt % 256 <= 2
+    for t in range(nt // 256 * 256, nt):
+        e = per_field_rel(torch, fmx[:, t:t + 1], f64[:, t:t + 1])
+        say(f"    row {t}: max {e.max():.2e}")
+        assert e.max() <= 1e-12
+    # The row before them comes from the MFMA items.
+    t = nt // 256 * 256 - 1
+    assert per_field_rel(torch, fmx[:, t:t + 1], f64[:, t:t + 1]).max() > 1e-12
+
+
 @pytest.mark.parametrize("prec", ["bf16x6", "bf16x3", "fp32"])
 @pytest.mark.parametrize("L,n,real,batch", [(64, 0, False, 3), (64, 2, True, 5),
                                             (512, 2, False, 8), (300, -1, False, 2),
