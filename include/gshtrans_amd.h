@@ -195,6 +195,10 @@ int gsh_plan_info(gsh_plan_t plan, int* ngroups, int cap, int* n, int* lmax, int
                   int* synth_tiles, int* fft_inverse, int* fft_forward);
 
 /* ---- per-kernel timing ------------------------------------------------- */
+/* In the split-BF16 synthesis of a grid whose colatitude count is one or two
+ * past a multiple of 256, the LEGENDRE_SYNTHESIS pair also spans the kernel
+ * that sums those last rows, and the COEF_SPLIT pair the kernel that writes
+ * their recurrence values. */
 enum gsh_kernel {
   GSH_KERNEL_LEGENDRE_SYNTHESIS = 0,
   GSH_KERNEL_FFT_SYNTHESIS = 1,
