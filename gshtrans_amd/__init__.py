@@ -218,6 +218,39 @@ def _ptr(t):
     raise TypeError("torch tensor expected")
 
 
+def _check_coeffs(t, rows, size, dtype, what):
+    """Coefficient arrays: a (rows, stride) cuda tensor of `dtype` with stride
+    >= size, since the kernels read or write `size` complex entries per row."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{what}: coefficients must be a torch tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{what}: coefficients must be {dtype}, got {t.dtype}")
+    if t.dim() != 2 or t.shape[0] != rows or t.shape[1] < size:
+        raise ValueError(f"{what}: coefficients must be ({rows}, >= {size}), got "
+                         f"{tuple(t.shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what}: coefficients must be contiguous")
+    if not t.is_cuda:
+        raise ValueError(f"{what}: coefficients must be a cuda tensor")
+
+
+def _check_field(t, rows, ntheta, nphi, dtype, what):
+    """Field arrays: a contiguous (rows, ntheta, nphi) cuda tensor of `dtype`."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{what}: field must be a torch tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{what}: field must be {dtype}, got {t.dtype}")
+    if tuple(t.shape) != (rows, ntheta, nphi):
+        raise ValueError(f"{what}: field must be ({rows}, {ntheta}, {nphi}), got "
+                         f"{tuple(t.shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what}: field must be contiguous")
+    if not t.is_cuda:
+        raise ValueError(f"{what}: field must be a cuda tensor")
+
+
 def _stream(stream):
     import torch
     s = stream if stream is not None else torch.cuda.current_stream()
@@ -412,10 +445,12 @@ class GaussLegendreGrid:
         expect = _lib().gsh_coefficient_size(lmax, n, REAL if real else COMPLEX)
         if expect >= 0 and C != expect:
             raise ValueError("coefficient size mismatch")
+        _check_coeffs(c, B, C, torch.complex128, "inverse")
+        fdt = torch.float64 if real else torch.complex128
         if field is None:
-            field = torch.empty((B, self._lmax + 1, 2 * self._lmax),
-                                dtype=torch.float64 if real else torch.complex128,
-                                device=c.device)
+            field = torch.empty((B, self._lmax + 1, 2 * self._lmax), dtype=fdt, device=c.device)
+        _check_field(field if field.dim() == 3 else field.unsqueeze(0), B, self._lmax + 1,
+                     2 * self._lmax, fdt, "inverse")
         _check(_lib().gsh_inverse(self._h, lmax, n, REAL if real else COMPLEX,
                                   PRECISION[precision], STORAGE["fp64"], B, _ptr(c), C,
                                   _ptr(field), self.FieldSize(), _stream(stream)),
@@ -428,13 +463,17 @@ class GaussLegendreGrid:
         import torch
         f = field if field.dim() == 3 else field.unsqueeze(0)
         B = f.shape[0]
+        _check_field(f, B, self._lmax + 1, 2 * self._lmax,
+                     torch.float64 if real else torch.complex128, "forward")
         C = max(int(_lib().gsh_coefficient_size(lmax, n, REAL if real else COMPLEX)), 1)
         if coeffs is None:
             coeffs = torch.zeros((B, C), dtype=torch.complex128, device=f.device)
+        co = coeffs if coeffs.dim() == 2 else coeffs.unsqueeze(0)
+        _check_coeffs(co, B, C, torch.complex128, "forward")
         _check(_lib().gsh_forward(self._h, lmax, n, REAL if real else COMPLEX,
                                   PRECISION[precision], STORAGE["fp64"], B, _ptr(f),
-                                  self.FieldSize(), _ptr(coeffs), C, 1 if accumulate else 0,
-                                  _stream(stream)), "gsh_forward")
+                                  self.FieldSize(), _ptr(co), co.shape[1],
+                                  1 if accumulate else 0, _stream(stream)), "gsh_forward")
         return coeffs if field.dim() == 3 else coeffs[0]
 
     def forward_product(self, lmax, n, field_a, field_b, coeffs=None, real=False,
@@ -446,14 +485,18 @@ class GaussLegendreGrid:
         import torch
         fa = field_a if field_a.dim() == 3 else field_a.unsqueeze(0)
         fb = field_b if field_b.dim() == 3 else field_b.unsqueeze(0)
-        assert fa.shape == fb.shape and fa.dtype == fb.dtype
         B = fa.shape[0]
+        fdt = torch.float64 if real else torch.complex128
+        _check_field(fa, B, self._lmax + 1, 2 * self._lmax, fdt, "forward_product")
+        _check_field(fb, B, self._lmax + 1, 2 * self._lmax, fdt, "forward_product")
         C = max(int(_lib().gsh_coefficient_size(lmax, n, REAL if real else COMPLEX)), 1)
         if coeffs is None:
             coeffs = torch.zeros((B, C), dtype=torch.complex128, device=fa.device)
+        co = coeffs if coeffs.dim() == 2 else coeffs.unsqueeze(0)
+        _check_coeffs(co, B, C, torch.complex128, "forward_product")
         _check(_lib().gsh_forward_product(self._h, lmax, n, REAL if real else COMPLEX,
                                           PRECISION[precision], B, _ptr(fa), _ptr(fb),
-                                          self.FieldSize(), _ptr(coeffs), C,
+                                          self.FieldSize(), _ptr(co), co.shape[1],
                                           1 if accumulate else 0, _stream(stream)),
                "gsh_forward_product")
         return coeffs if field_a.dim() == 3 else coeffs[0]
@@ -483,8 +526,11 @@ class GaussLegendreGrid:
         import torch
         fa = field if field.dim() == 3 else field.unsqueeze(0)
         fb = None if other is None else (other if other.dim() == 3 else other.unsqueeze(0))
-        assert fb is None or (fb.shape == fa.shape and fb.dtype == fa.dtype)
         real = not fa.is_complex()
+        fdt = torch.float64 if real else torch.complex128
+        _check_field(fa, fa.shape[0], self._lmax + 1, 2 * self._lmax, fdt, "integrate")
+        if fb is not None:
+            _check_field(fb, fa.shape[0], self._lmax + 1, 2 * self._lmax, fdt, "integrate")
         out = torch.zeros(fa.shape[0], dtype=torch.complex128, device=fa.device)
         _check(_lib().gsh_integrate(self._h, REAL if real else COMPLEX, fa.shape[0], _ptr(fa),
                                     None if fb is None else _ptr(fb), self.FieldSize(),
@@ -521,12 +567,17 @@ class GaussLegendreGrid:
         import torch
         f = field if field.dim() == 3 else field.unsqueeze(0)
         B = f.shape[0]
+        _check_field(f, B, self._lmax + 1, 2 * self._lmax,
+                     torch.float64 if real else torch.complex128, "adjoint")
         C = max(int(_lib().gsh_coefficient_size(lmax, n, REAL if real else COMPLEX)), 1)
         if coeffs is None:
             coeffs = torch.empty((B, C), dtype=torch.complex128, device=f.device)
+        co = coeffs if coeffs.dim() == 2 else coeffs.unsqueeze(0)
+        _check_coeffs(co, B, C, torch.complex128, "adjoint")
         _check(_lib().gsh_adjoint_inverse(self._h, lmax, n, REAL if real else COMPLEX,
                                           PRECISION[precision], STORAGE["fp64"], B, _ptr(f),
-                                          self.FieldSize(), _ptr(coeffs), C, _stream(stream)),
+                                          self.FieldSize(), _ptr(co), co.shape[1],
+                                          _stream(stream)),
                "gsh_adjoint_inverse")
         return coeffs if field.dim() == 3 else coeffs[0]
 
@@ -578,7 +629,10 @@ class BatchPlan:
     layout; fields are (count, ntheta, nphi).  The first call in each
     direction autotunes the tile shapes; graph=True captures the launch
     sequence for the given tensors into a HIP graph on first use and replays
-    it afterwards."""
+    it afterwards.  A graph is keyed on the tensors' addresses and strides,
+    so graph=True pays only with stable caller buffers: with field=None or
+    coeffs=None every call allocates a new output and captures anew (the
+    plan keeps its 8 most recently used graphs)."""
 
     def __init__(self, grid, lmax, n, real=False, precision="fp64", storage="fp64"):
         import numpy as _np
@@ -621,9 +675,12 @@ class BatchPlan:
     def inverse(self, coeffs, field=None, graph=False, stream=None):
         import torch
         g = self.grid
+        _check_coeffs(coeffs, self.count, self.cmax, self.coeff_dtype(), "BatchPlan.inverse")
         if field is None:
             field = torch.empty((self.count, g.NumberOfCoLatitudes(), g.NumberOfLongitudes()),
                                 dtype=self.field_dtype(), device=coeffs.device)
+        _check_field(field, self.count, g.NumberOfCoLatitudes(), g.NumberOfLongitudes(),
+                     self.field_dtype(), "BatchPlan.inverse")
         _check(_lib().gsh_plan_inverse(self._h, _ptr(coeffs), coeffs.shape[1], _ptr(field),
                                        g.FieldSize(), 1 if graph else 0, _stream(stream)),
                "gsh_plan_inverse")
@@ -631,9 +688,13 @@ class BatchPlan:
 
     def forward(self, field, coeffs=None, graph=False, stream=None):
         import torch
+        g = self.grid
+        _check_field(field, self.count, g.NumberOfCoLatitudes(), g.NumberOfLongitudes(),
+                     self.field_dtype(), "BatchPlan.forward")
         if coeffs is None:
             coeffs = torch.zeros((self.count, self.cmax), dtype=self.coeff_dtype(),
                                  device=field.device)
+        _check_coeffs(coeffs, self.count, self.cmax, self.coeff_dtype(), "BatchPlan.forward")
         _check(_lib().gsh_plan_forward(self._h, _ptr(field), self.grid.FieldSize(),
                                        _ptr(coeffs), coeffs.shape[1], 1 if graph else 0,
                                        _stream(stream)), "gsh_plan_forward")

@@ -191,6 +191,8 @@ struct WorkList {
 
 }  // namespace
 
+struct gsh_plan_s;
+
 struct gsh_grid_s {
   int L = 0, nmax = 0, mrange = 0, nrange = 0, device = 0;
   std::vector<double> theta, weight;
@@ -215,6 +217,13 @@ struct gsh_grid_s {
   std::map<std::tuple<int, int, int, int, std::vector<int>>, WorkList> worklists;
   double* stage = nullptr;  // host-call staging
   size_t stage_bytes = 0;
+  // Incremented whenever work or work2 is (re)allocated: a captured plan
+  // graph embeds those pointers and is valid only for the generation it was
+  // captured in.
+  unsigned long long work_gen = 0;
+  // Live batch plans on this grid (gsh_grid_destroy releases their device
+  // resources and detaches them).
+  std::vector<gsh_plan_s*> plans;
   std::mutex mu;
   // Kernel timing (gsh_timing_*): event pairs per kernel id.
   bool timing = false;
@@ -263,12 +272,23 @@ int ensure_ndata(gsh_grid_s* g, int n, NData** out) {
   });
   NData d;
   GSH_HIP(hipSetDevice(g->device));
-  GSH_HIP(hipMalloc(&d.seeds, seeds.size() * sizeof(double)));
-  GSH_HIP(hipMalloc(&d.rec, total * sizeof(double4)));
-  GSH_HIP(hipMalloc(&d.roff, norders * sizeof(long)));
-  GSH_HIP(hipMemcpy(d.seeds, seeds.data(), seeds.size() * sizeof(double), hipMemcpyHostToDevice));
-  GSH_HIP(hipMemcpy(d.rec, rec.data(), total * sizeof(double4), hipMemcpyHostToDevice));
-  GSH_HIP(hipMemcpy(d.roff, roff.data(), norders * sizeof(long), hipMemcpyHostToDevice));
+  // Every partial allocation is released on failure.
+  auto release = [&](int status) {
+    if (d.seeds) (void)hipFree(d.seeds);
+    if (d.rec) (void)hipFree(d.rec);
+    if (d.roff) (void)hipFree(d.roff);
+    return status;
+  };
+  if (hipMalloc(&d.seeds, seeds.size() * sizeof(double)) != hipSuccess ||
+      hipMalloc(&d.rec, total * sizeof(double4)) != hipSuccess ||
+      hipMalloc(&d.roff, norders * sizeof(long)) != hipSuccess)
+    return release(GSH_ERR_OUT_OF_MEMORY);
+  if (hipMemcpy(d.seeds, seeds.data(), seeds.size() * sizeof(double), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(d.rec, rec.data(), total * sizeof(double4), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(d.roff, roff.data(), norders * sizeof(long), hipMemcpyHostToDevice) != hipSuccess)
+    return release(GSH_ERR_DEVICE);
   auto res = g->ndata.emplace(n, d);
   *out = &res.first->second;
   return GSH_OK;
@@ -305,6 +325,7 @@ int ensure_work(gsh_grid_s* g, size_t bytes) {
     g->work = nullptr;
     g->work_bytes = 0;
   }
+  g->work_gen++;
   if (hipMalloc(&g->work, bytes) != hipSuccess) {
     g->work = nullptr;
     return GSH_ERR_OUT_OF_MEMORY;
@@ -321,6 +342,7 @@ int ensure_work2(gsh_grid_s* g, size_t bytes) {
     g->work2 = nullptr;
     g->work2_bytes = 0;
   }
+  g->work_gen++;
   if (hipMalloc(&g->work2, bytes) != hipSuccess) {
     g->work2 = nullptr;
     return GSH_ERR_OUT_OF_MEMORY;
@@ -608,6 +630,10 @@ void fft_ld(std::vector<std::complex<long double>>& a) {
   }
 }
 
+// Releases a plan's device resources and detaches it from its grid
+// (defined with the plans below).
+void plan_detach(gsh_plan_s* P);
+
 }  // namespace
 
 // ===========================================================================
@@ -765,6 +791,11 @@ int gsh_grid_destroy(gsh_grid_t g) {
   if (!g) return GSH_ERR_INVALID_ARGUMENT;
   (void)hipSetDevice(g->device);
   (void)hipDeviceSynchronize();
+  // Plans still alive lose their graphs and device arrays (which embed this
+  // grid's buffers) and are detached: their later calls fail with
+  // GSH_ERR_INVALID_ARGUMENT and gsh_plan_destroy frees only the host object.
+  for (gsh_plan_s* P : g->plans) plan_detach(P);
+  g->plans.clear();
   for (auto& kv : g->ndata) {
     (void)hipFree(kv.second.seeds);
     (void)hipFree(kv.second.rec);
@@ -1300,8 +1331,22 @@ struct GraphKey {
   }
 };
 
+// A captured launch sequence.  It embeds the grid's work buffers and the
+// plan's staging buffer as they were at capture, so it is replayed only while
+// both generations are unchanged (a larger call on the grid, or a larger
+// stride on an FP32 plan, reallocates them); otherwise it is re-captured.
+struct PlanGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  unsigned long long work_gen = 0, stage_gen = 0;
+  unsigned long long last_use = 0;
+};
+
+// Graphs kept per plan (least recently used evicted first).
+constexpr size_t kPlanGraphCap = 8;
+
 struct gsh_plan_s {
-  gsh_grid_s* g = nullptr;
+  gsh_grid_s* g = nullptr;  // null once the grid has been destroyed
   int count = 0, kind = 0, precision = 0, storage = 0;
   std::vector<int> lmax, n;
   long cmax = 0;  // largest coefficient count of a field
@@ -1309,12 +1354,47 @@ struct gsh_plan_s {
   bool tuned[2] = {false, false};
   double* stage = nullptr;  // FP64 coefficients of FP32-storage plans
   size_t stage_bytes = 0;
+  unsigned long long stage_gen = 0;  // incremented whenever stage is (re)allocated
   long* d_csize = nullptr;  // coefficient count per field
   hipStream_t cap = nullptr;
-  std::map<GraphKey, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+  std::map<GraphKey, PlanGraph> graphs;
+  unsigned long long clock = 0;  // graph use counter (LRU)
 };
 
 namespace {
+
+void drop_graph(PlanGraph& G) {
+  if (G.exec) (void)hipGraphExecDestroy(G.exec);
+  if (G.graph) (void)hipGraphDestroy(G.graph);
+  G.exec = nullptr;
+  G.graph = nullptr;
+}
+
+// Releases every device resource of a plan (its grid's lock held, device
+// current); the plan object itself stays.
+void plan_release(gsh_plan_s* P) {
+  (void)hipDeviceSynchronize();
+  for (auto& kv : P->graphs) drop_graph(kv.second);
+  P->graphs.clear();
+  for (PlanGroup& G : P->groups) {
+    (void)hipFree(G.d_fmap);
+    (void)hipFree(G.d_flmax);
+    (void)hipFree(G.d_cglmax);
+    G.d_fmap = G.d_flmax = G.d_cglmax = nullptr;
+  }
+  if (P->stage) (void)hipFree(P->stage);
+  P->stage = nullptr;
+  P->stage_bytes = 0;
+  if (P->d_csize) (void)hipFree(P->d_csize);
+  P->d_csize = nullptr;
+  if (P->cap) (void)hipStreamDestroy(P->cap);
+  P->cap = nullptr;
+}
+
+void plan_detach(gsh_plan_s* P) {
+  plan_release(P);
+  P->g = nullptr;
+}
 
 int plan_stage(gsh_plan_s* P, long cstride) {
   const size_t bytes = 16 * (size_t)P->count * cstride;
@@ -1325,6 +1405,7 @@ int plan_stage(gsh_plan_s* P, long cstride) {
   }
   P->stage = nullptr;
   P->stage_bytes = 0;
+  P->stage_gen++;
   if (hipMalloc(&P->stage, bytes) != hipSuccess) return GSH_ERR_OUT_OF_MEMORY;
   P->stage_bytes = bytes;
   return GSH_OK;
@@ -1453,6 +1534,7 @@ int plan_exec(gsh_plan_s* P, int dir, const void* in, long istride, void* out, l
               int graph, void* stream) {
   if (!P || !in || !out) return GSH_ERR_INVALID_ARGUMENT;
   gsh_grid_s* g = P->g;
+  if (!g) return GSH_ERR_INVALID_ARGUMENT;  // its grid has been destroyed
   const long fsize = (long)g->ntheta() * g->nphi();
   const long cs = dir == 0 ? istride : ostride;
   const long fs = dir == 0 ? ostride : istride;
@@ -1465,8 +1547,17 @@ int plan_exec(gsh_plan_s* P, int dir, const void* in, long istride, void* out, l
   if (graph) {
     auto it = P->graphs.find(key);
     if (it != P->graphs.end()) {
-      GSH_HIP(hipGraphLaunch(it->second.second, s));
-      return GSH_OK;
+      PlanGraph& G = it->second;
+      if (G.work_gen == g->work_gen && G.stage_gen == P->stage_gen) {
+        G.last_use = ++P->clock;
+        GSH_HIP(hipGraphLaunch(G.exec, s));
+        return GSH_OK;
+      }
+      // A buffer the graph embeds has been reallocated since its capture:
+      // the graph is stale and is captured again below.
+      GSH_HIP(hipStreamSynchronize(s));
+      drop_graph(G);
+      P->graphs.erase(it);
     }
   }
   if (!P->tuned[dir] && (st = tune_plan(P, dir, in, istride, out, ostride, s))) return st;
@@ -1475,6 +1566,7 @@ int plan_exec(gsh_plan_s* P, int dir, const void* in, long istride, void* out, l
   if (!graph) return GSH_OK;
   if (!P->cap) GSH_HIP(hipStreamCreateWithFlags(&P->cap, hipStreamNonBlocking));
   GSH_HIP(hipStreamSynchronize(s));
+  const unsigned long long wgen = g->work_gen, sgen = P->stage_gen;
   const bool was_timing = g->timing;
   g->timing = false;
   hipGraph_t gr = nullptr;
@@ -1482,14 +1574,75 @@ int plan_exec(gsh_plan_s* P, int dir, const void* in, long istride, void* out, l
   st = run_plan(P, dir, in, istride, out, ostride, P->cap);
   const hipError_t ec = hipStreamEndCapture(P->cap, &gr);
   g->timing = was_timing;
-  if (st) return st;
+  if (st) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return st;
+  }
   if (ec != hipSuccess) return GSH_ERR_DEVICE;
+  // The eager run sized every buffer, so the capture must not have
+  // reallocated one; if it had, the graph would embed freed memory.
+  if (g->work_gen != wgen || P->stage_gen != sgen) {
+    (void)hipGraphDestroy(gr);
+    return GSH_ERR_DEVICE;
+  }
   hipGraphExec_t ex = nullptr;
   if (hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) != hipSuccess) {
     (void)hipGraphDestroy(gr);
     return GSH_ERR_DEVICE;
   }
-  P->graphs.emplace(key, std::make_pair(gr, ex));
+  if (P->graphs.size() >= kPlanGraphCap) {
+    auto lru = P->graphs.begin();
+    for (auto jt = P->graphs.begin(); jt != P->graphs.end(); ++jt)
+      if (jt->second.last_use < lru->second.last_use) lru = jt;
+    drop_graph(lru->second);
+    P->graphs.erase(lru);
+  }
+  PlanGraph G;
+  G.graph = gr;
+  G.exec = ex;
+  G.work_gen = wgen;
+  G.stage_gen = sgen;
+  G.last_use = ++P->clock;
+  P->graphs.emplace(key, G);
+  return GSH_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// Groups of a new plan by upper index, with their device arrays.
+int plan_build(gsh_grid_s* g, gsh_plan_s* P, const std::map<int, std::vector<int>>& byn,
+               const int* lmax, const std::vector<long>& csize) {
+  int st;
+  const int count = P->count;
+  for (auto& kv : byn) {
+    PlanGroup G;
+    G.n = kv.first;
+    G.fmap = kv.second;
+    std::stable_sort(G.fmap.begin(), G.fmap.end(),
+                     [&](int a, int b) { return lmax[a] > lmax[b]; });
+    G.count = static_cast<int>(G.fmap.size());
+    for (int f : G.fmap) G.flmax.push_back(lmax[f]);
+    G.lmax = G.flmax[0];
+    const int ncg = (2 * G.count + 15) / 16;
+    std::vector<int> cg(ncg);
+    for (int c = 0; c < ncg; c++) cg[c] = G.flmax[8 * c];  // descending: first is largest
+    G.cglmax = cg;
+    if ((st = ensure_ndata(g, G.n, &G.nd))) return st;
+    // Held by the plan before its arrays are allocated, so a failure below
+    // leaves them to plan_release.
+    P->groups.push_back(std::move(G));
+    PlanGroup& H = P->groups.back();
+    GSH_HIP(hipMalloc(&H.d_fmap, H.count * sizeof(int)));
+    GSH_HIP(hipMalloc(&H.d_flmax, H.count * sizeof(int)));
+    GSH_HIP(hipMalloc(&H.d_cglmax, ncg * sizeof(int)));
+    GSH_HIP(hipMemcpy(H.d_fmap, H.fmap.data(), H.count * sizeof(int), hipMemcpyHostToDevice));
+    GSH_HIP(hipMemcpy(H.d_flmax, H.flmax.data(), H.count * sizeof(int), hipMemcpyHostToDevice));
+    GSH_HIP(hipMemcpy(H.d_cglmax, H.cglmax.data(), ncg * sizeof(int), hipMemcpyHostToDevice));
+  }
+  GSH_HIP(hipMalloc(&P->d_csize, count * sizeof(long)));
+  GSH_HIP(hipMemcpy(P->d_csize, csize.data(), count * sizeof(long), hipMemcpyHostToDevice));
   return GSH_OK;
 }
 
@@ -1523,52 +1676,24 @@ int gsh_plan_create(gsh_grid_t g, int count, const int* lmax, const int* n, int 
   for (int i = 0; i < count; i++) byn[n[i]].push_back(i);
   std::lock_guard<std::mutex> lock(g->mu);
   GSH_HIP(hipSetDevice(g->device));
-  for (auto& kv : byn) {
-    PlanGroup G;
-    G.n = kv.first;
-    G.fmap = kv.second;
-    std::stable_sort(G.fmap.begin(), G.fmap.end(),
-                     [&](int a, int b) { return lmax[a] > lmax[b]; });
-    G.count = static_cast<int>(G.fmap.size());
-    for (int f : G.fmap) G.flmax.push_back(lmax[f]);
-    G.lmax = G.flmax[0];
-    const int ncg = (2 * G.count + 15) / 16;
-    std::vector<int> cg(ncg);
-    for (int c = 0; c < ncg; c++) cg[c] = G.flmax[8 * c];  // descending: first is largest
-    G.cglmax = cg;
-    if ((st = ensure_ndata(g, G.n, &G.nd))) return st;
-    GSH_HIP(hipMalloc(&G.d_fmap, G.count * sizeof(int)));
-    GSH_HIP(hipMalloc(&G.d_flmax, G.count * sizeof(int)));
-    GSH_HIP(hipMalloc(&G.d_cglmax, ncg * sizeof(int)));
-    GSH_HIP(hipMemcpy(G.d_fmap, G.fmap.data(), G.count * sizeof(int), hipMemcpyHostToDevice));
-    GSH_HIP(hipMemcpy(G.d_flmax, G.flmax.data(), G.count * sizeof(int), hipMemcpyHostToDevice));
-    GSH_HIP(hipMemcpy(G.d_cglmax, cg.data(), ncg * sizeof(int), hipMemcpyHostToDevice));
-    P->groups.push_back(std::move(G));
+  // A failure part-way releases what the plan has allocated so far.
+  st = plan_build(g, P.get(), byn, lmax, csize);
+  if (st) {
+    plan_release(P.get());
+    return st;
   }
-  GSH_HIP(hipMalloc(&P->d_csize, count * sizeof(long)));
-  GSH_HIP(hipMemcpy(P->d_csize, csize.data(), count * sizeof(long), hipMemcpyHostToDevice));
+  g->plans.push_back(P.get());
   *plan = P.release();
   return GSH_OK;
 }
 
 int gsh_plan_destroy(gsh_plan_t P) {
   if (!P) return GSH_ERR_INVALID_ARGUMENT;
-  {
-    std::lock_guard<std::mutex> lock(P->g->mu);
-    (void)hipSetDevice(P->g->device);
-    (void)hipDeviceSynchronize();
-    for (auto& kv : P->graphs) {
-      (void)hipGraphExecDestroy(kv.second.second);
-      (void)hipGraphDestroy(kv.second.first);
-    }
-    for (PlanGroup& G : P->groups) {
-      (void)hipFree(G.d_fmap);
-      (void)hipFree(G.d_flmax);
-      (void)hipFree(G.d_cglmax);
-    }
-    if (P->stage) (void)hipFree(P->stage);
-    (void)hipFree(P->d_csize);
-    if (P->cap) (void)hipStreamDestroy(P->cap);
+  if (gsh_grid_s* g = P->g) {
+    std::lock_guard<std::mutex> lock(g->mu);
+    (void)hipSetDevice(g->device);
+    plan_release(P);
+    g->plans.erase(std::remove(g->plans.begin(), g->plans.end(), P), g->plans.end());
   }
   delete P;
   return GSH_OK;
@@ -1586,7 +1711,7 @@ int gsh_plan_forward(gsh_plan_t P, const void* field, long field_stride, void* c
 
 int gsh_plan_info(gsh_plan_t P, int* ngroups, int cap, int* n, int* lmax, int* count,
                   int* synth_tiles, int* fft_inverse, int* fft_forward) {
-  if (!P || !ngroups) return GSH_ERR_INVALID_ARGUMENT;
+  if (!P || !ngroups || !P->g) return GSH_ERR_INVALID_ARGUMENT;
   *ngroups = static_cast<int>(P->groups.size());
   for (int i = 0; i < cap && i < *ngroups; i++) {
     const PlanGroup& G = P->groups[i];

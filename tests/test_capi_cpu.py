@@ -120,3 +120,35 @@ def test_cpp_facade_compiles_and_reports_errors():
         return
     r = subprocess.run([binary], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "no gfx950 device" in r.stderr
+
+
+def test_wrapper_argument_validation():
+    """The Python wrappers check dtype, row count and per-row size of every
+    array before they pass a raw pointer to the C ABI, so a wrong tensor
+    raises ValueError instead of letting a kernel read or write past the
+    allocation.  Host tensors pass the dtype and shape checks and then fail
+    the device check."""
+    import torch
+    from gshtrans_amd import _check_coeffs, _check_field
+    c = torch.zeros((3, 10), dtype=torch.complex128)
+    _bad = [
+        (lambda: _check_coeffs(c.to(torch.complex64), 3, 10, torch.complex128, "t"), "complex128"),
+        (lambda: _check_coeffs(c, 4, 10, torch.complex128, "t"), "(4, >= 10)"),
+        (lambda: _check_coeffs(c, 3, 11, torch.complex128, "t"), "(3, >= 11)"),
+        (lambda: _check_coeffs(c[:, ::2], 3, 5, torch.complex128, "t"), "contiguous"),
+        (lambda: _check_coeffs(c, 3, 10, torch.complex128, "t"), "cuda"),
+        (lambda: _check_coeffs(np.zeros((3, 10), complex), 3, 10, torch.complex128, "t"),
+         "torch tensor"),
+    ]
+    f = torch.zeros((2, 5, 8), dtype=torch.float64)
+    _bad += [
+        (lambda: _check_field(f, 2, 5, 8, torch.complex128, "t"), "complex128"),
+        (lambda: _check_field(f.to(torch.float32), 2, 5, 8, torch.float64, "t"), "float64"),
+        (lambda: _check_field(f, 3, 5, 8, torch.float64, "t"), "(3, 5, 8)"),
+        (lambda: _check_field(f[:, :4], 2, 4, 8, torch.float64, "t"), "contiguous"),
+        (lambda: _check_field(f, 2, 5, 8, torch.float64, "t"), "cuda"),
+    ]
+    for fn, msg in _bad:
+        with pytest.raises(ValueError) as e:
+            fn()
+        assert msg in str(e.value), (msg, str(e.value))

@@ -139,3 +139,69 @@ def test_config4_uneven_batch_round_trip(torch):
         C = plan.sizes[i]
         ref = g.inverse(int(lmax[i]), int(n[i]), c[i, :C].to(torch.complex128).contiguous())
         assert rel(f[i].to(torch.complex128), ref) < 1e-6, i
+
+
+def test_plan_graph_recaptured_after_buffer_growth(torch):
+    """A captured plan graph embeds the grid's work buffer and the plan's
+    FP32 staging buffer.  A larger eager call on the same grid reallocates
+    the work buffer, and a larger coefficient stride on an FP32 plan
+    reallocates the staging buffer; the plan must then capture again rather
+    than replay a graph holding freed memory.  Every replay equals the eager
+    result of the same call."""
+    L = 128
+    g = G.GaussLegendreGrid(L, 2)
+    lmax = np.array([40, 33, 20, 12])
+    n = np.array([0, 1, -1, 2])
+    for storage in ("fp64", "fp32"):
+        say(f"[plan graph, buffer growth] storage={storage}")
+        plan = G.BatchPlan(g, lmax, n, storage=storage)
+        cdt = plan.coeff_dtype()
+        c = make_coeffs(torch, lmax, n, False, plan.cmax, SEED0 + 77, dtype=cdt)
+        eager_f = plan.inverse(c).clone()
+        eager_c = plan.forward(eager_f).clone()
+        fg = torch.empty_like(eager_f)
+        cg = torch.zeros_like(c)
+        for rep in range(3):
+            fg.zero_()
+            plan.inverse(c, field=fg, graph=True)
+            cg.zero_()
+            plan.forward(fg, coeffs=cg, graph=True)
+            torch.cuda.synchronize()
+            assert torch.equal(fg, eager_f), (storage, rep)
+            assert torch.equal(cg, eager_c), (storage, rep)
+            # Eager work between replays that grows the grid's work buffer
+            # (8, 24, then 40 complex fields at the grid's full degree).
+            big = torch.from_numpy(G.random_coefficients(L, 0, seed=SEED0 + rep,
+                                                         batch=8 + 16 * rep)).cuda()
+            _ = g.forward(L, 0, g.inverse(L, 0, big))
+            torch.cuda.synchronize()
+        if storage == "fp32":
+            # A wider coefficient stride reallocates the staging buffer; the
+            # graph captured for the first stride is stale afterwards.
+            wide = torch.zeros((plan.count, plan.cmax + 300), dtype=cdt, device="cuda")
+            wide[:, :plan.cmax] = c
+            fw = torch.empty_like(eager_f)
+            for rep in range(2):
+                fw.zero_()
+                plan.inverse(wide, field=fw, graph=True)
+                torch.cuda.synchronize()
+                assert torch.equal(fw, eager_f), rep
+                fg.zero_()
+                plan.inverse(c, field=fg, graph=True)
+                torch.cuda.synchronize()
+                assert torch.equal(fg, eager_f), rep
+        plan.close()
+
+
+def test_plan_outlives_grid(torch):
+    """Destroying a grid detaches its plans: later plan calls raise, and
+    closing the plan afterwards is safe."""
+    g = G.GaussLegendreGrid(32, 1)
+    plan = G.BatchPlan(g, [32, 16], [0, 1])
+    c = make_coeffs(torch, [32, 16], [0, 1], False, plan.cmax, SEED0 + 5)
+    plan.inverse(c, graph=True)
+    torch.cuda.synchronize()
+    g.close()
+    with pytest.raises(G.GSHError):
+        plan.inverse(c)
+    plan.close()

@@ -1,10 +1,17 @@
 """Mixed-precision Legendre contraction (split-BF16 MFMA, FP64 recurrence,
 seeds, weights and FFT): relative L2 error of each field against the FP64
 GPU transform, which is itself pinned to the oracle at <= 1e-12 by
-test_gpu_parity.py.  Bars: bf16x6 <= 1e-7 (the precision target), bf16x3
-<= 1e-4 (two-way split, ~16 significant bits per operand), fp32 <= 3e-5
-(FP32 MFMA on FP32-rounded operands with FP32 accumulation over up to
-L + 1 terms; the mode is reported, not held to the 1e-7 target)."""
+test_gpu_parity.py.  Bars: bf16x6 <= 1e-7 (the precision target); bf16x3
+<= 3e-5 (two-way split, ~16 significant bits per operand; measured up to
+9.2e-6 at lMax=1024, batch 64) and fp32 <= 3e-6 (FP32 MFMA on FP32-rounded
+operands with FP32 accumulation over up to L + 1 terms; measured up to
+7.6e-7 at lMax=1024, batch 64): these two modes are reported, not held to
+the 1e-7 target, and their bars sit about 3-4x above the measured maxima so
+that a lost product term or chunk shows.
+
+The lMax=1024 batch-64 cases are the headline workload: 64 real n=0 fields
+and 64 complex n=2 fields, i.e. 128 columns, one whole 8-tile super-group
+of the split-BF16 kernels."""
 import sys
 
 import numpy as np
@@ -14,7 +21,7 @@ import gshtrans_amd as G
 
 pytestmark = pytest.mark.gpu
 SEED0 = 20261202
-BARS = {"bf16x6": 1e-7, "bf16x3": 1e-4, "fp32": 3e-5}
+BARS = {"bf16x6": 1e-7, "bf16x3": 3e-5, "fp32": 3e-6}
 
 
 def say(*a):
@@ -47,7 +54,8 @@ def per_field_rel(torch, a, b):
 @pytest.mark.parametrize("L,n,real,batch", [(64, 0, False, 3), (64, 2, True, 5),
                                             (512, 2, False, 8), (300, -1, False, 2),
                                             (1024, 0, True, 16), (1024, 2, False, 16),
-                                            (100, 1, True, 70)])
+                                            (100, 1, True, 70), (1024, 0, True, 64),
+                                            (1024, 2, False, 64)])
 def test_mixed_synthesis_error(torch, prec, L, n, real, batch):
     say(f"[mixed synthesis] {prec} L={L} n={n} real={real} batch={batch}")
     g = grid(L)
@@ -56,7 +64,7 @@ def test_mixed_synthesis_error(torch, prec, L, n, real, batch):
     f64 = g.inverse(L, n, c, real=real)
     fmx = g.inverse(L, n, c, real=real, precision=prec)
     e = per_field_rel(torch, fmx, f64)
-    say(f"    max {e.max():.2e} median {np.median(e):.2e}")
+    say(f"    max {e.max():.2e} (field {int(e.argmax())}) median {np.median(e):.2e}")
     assert e.max() <= BARS[prec]
     assert e.min() > 0 or prec == "fp64"
 
@@ -96,7 +104,8 @@ def test_mixed_synthesis_last_rows(torch, prec, L, n, real, batch):
 @pytest.mark.parametrize("L,n,real,batch", [(64, 0, False, 3), (64, 2, True, 5),
                                             (512, 2, False, 8), (300, -1, False, 2),
                                             (1024, 0, True, 16), (1024, 2, False, 16),
-                                            (100, 1, True, 70), (24, 1, False, 4)])
+                                            (100, 1, True, 70), (24, 1, False, 4),
+                                            (1024, 0, True, 64), (1024, 2, False, 64)])
 def test_mixed_analysis_error(torch, prec, L, n, real, batch):
     say(f"[mixed analysis] {prec} L={L} n={n} real={real} batch={batch}")
     g = grid(L)
@@ -108,7 +117,8 @@ def test_mixed_analysis_error(torch, prec, L, n, real, batch):
     e = per_field_rel(torch, amx, a64)
     rt = per_field_rel(torch, g.forward(L, n, g.inverse(L, n, c, real=real, precision=prec),
                                         real=real, precision=prec), c)
-    say(f"    analysis max {e.max():.2e} median {np.median(e):.2e}; round trip max {rt.max():.2e}")
+    say(f"    analysis max {e.max():.2e} (field {int(e.argmax())}) median {np.median(e):.2e}; "
+        f"round trip max {rt.max():.2e}")
     assert e.max() <= BARS[prec]
     assert rt.max() <= 2 * BARS[prec]
 

@@ -229,6 +229,33 @@ def test_L1024_parity_one_field(torch):
     check_pair(torch, 1024, 2, 1024, 2, False, 1.0, SEED0 + 2, grid=_grid(1024, 2))
 
 
+@pytest.mark.parametrize("n,real", [(0, True), (2, False)])
+def test_config3_fp64_batch64(torch, n, real):
+    """The headline workload in FP64: a batch of 64 real n=0 fields or 64
+    complex n=2 fields at lMax=1024 (128 columns per launch), (l+1)^-1
+    decay.  Fields 0 and 63 against the oracle's synthesis and analysis
+    (<= 1e-12), and the whole batch's round trip (5e4 eps)."""
+    L, B = 1024, 64
+    g = _grid(L, 2)
+    og = oracle_grid(L, 2)
+    say(f"[config3 fp64] L={L} n={n} real={real} batch={B}")
+    cs = G.random_coefficients(L, n, real=real, seed=SEED0 + 2, decay=1.0, batch=B)
+    f = g.inverse(L, n, dev(torch, cs), real=real)
+    c2 = host(g.forward(L, n, f, real=real))
+    fh = host(f)
+    for b in (0, B - 1):
+        f_ref = og.inverse(L, n, cs[b], real)
+        e_syn = rel_l2(fh[b], f_ref)
+        c_ref = og.forward(L, n, f_ref, real)
+        c_gpu = host(g.forward(L, n, dev(torch, f_ref), real=real))
+        e_ana = rel_l2(c_gpu, c_ref)
+        say(f"    field {b}: synthesis {e_syn:.2e}  analysis {e_ana:.2e}")
+        assert e_syn <= 1e-12 and e_ana <= 1e-12, b
+    e_rt = float(np.max(np.abs(c2 - cs)))
+    say(f"    batch round trip max {e_rt:.2e}")
+    assert e_rt <= 5e4 * EPS
+
+
 def test_L1024_batch_round_trip_and_linearity(torch):
     """Size-independent properties on a 16-field batch at lMax=1024."""
     L = 1024
