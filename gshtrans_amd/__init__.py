@@ -20,10 +20,12 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgshtrans_hip.so")
 
 GSH_OK = 0
+GSH_ERR_SCRATCH_LIMIT = 9
 STATUS = {
     0: "GSH_OK", 1: "GSH_ERR_INVALID_ARGUMENT", 2: "GSH_ERR_UPPER_INDEX",
     3: "GSH_ERR_DEGREE", 4: "GSH_ERR_VALUE_KIND", 5: "GSH_ERR_UNSUPPORTED",
     6: "GSH_ERR_DEVICE", 7: "GSH_ERR_NO_DEVICE", 8: "GSH_ERR_OUT_OF_MEMORY",
+    9: "GSH_ERR_SCRATCH_LIMIT",
 }
 RANGE = {"All": 0, "NonNegative": 1, "Single": 2}
 PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16x6": 3}
@@ -40,7 +42,7 @@ EXPORTS = [
     "gsh_plan_inverse", "gsh_plan_forward", "gsh_plan_info", "gsh_device_alloc",
     "gsh_device_free", "gsh_copy", "gsh_synchronize", "gsh_wigner_size", "gsh_wigner_offset",
     "gsh_wigner_table", "gsh_wigner_table_host", "gsh_forward_product", "gsh_integrate",
-    "gsh_random_coefficients",
+    "gsh_random_coefficients", "gsh_integrate_noalloc",
 ]
 KERNELS = ["legendre_synthesis", "fft_synthesis", "fft_analysis", "legendre_analysis",
            "coef_split", "bin_split"]
@@ -92,6 +94,7 @@ def _lib():
         L.gsh_timing_read.argtypes = [vp, dp, ctypes.POINTER(l), i]
         L.gsh_forward_product.argtypes = [vp, i, i, i, i, i, vp, vp, l, vp, l, i, vp]
         L.gsh_integrate.argtypes = [vp, i, i, vp, vp, l, vp, vp]
+        L.gsh_integrate_noalloc.argtypes = [vp, i, i, vp, vp, l, vp, vp]
         L.gsh_random_coefficients.argtypes = [vp, i, i, i, ctypes.c_ulonglong, d, i, vp, l, vp]
         L.gsh_wigner_size.restype = l
         L.gsh_wigner_size.argtypes = [i, i, i, i, i, i]
@@ -104,7 +107,8 @@ def _lib():
                      "gsh_inverse", "gsh_forward", "gsh_adjoint_inverse",
                      "gsh_inverse_host", "gsh_forward_host", "gsh_timing_enable",
                      "gsh_timing_read", "gsh_wigner_table", "gsh_wigner_table_host",
-                     "gsh_forward_product", "gsh_integrate", "gsh_random_coefficients"):
+                     "gsh_forward_product", "gsh_integrate", "gsh_integrate_noalloc",
+                     "gsh_random_coefficients"):
             getattr(L, name).restype = i
         _LIB = L
     return _LIB

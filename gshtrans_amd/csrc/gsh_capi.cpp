@@ -652,6 +652,8 @@ const char* gsh_status_string(int status) {
     case GSH_ERR_DEVICE: return "HIP runtime error";
     case GSH_ERR_NO_DEVICE: return "no gfx950 device";
     case GSH_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case GSH_ERR_SCRATCH_LIMIT:
+      return "batch x colatitudes exceed the static scratch of the allocation-free entry point";
     default: return "unknown status";
   }
 }
@@ -783,6 +785,9 @@ int gsh_grid_create(int lmax, int nmax, int mrange, int nrange, gsh_grid_t* grid
     GSH_HIP(hipMemcpy(g->d_one, one.data(), one.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   GSH_HIP(hipMemcpy(g->d_tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
+  // Address of this device's copy of the allocation-free entry point's
+  // scratch array (looked up once per device, with the device current).
+  GSH_HIP(gsh::noalloc_scratch_init(dev));
   *grid = g.release();
   return GSH_OK;
 }
@@ -1107,32 +1112,64 @@ int gsh_forward(gsh_grid_t g, int lmax, int n, int value_kind, int precision, in
 // Integrate (ScalarFieldOverloads.h:288-301) and the L2 inner product
 // (:303-316) with the grid's weights (GridBase.h:60-66): ring sums on the
 // device, then each field's sum over the colatitudes in order.
-int gsh_integrate(gsh_grid_t g, int value_kind, int batch, const void* field_a,
-                  const void* field_b, long field_stride, double* out, void* stream) {
+namespace {
+// Argument checks and launch parameters shared by gsh_integrate and
+// gsh_integrate_noalloc (everything but the ring-sum buffer p.part).
+int integrate_params(gsh_grid_t g, int value_kind, int batch, const void* field_a,
+                     const void* field_b, long field_stride, double* out,
+                     gsh::GridOpParams* p) {
   if (!g || batch < 0 || (value_kind != GSH_REAL && value_kind != GSH_COMPLEX))
     return GSH_ERR_INVALID_ARGUMENT;
   if (value_kind == GSH_COMPLEX && g->mrange != GSH_ALL) return GSH_ERR_VALUE_KIND;
-  if (batch == 0) return GSH_OK;
+  if (batch == 0) return GSH_OK;  // nothing to do (the callers return here too)
   const long fsize = static_cast<long>(g->ntheta()) * g->nphi();
   if (!field_a || !out || (batch > 1 && field_stride < fsize)) return GSH_ERR_INVALID_ARGUMENT;
   if (g->L == 0) return GSH_ERR_UNSUPPORTED;  // no longitudes
+  *p = gsh::GridOpParams{};
+  p->ntheta = g->ntheta();
+  p->nphi = g->nphi();
+  p->batch = batch;
+  p->real = value_kind == GSH_REAL;
+  p->fstride = batch > 1 ? field_stride : fsize;
+  p->fa = field_a;
+  p->fb = field_b;
+  p->wq = g->d_wq;  // w_i 2 pi / nphi: the colatitude weight times the longitude weight
+  p->dphi = 1.0;
+  p->out = out;
+  return GSH_OK;
+}
+}  // namespace
+
+int gsh_integrate(gsh_grid_t g, int value_kind, int batch, const void* field_a,
+                  const void* field_b, long field_stride, double* out, void* stream) {
+  gsh::GridOpParams p;
+  int st = integrate_params(g, value_kind, batch, field_a, field_b, field_stride, out, &p);
+  if (st || batch == 0) return st;
   hipStream_t s = static_cast<hipStream_t>(stream);
   std::lock_guard<std::mutex> lock(g->mu);
   GSH_HIP(hipSetDevice(g->device));
-  int st;
   if ((st = ensure_work(g, sizeof(double2) * (size_t)batch * g->ntheta()))) return st;
-  gsh::GridOpParams p{};
-  p.ntheta = g->ntheta();
-  p.nphi = g->nphi();
-  p.batch = batch;
-  p.real = value_kind == GSH_REAL;
-  p.fstride = batch > 1 ? field_stride : fsize;
-  p.fa = field_a;
-  p.fb = field_b;
-  p.wq = g->d_wq;  // w_i 2 pi / nphi: the colatitude weight times the longitude weight
-  p.dphi = 1.0;
   p.part = reinterpret_cast<double2*>(g->work);
-  p.out = out;
+  GSH_HIP(gsh::launch_integrate(p, s));
+  return GSH_OK;
+}
+
+// The same with the ring sums in the library's static scratch array: no
+// allocation, no host synchronisation, two kernel launches on `stream`.
+int gsh_integrate_noalloc(gsh_grid_t g, int value_kind, int batch, const void* field_a,
+                          const void* field_b, long field_stride, double* out, void* stream) {
+  gsh::GridOpParams p;
+  int st = integrate_params(g, value_kind, batch, field_a, field_b, field_stride, out, &p);
+  if (st || batch == 0) return st;
+  if (static_cast<long>(batch) * g->ntheta() > gsh::kNoAllocRingSums)
+    return GSH_ERR_SCRATCH_LIMIT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> lock(g->mu);
+  GSH_HIP(hipSetDevice(g->device));
+  // Every entry the second kernel reads, part[f * ntheta + i] for f < batch
+  // and i < ntheta, is written by the first, so nothing is cleared first.
+  p.part = gsh::noalloc_scratch(g->device);
+  if (!p.part) return GSH_ERR_DEVICE;  // set up by gsh_grid_create
   GSH_HIP(gsh::launch_integrate(p, s));
   return GSH_OK;
 }

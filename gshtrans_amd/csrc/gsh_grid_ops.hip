@@ -4,6 +4,8 @@
 // (GridBase.h:115-172 with a counter-based stream).
 #include "gsh_internal.h"
 
+#include <mutex>
+
 namespace gsh {
 namespace {
 
@@ -122,6 +124,40 @@ hipError_t launch_integrate(const GridOpParams& p, hipStream_t s) {
   hipLaunchKernelGGL(ring_sums, dim3(p.ntheta, p.batch), dim3(kRingThreads), 0, s, p);
   hipLaunchKernelGGL(theta_sums, dim3((p.batch + 63) / 64), dim3(64), 0, s, p);
   return hipGetLastError();
+}
+
+// Scratch of the allocation-free integration (gsh_integrate_noalloc): the
+// ring sums part[f * ntheta + i].  Every device that loads this library holds
+// its own copy until the process ends.  In host code `scratch` names a
+// host-side stand-in, so kernels receive only the device address looked up
+// below, never the array's name.
+__device__ __align__(256) double2 scratch[kNoAllocRingSums];
+
+namespace {
+std::mutex scratch_mu;
+double2* scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_dev[device]) return hipSuccess;
+  int cur = -1;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return e;
+  if (cur != device) return hipErrorInvalidDevice;  // the lookup needs `device` current
+  void* ptr = nullptr;
+  e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!ptr) return hipErrorInvalidSymbol;
+  scratch_dev[device] = static_cast<double2*>(ptr);
+  return hipSuccess;
+}
+
+double2* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  return scratch_dev[device];
 }
 
 hipError_t launch_random_coeffs(const RandomParams& p, hipStream_t s) {

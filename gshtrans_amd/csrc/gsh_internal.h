@@ -235,6 +235,15 @@ struct GridOpParams {
   double* out;              // complex results, interleaved
 };
 hipError_t launch_integrate(const GridOpParams& p, hipStream_t s);
+// Allocation-free integration: ring sums of at most kNoAllocRingSums (field,
+// colatitude) pairs in a file-scope device array (4 MiB per device).
+// noalloc_scratch_init looks up the calling thread's current device's copy
+// (once per device; gsh_grid_create calls it), noalloc_scratch returns that
+// address (null before the lookup).
+constexpr long kNoAllocRingSums = 262144;
+constexpr int kMaxDevices = 64;
+hipError_t noalloc_scratch_init(int device);
+double2* noalloc_scratch(int device);
 struct RandomParams {
   int lmax, grid_lmax, nabs, real, batch;
   long size, cstride;       // coefficients per field, complex elements between fields

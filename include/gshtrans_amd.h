@@ -34,6 +34,8 @@
  *                          launch, on the launch stream, for benchmarks
  *   gsh_integrate          Integrate / L2InnerProduct, ScalarFieldOverloads.h:
  *                          288-316, with GridBase::Weights (GridBase.h:60-66)
+ *   gsh_integrate_noalloc  no reference counterpart: gsh_integrate for callers
+ *                          that cannot allocate (static scratch, see below)
  *   gsh_random_coefficients GridBase::Random*Coefficient (GridBase.h:115-172),
  *                          seeded, on the device
  *   gsh_forward_product    ForwardTransformation of the product field of
@@ -86,7 +88,9 @@ enum gsh_status {
   GSH_ERR_UNSUPPORTED = 5,      /* precision/size combination not built    */
   GSH_ERR_DEVICE = 6,           /* HIP runtime error                       */
   GSH_ERR_NO_DEVICE = 7,        /* no gfx950 device visible                */
-  GSH_ERR_OUT_OF_MEMORY = 8
+  GSH_ERR_OUT_OF_MEMORY = 8,
+  GSH_ERR_SCRATCH_LIMIT = 9     /* call larger than the static scratch of an
+                                   allocation-free entry point            */
 };
 
 /* MRange / NRange tags (GSHTrans/src/Concepts.h:24-27). */
@@ -149,6 +153,18 @@ int gsh_forward_product(gsh_grid_t grid, int lmax, int n, int value_kind, int pr
  * complex (interleaved; imaginary part 0 for real fields), device memory. */
 int gsh_integrate(gsh_grid_t grid, int value_kind, int batch, const void* field_a,
                   const void* field_b, long field_stride, double* out, void* stream);
+/* gsh_integrate without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The ring
+ * sums go to a static device array of the library (4 MiB per device, held
+ * until the process ends) instead of the grid's work buffer, so batch x
+ * (lMax + 1) may not exceed 262144 (GSH_ERR_SCRATCH_LIMIT otherwise; e.g.
+ * 255 fields at lMax = 1024).  The call makes no allocation, copy or host
+ * synchronisation: it queues two kernels on `stream`.  Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU (other streams,
+ * other grids, other threads) race; order them. */
+int gsh_integrate_noalloc(gsh_grid_t grid, int value_kind, int batch, const void* field_a,
+                          const void* field_b, long field_stride, double* out, void* stream);
 /* Seeded coefficients generated on the device (device memory): the stream
  * of gshtrans_amd.random_coefficients, scaled by (l+1)^-decay, with the
  * masks of GridBase.h:115-172 for the grid's lMax. */
