@@ -260,6 +260,99 @@ def traffic_per_launch(family, mode):
     return sum(r["fetch_bytes"] + r["write_bytes"] for r in rows) / len(rows)
 
 
+def host_info():
+    """The host the CPU baseline ran on: logical CPUs, the CPUs this process
+    may use, and the CPU model (from /proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "model": model}
+
+
+def plan_vs_loop(args):
+    """The batching workload: 256 complex fields, lMax drawn log-uniform from
+    [32, 1024], n uniform in -2..2, coefficients normal with (l+1)^-1 decay
+    (seeded), FP32 storage, FP64 arithmetic: one batch plan (every group's
+    synthesis, then every group's analysis, replayed from a HIP graph)
+    against a loop of the plain per-field calls (gsh_inverse / gsh_forward,
+    FP64 storage) over the same fields.  512 transforms per step."""
+    import numpy as np
+    import torch
+    import gshtrans_amd as G
+    rng = np.random.default_rng(SEED + 3)
+    count = 256
+    lmax = np.rint(np.exp(rng.uniform(np.log(32), np.log(1024), count))).astype(int)
+    lmax = np.clip(lmax, 32, 1024)
+    nn = rng.integers(-2, 3, count)
+    grid = G.GaussLegendreGrid(L, NMAX)
+    for n in range(-NMAX, NMAX + 1):
+        grid.prepare(n)
+    sizes = [G.coefficient_size(int(a), int(b)) for a, b in zip(lmax, nn)]
+    cmax = max(sizes)
+    c64 = torch.zeros((count, cmax), dtype=torch.complex128)
+    for i in range(count):
+        c64[i, :sizes[i]] = torch.from_numpy(G.random_coefficients(int(lmax[i]), int(nn[i]),
+                                                                   seed=SEED + 10 + i,
+                                                                   grid_lmax=L))
+    c64 = c64.cuda()
+    plan = G.BatchPlan(grid, lmax, nn, real=False, precision="fp64", storage="fp32")
+    c32 = c64.to(torch.complex64)
+    f32 = torch.empty((count, L + 1, 2 * L), dtype=torch.complex64, device="cuda")
+    o32 = torch.empty_like(c32)
+
+    def plan_step():
+        plan.inverse(c32, field=f32, graph=True)
+        plan.forward(f32, coeffs=o32, graph=True)
+
+    fbuf = torch.empty((1, L + 1, 2 * L), dtype=torch.complex128, device="cuda")
+    obuf = [torch.empty((1, s), dtype=torch.complex128, device="cuda") for s in sizes]
+    cin = [c64[i:i + 1, :sizes[i]].contiguous() for i in range(count)]
+
+    def loop_step():
+        for i in range(count):
+            grid.inverse(int(lmax[i]), int(nn[i]), cin[i], field=fbuf)
+            grid.forward(int(lmax[i]), int(nn[i]), fbuf, coeffs=obuf[i])
+
+    out = {}
+    for name, fn in (("plan_graph", plan_step), ("loop", loop_step)):
+        for _ in range(2):  # tuning, graph capture, warm-up
+            fn()
+        torch.cuda.synchronize()
+        reps = max(2, args.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        out[name] = {"transforms_per_s": 2 * count / dt, "ms_per_step": 1e3 * dt}
+        log(f"[bench] batching workload, {name}: {2 * count / dt:.1f} transforms/s")
+    # The plan's FP32-stored results against the FP64 loop.
+    err = []
+    for i in range(count):
+        a = o32[i, :sizes[i]].to(torch.complex128)
+        b = obuf[i][0]
+        err.append(float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)))
+    info = plan.info()
+    plan.close()
+    return {"workload": "256 complex fields, lMax log-uniform [32, 1024], n in -2..2, FP32 "
+                        "storage, FP64 arithmetic; synthesis + analysis of every field "
+                        "(512 transforms/step)",
+            "plan_graph": out["plan_graph"], "loop_per_field": out["loop"],
+            "speedup_plan_vs_loop": out["plan_graph"]["transforms_per_s"] /
+            out["loop"]["transforms_per_s"],
+            "plan_vs_loop_rel_l2_max": max(err), "groups": len(info)}
+
+
 def cpu_baseline(args):
     """The CPU oracle restating the reference's path: Wigner table built with
     OpenMP on all host cores (setup, reported separately), then serial
@@ -286,7 +379,96 @@ def cpu_baseline(args):
     return {"value": 4.0 / dt, "unit": "transforms/s", "cores": 1, "kind": "port",
             "sample": "lMax=1024: 1 real n=0 + 1 complex n=2 field, synthesis + analysis "
                       "(4 transforms, serial as the reference) with precomputed Wigner tables",
-            "setup_s": setup, "setup_threads": oracle.lib().gsho_num_threads()}
+            "setup_s": setup, "setup_threads": oracle.lib().gsho_num_threads(),
+            "host": host_info()}
+
+
+def config_rows(args):
+    """The CPU-baseline rows of the other workloads, each beside the FP64 GPU
+    figure on the same seeded inputs (one JSON line of its own; not the
+    headline).  CPU: the oracle with its precomputed Wigner table (built with
+    OpenMP on every usable core, reported as setup_s), serial transforms as
+    GaussLegendreGrid.h:163, 281; median of the repetitions.
+      grid64    lMax=64, n=0, one complex field, (l+1)^-1 decay
+      grid512   lMax=512, n=-2..2, one complex field per n, (l+1)^-1.5 decay
+      lsq512    lMax=512, n=1, 16 complex fields: the GPU adjoint of the
+                synthesis (S^H, the least-squares gradient's transform)
+                against the oracle's analysis, its CPU counterpart (the
+                same contraction with quadrature weights)."""
+    import statistics
+    import numpy as np
+    import torch
+    import gshtrans_amd as G
+    import oracle
+
+    def med(fn, reps):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return statistics.median(ts)
+
+    def gpu_med(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return statistics.median(ts)
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    rows = {}
+    for name, Lc, ns, decay in (("grid64", 64, [0], 1.0), ("grid512", 512, [-2, -1, 0, 1, 2], 1.5)):
+        t0 = time.perf_counter()
+        og = oracle.OracleGrid(Lc, 2, table=True)
+        setup = time.perf_counter() - t0
+        g = G.GaussLegendreGrid(Lc, 2)
+        per_n = {}
+        for n in ns:
+            c = G.random_coefficients(Lc, n, seed=SEED + 100 + n, decay=decay)
+            reps = 5 if Lc <= 64 else 3
+            f_cpu = og.inverse(Lc, n, c)
+            a_cpu = og.forward(Lc, n, f_cpu)
+            ts = med(lambda: og.inverse(Lc, n, c), reps)
+            ta = med(lambda: og.forward(Lc, n, f_cpu), reps)
+            cd = torch.from_numpy(c).cuda()
+            fd = g.inverse(Lc, n, cd)
+            ad = g.forward(Lc, n, fd)
+            gs = gpu_med(lambda: g.inverse(Lc, n, cd, field=fd), 10)
+            ga = gpu_med(lambda: g.forward(Lc, n, fd, coeffs=ad), 10)
+            per_n[str(n)] = {
+                "cpu_synthesis_s": ts, "cpu_analysis_s": ta,
+                "gpu_synthesis_s": gs, "gpu_analysis_s": ga,
+                "rel_l2_synthesis_vs_cpu": rel(fd.cpu().numpy(), f_cpu),
+                "rel_l2_analysis_vs_cpu": rel(ad.cpu().numpy(), a_cpu),
+                "round_trip_rel_l2": rel(ad.cpu().numpy(), c)}
+        rows[name] = {"lmax": Lc, "setup_s": setup, "fields": per_n}
+        og.close()
+        g.close()
+        log(f"[bench] {name}: setup {setup:.2f} s")
+    Lc, n, B = 512, 1, 16
+    og = oracle.OracleGrid(Lc, 1, nrange="Single", table=True)
+    g = G.GaussLegendreGrid(Lc, 1)
+    c = G.random_coefficients(Lc, n, seed=SEED + 200, batch=B)
+    f_cpu = og.inverse(Lc, n, c[0])
+    ta = med(lambda: og.forward(Lc, n, f_cpu), 3)
+    fd = g.inverse(Lc, n, torch.from_numpy(c).cuda())
+    adj = g.adjoint(Lc, n, fd)
+    gt = gpu_med(lambda: g.adjoint(Lc, n, fd, coeffs=adj), 10)
+    rows["lsq512"] = {"lmax": Lc, "n": n, "batch": B, "cpu_analysis_s_per_field": ta,
+                      "gpu_adjoint_s_per_field": gt / B,
+                      "gpu_adjoint_batch_s": gt}
+    og.close()
+    g.close()
+    return {"config_rows": rows, "host": host_info(),
+            "setup_threads": oracle.lib().gsho_num_threads()}
 
 
 def copy_probe():
@@ -312,15 +494,25 @@ def main():
                     help="comma-separated reduced-precision modes reported beside the "
                          "headline (default bf16x3,fp32)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batching", action="store_true",
+                    help="skip the batching workload (plan graph replay against a loop of "
+                         "per-field calls)")
     ap.add_argument("--two-streams", action="store_true",
                     help="also time FP64 and the headline mode with the real and complex "
                          "batches on two HIP streams")
+    ap.add_argument("--config-rows", action="store_true",
+                    help="only time the CPU-baseline rows of the other workloads (lMax=64, "
+                         "lMax=512 for n=-2..2, the lMax=512 adjoint) beside the FP64 GPU "
+                         "figures, print them as one JSON line and exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (the "
                          "known-size reference for HBM byte counters) and exit")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.config_rows:
+        print(json.dumps(config_rows(args)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -329,6 +521,7 @@ def main():
         sys.exit(2)
     results, headline, mixed, errors = run_gpu(args)
     rl = roofline(results, headline)
+    batching = None if args.no_batching else plan_vs_loop(args)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args)
     head = results[headline]
     fp64 = results["fp64"]
@@ -361,6 +554,7 @@ def main():
             "own grid handle; value and the per-kernel figures are the one-stream run"))
         if results["two_streams"] else None,
         "rel_l2": errors,
+        "batching": batching,
         "roofline": rl,
         "cpu_baseline": cpu,
     }

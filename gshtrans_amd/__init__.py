@@ -28,6 +28,9 @@ STATUS = {
     9: "GSH_ERR_SCRATCH_LIMIT",
 }
 RANGE = {"All": 0, "NonNegative": 1, "Single": 2}
+# Pointwise field expressions of gsh_forward_expr (gsh_field_expr).
+EXPR = {"product": 1, "sum": 2, "difference": 3, "quotient": 4, "negate": 5, "conj": 6,
+        "real": 7, "imag": 8, "scale": 9, "shift": 10}
 PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16x6": 3}
 STORAGE = {"fp64": 0, "fp32": 1}
 REAL, COMPLEX = 0, 1
@@ -42,7 +45,8 @@ EXPORTS = [
     "gsh_plan_inverse", "gsh_plan_forward", "gsh_plan_info", "gsh_device_alloc",
     "gsh_device_free", "gsh_copy", "gsh_synchronize", "gsh_wigner_size", "gsh_wigner_offset",
     "gsh_wigner_table", "gsh_wigner_table_host", "gsh_forward_product", "gsh_integrate",
-    "gsh_random_coefficients", "gsh_integrate_noalloc",
+    "gsh_random_coefficients", "gsh_integrate_noalloc", "gsh_forward_expr",
+    "gsh_plan_analysis_pass",
 ]
 KERNELS = ["legendre_synthesis", "fft_synthesis", "fft_analysis", "legendre_analysis",
            "coef_split", "bin_split"]
@@ -93,6 +97,7 @@ def _lib():
         L.gsh_timing_enable.argtypes = [vp, i]
         L.gsh_timing_read.argtypes = [vp, dp, ctypes.POINTER(l), i]
         L.gsh_forward_product.argtypes = [vp, i, i, i, i, i, vp, vp, l, vp, l, i, vp]
+        L.gsh_forward_expr.argtypes = [vp, i, i, i, i, i, i, vp, vp, l, d, d, vp, l, i, vp]
         L.gsh_integrate.argtypes = [vp, i, i, vp, vp, l, vp, vp]
         L.gsh_integrate_noalloc.argtypes = [vp, i, i, vp, vp, l, vp, vp]
         L.gsh_random_coefficients.argtypes = [vp, i, i, i, ctypes.c_ulonglong, d, i, vp, l, vp]
@@ -107,7 +112,8 @@ def _lib():
                      "gsh_inverse", "gsh_forward", "gsh_adjoint_inverse",
                      "gsh_inverse_host", "gsh_forward_host", "gsh_timing_enable",
                      "gsh_timing_read", "gsh_wigner_table", "gsh_wigner_table_host",
-                     "gsh_forward_product", "gsh_integrate", "gsh_integrate_noalloc",
+                     "gsh_forward_product", "gsh_forward_expr", "gsh_integrate",
+                     "gsh_integrate_noalloc",
                      "gsh_random_coefficients"):
             getattr(L, name).restype = i
         _LIB = L
@@ -505,6 +511,46 @@ class GaussLegendreGrid:
                "gsh_forward_product")
         return coeffs if field_a.dim() == 3 else coeffs[0]
 
+    def forward_expr(self, lmax, n, op, field_a, field_b=None, scalar=None, coeffs=None,
+                     real=False, precision="fp64", accumulate=False, stream=None):
+        """Analysis of a pointwise expression of fields (the expression
+        fields of CanonicalComponentFieldOverloads.h:20-266), formed inside
+        the ring FFT's loads: op is one of EXPR ("product", "sum",
+        "difference", "quotient", "negate", "conj", "real", "imag", "scale",
+        "shift"); field_b is the second operand of the first four, scalar the
+        operand of "scale" and "shift".  `real` is the kind of the input
+        fields; "real" / "imag" of complex fields give real fields (NonNegative
+        coefficients).  n is the expression's upper index."""
+        import torch
+        code = EXPR[op]
+        binary = code <= EXPR["quotient"]
+        fa = field_a if field_a.dim() == 3 else field_a.unsqueeze(0)
+        B = fa.shape[0]
+        fdt = torch.float64 if real else torch.complex128
+        _check_field(fa, B, self._lmax + 1, 2 * self._lmax, fdt, "forward_expr")
+        fb = None
+        if binary:
+            if field_b is None:
+                raise ValueError(f"forward_expr: {op} needs field_b")
+            fb = field_b if field_b.dim() == 3 else field_b.unsqueeze(0)
+            _check_field(fb, B, self._lmax + 1, 2 * self._lmax, fdt, "forward_expr")
+        s = complex(0.0 if scalar is None else scalar)
+        if real and s.imag != 0:
+            raise ValueError("forward_expr: the scalar of a real field must be real")
+        out_real = real or op in ("real", "imag")
+        C = max(int(_lib().gsh_coefficient_size(lmax, n, REAL if out_real else COMPLEX)), 1)
+        if coeffs is None:
+            coeffs = torch.zeros((B, C), dtype=torch.complex128, device=fa.device)
+        co = coeffs if coeffs.dim() == 2 else coeffs.unsqueeze(0)
+        _check_coeffs(co, B, C, torch.complex128, "forward_expr")
+        _check(_lib().gsh_forward_expr(self._h, lmax, n, REAL if real else COMPLEX, code,
+                                       PRECISION[precision], B, _ptr(fa),
+                                       None if fb is None else _ptr(fb), self.FieldSize(),
+                                       s.real, s.imag, _ptr(co), co.shape[1],
+                                       1 if accumulate else 0, _stream(stream)),
+               "gsh_forward_expr")
+        return coeffs if field_a.dim() == 3 else coeffs[0]
+
     def weights(self, device="cuda"):
         """GridBase::Weights (GridBase.h:60-66) as a (ntheta, nphi) tensor:
         w_i dphi at every point."""
@@ -711,5 +757,11 @@ class BatchPlan:
         arrs = [(ctypes.c_int * cap)() for _ in range(6)]
         _check(_lib().gsh_plan_info(self._h, ctypes.byref(ng), cap, *arrs), "gsh_plan_info")
         keys = ["n", "lmax", "count", "synth_tiles", "fft_inverse", "fft_forward"]
-        return [{k: arrs[j][i] for j, k in enumerate(keys)} for i in range(ng.value)]
+        out = [{k: arrs[j][i] for j, k in enumerate(keys)} for i in range(ng.value)]
+        ap = (ctypes.c_int * cap)()
+        _check(_lib().gsh_plan_analysis_pass(self._h, ctypes.byref(ng), cap, ap),
+               "gsh_plan_analysis_pass")
+        for i, d in enumerate(out):
+            d["analysis_pass"] = ap[i]
+        return out
 

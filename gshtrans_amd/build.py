@@ -55,10 +55,19 @@ FACADE_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_facade.cpp")
 FACADE_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_facade")
 
 
+def cxx23():
+    """A C++23 host compiler (multi-argument operator[], as the reference's
+    headers use): ROCm's clang++."""
+    for c in (os.environ.get("GSH_CXX23"), "/opt/rocm/llvm/bin/clang++", "clang++"):
+        if c and (not os.path.isabs(c) or os.path.exists(c)):
+            return c
+    return "clang++"
+
+
 def build_facade_test(verbose: bool = False) -> str:
     """Compile the C++ facade test (include/GSHTrans) against the library;
     the binary finds the library through an $ORIGIN-relative rpath."""
-    cmd = ["g++", "-std=c++20", "-O1", "-Wall", "-I", os.path.join(ROOT, "include"),
+    cmd = [cxx23(), "-std=c++23", "-O1", "-Wall", "-I", os.path.join(ROOT, "include"),
            FACADE_TEST_SRC, "-o", FACADE_TEST_BIN + ".tmp", "-L", HERE, "-lgshtrans_hip",
            "-Wl,-rpath,$ORIGIN/../../gshtrans_amd", "-Wl,--allow-shlib-undefined"]
     if verbose:

@@ -12,8 +12,12 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <limits>
 #include <map>
 #include <memory>
@@ -127,6 +131,68 @@ double ortho_scale(long l, double v) {
   return factor * std::sqrt(static_cast<double>(2 * l + 1)) * v;
 }
 
+// log2 |WignerMinOrder(l, n)| and its sign (false: exactly zero), in long
+// double: the closed form of min_order without the final exp, so values far
+// below the double range keep their exponent.
+bool min_order_log2(long l, long n, long double lsin, long double lcos, bool atLeft,
+                    bool atRight, long double* lg2) {
+  if (l == 0 || (atLeft && n == -l) || (atRight && n == l)) {
+    *lg2 = 0.0L;
+    return true;
+  }
+  if (atLeft || atRight) return false;
+  const long double Fl = l, Fn = n;
+  const long double ln = 0.5L * (std::lgamma(2 * Fl + 1) - std::lgamma(Fl - Fn + 1) -
+                                 std::lgamma(Fl + Fn + 1)) +
+                         (Fl + Fn) * lsin + (Fl - Fn) * lcos;
+  *lg2 = ln / std::log(2.0L);
+  return true;
+}
+
+// The Ortho-normalised seed of order m as s * 2^-k with s a normal double:
+// k = 0 and s = seed (the double value every kernel starts from) while the
+// seed is at least 2^-960; below that, where the double expression gives a
+// subnormal or zero, the same closed form in long double logarithms, scaled
+// to 2^-960 <= |s| < 2^-959.  Seeds of exactly zero (the pole cases) stay 0.
+void seed_extended(long m, long n, double theta, double seed, double* s, int* k) {
+  *s = seed;
+  *k = 0;
+  if (seed == 0.0 ? false : std::fabs(seed) >= 0x1p-960) return;
+  const long double half = static_cast<long double>(theta) / 2;
+  const long double sh = std::sin(half), ch = std::cos(half);
+  const bool atLeft = sh < std::numeric_limits<double>::min();
+  const bool atRight = ch < std::numeric_limits<double>::min();
+  const long double lsin = atLeft ? 0.0L : std::log(sh), lcos = atRight ? 0.0L : std::log(ch);
+  const long nabs = std::labs(n), mabs = std::labs(m);
+  long l, nn;
+  double sign = 1.0;
+  if (mabs <= nabs) {  // seed_value's choice of closed form
+    l = nabs;
+    if (n >= 0) {
+      nn = -m;
+    } else {
+      sign = sign_pow(-m + l);
+      nn = m;
+    }
+  } else {
+    l = mabs;
+    if (m < 0) {
+      nn = n;
+    } else {
+      sign = sign_pow(n + l);
+      nn = -n;
+    }
+  }
+  long double lg2;
+  if (!min_order_log2(l, nn, lsin, lcos, atLeft, atRight, &lg2)) return;  // exactly zero
+  const long double factor = 0.5L / std::sqrt(3.14159265358979323846264338327950288L);
+  lg2 += std::log2(factor * std::sqrt(static_cast<long double>(2 * l + 1)));
+  const long double e = std::floor(lg2);
+  if (e >= -960.0L) return;  // normal after all: keep the double seed
+  *k = static_cast<int>(std::min<long double>(-960.0L - e, 1e9L));
+  *s = sign * std::ldexp(static_cast<double>(std::exp2(lg2 - e)), -960);
+}
+
 // Recurrence coefficients {a, b, c} of the Ortho-normalised values of order
 // m at degree l > l0 = max(|m|, |n|), in long double:
 //   d_l = [(2l-1)(l(l-1)cos - m n) d_{l-1} - l sqrt(((l-1)^2-n^2)((l-1)^2-m^2)) d_{l-2}]
@@ -169,6 +235,12 @@ void parallel_for(long n, F f) {
 // ---------------------------------------------------------------------------
 struct NData {
   double* seeds = nullptr;  // [(m+L)*ntheta + t]
+  // Seeds below the normal range in extended form (see seed_extended), or
+  // null when there are none: sseed * 2^-skexp.
+  double* sseed = nullptr;
+  int* skexp = nullptr;
+  bool ck_restart = false;  // some such order rises to significance: the FP64
+                            // synthesis restarts from the checkpoints
   double4* rec = nullptr;  // {a, b, c, 0} per (m, l), see gsh_internal.h
   long* roff = nullptr;
   // Recurrence checkpoints at the start of every analysis pass k >= 1
@@ -249,14 +321,20 @@ int ensure_ndata(gsh_grid_s* g, int n, NData** out) {
   const long nabs = std::labs(n);
   const long norders = 2L * L + 1;
   // Seeds, Ortho-normalised.
-  std::vector<double> seeds(norders * nt);
+  std::vector<double> seeds(norders * nt), sseed(norders * nt);
+  std::vector<int> skexp(norders * nt);
+  std::vector<char> any_scaled(nt, 0);
   parallel_for(nt, [&](long t) {
     SeedArgs a(g->theta[t]);
     for (long m = -L; m <= L; m++) {
       const long l0 = std::max(std::labs(m), nabs);
-      seeds[(m + L) * nt + t] = ortho_scale(l0, seed_value(m, n, a));
+      const long i = (m + L) * nt + t;
+      seeds[i] = ortho_scale(l0, seed_value(m, n, a));
+      seed_extended(m, n, g->theta[t], seeds[i], &sseed[i], &skexp[i]);
+      if (skexp[i] > 0) any_scaled[t] = 1;
     }
   });
+  const bool scaled = std::find(any_scaled.begin(), any_scaled.end(), 1) != any_scaled.end();
   // Recurrence coefficients per (m, l), l0 < l <= L (rec_coef).
   std::vector<long> roff(norders);
   long total = 0;
@@ -277,17 +355,25 @@ int ensure_ndata(gsh_grid_s* g, int n, NData** out) {
     if (d.seeds) (void)hipFree(d.seeds);
     if (d.rec) (void)hipFree(d.rec);
     if (d.roff) (void)hipFree(d.roff);
+    if (d.sseed) (void)hipFree(d.sseed);
+    if (d.skexp) (void)hipFree(d.skexp);
     return status;
   };
   if (hipMalloc(&d.seeds, seeds.size() * sizeof(double)) != hipSuccess ||
       hipMalloc(&d.rec, total * sizeof(double4)) != hipSuccess ||
-      hipMalloc(&d.roff, norders * sizeof(long)) != hipSuccess)
+      hipMalloc(&d.roff, norders * sizeof(long)) != hipSuccess ||
+      (scaled && (hipMalloc(&d.sseed, sseed.size() * sizeof(double)) != hipSuccess ||
+                  hipMalloc(&d.skexp, skexp.size() * sizeof(int)) != hipSuccess)))
     return release(GSH_ERR_OUT_OF_MEMORY);
   if (hipMemcpy(d.seeds, seeds.data(), seeds.size() * sizeof(double), hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMemcpy(d.rec, rec.data(), total * sizeof(double4), hipMemcpyHostToDevice) !=
           hipSuccess ||
-      hipMemcpy(d.roff, roff.data(), norders * sizeof(long), hipMemcpyHostToDevice) != hipSuccess)
+      hipMemcpy(d.roff, roff.data(), norders * sizeof(long), hipMemcpyHostToDevice) != hipSuccess ||
+      (scaled && (hipMemcpy(d.sseed, sseed.data(), sseed.size() * sizeof(double),
+                            hipMemcpyHostToDevice) != hipSuccess ||
+                  hipMemcpy(d.skexp, skexp.data(), skexp.size() * sizeof(int),
+                            hipMemcpyHostToDevice) != hipSuccess)))
     return release(GSH_ERR_DEVICE);
   auto res = g->ndata.emplace(n, d);
   *out = &res.first->second;
@@ -395,10 +481,21 @@ int ensure_ck(gsh_grid_s* g, NData* d, int n, hipStream_t s) {
   }
   double2* ck = nullptr;
   long* ckoff = nullptr;
-  if (hipMalloc(&ck, std::max(rows, 1L) * nt * sizeof(double2)) != hipSuccess)
-    return GSH_ERR_OUT_OF_MEMORY;
-  GSH_HIP(hipMalloc(&ckoff, off.size() * sizeof(long)));
-  GSH_HIP(hipMemcpy(ckoff, off.data(), off.size() * sizeof(long), hipMemcpyHostToDevice));
+  int* flag = nullptr;
+  auto release = [&](int status) {
+    if (ck) (void)hipFree(ck);
+    if (ckoff) (void)hipFree(ckoff);
+    if (flag) (void)hipFree(flag);
+    return status;
+  };
+  if (hipMalloc(&ck, std::max(rows, 1L) * nt * sizeof(double2)) != hipSuccess ||
+      hipMalloc(&ckoff, off.size() * sizeof(long)) != hipSuccess ||
+      (d->sseed && hipMalloc(&flag, sizeof(int)) != hipSuccess))
+    return release(GSH_ERR_OUT_OF_MEMORY);
+  if (hipMemcpy(ckoff, off.data(), off.size() * sizeof(long), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      (flag && hipMemset(flag, 0, sizeof(int)) != hipSuccess))
+    return release(GSH_ERR_DEVICE);
   LegParams p{};
   p.L = L;
   p.ntheta = nt;
@@ -410,19 +507,33 @@ int ensure_ck(gsh_grid_s* g, NData* d, int n, hipStream_t s) {
   p.roff = d->roff;
   p.ck = ck;
   p.ckoff = ckoff;
-  GSH_HIP(gsh::launch_rec_checkpoints(p, s));
-  GSH_HIP(hipStreamSynchronize(s));
+  p.sseed = d->sseed;
+  p.skexp = d->skexp;
+  p.ck_flag = flag;
+  if (gsh::launch_rec_checkpoints(p, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+    return release(GSH_ERR_DEVICE);
+  int rose = 0;
+  if (flag && hipMemcpy(&rose, flag, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    return release(GSH_ERR_DEVICE);
+  if (flag) (void)hipFree(flag);
   d->ck = ck;
   d->ckoff = ckoff;
+  d->ck_restart = rose != 0;
   return GSH_OK;
 }
+
+constexpr int kMaxGridDegree = 2048;
+// Degrees per pass of the split-BF16 analysis when neither a plan's tuning
+// nor the caller chose: gsh::kWideAnalPass (128) or gsh::kPass (64).
+constexpr int kDefaultAnalPass = gsh::kWideAnalPass;
 
 int order_of_host(int mw, int real) { return real ? mw : ((mw & 1) ? -((mw + 1) >> 1) : (mw >> 1)); }
 
 // Work list of an analysis launch: items (mw, pass | group << 16) ordered by
 // order, group, pass (the passes of one order and group share their input
 // blocks), padded to a multiple of 8 with (~0u, 0); glmax[g] bounds the
-// degrees of group g (empty: lmax for every group).
+// degrees of group g (empty: lmax for every group).  Passes are kPass
+// degrees, or kWideAnalPass for family 3 (the 128-degree split-BF16 kernel).
 int ensure_worklist(gsh_grid_s* g, int family, int lmax, int n, int real, int ngroups,
                     const std::vector<int>& glmax, WorkList* out) {
   auto key = std::make_tuple(family, lmax, n, real,
@@ -440,7 +551,8 @@ int ensure_worklist(gsh_grid_s* g, int family, int lmax, int n, int real, int ng
     for (int grp = 0; grp < ngroups; grp++) {
       const int lw = glmax.empty() ? lmax : glmax[grp];
       if (l0 > lw) continue;
-      const int np = (lw - l0 + gsh::kPass) / gsh::kPass;
+      const int plen = family == 3 ? gsh::kWideAnalPass : gsh::kPass;  // degrees per pass
+      const int np = (lw - l0 + plen) / plen;
       for (int k = 0; k < np; k++)
         items.push_back(make_uint2(static_cast<unsigned>(mw),
                                    static_cast<unsigned>(k) | (static_cast<unsigned>(grp) << 16)));
@@ -693,8 +805,13 @@ int gsh_grid_create(int lmax, int nmax, int mrange, int nrange, gsh_grid_t* grid
   if (mrange != GSH_ALL && mrange != GSH_NONNEGATIVE) return GSH_ERR_INVALID_ARGUMENT;
   if (nrange != GSH_ALL && nrange != GSH_NONNEGATIVE && nrange != GSH_SINGLE)
     return GSH_ERR_INVALID_ARGUMENT;
-  if (lmax > 0 && fft_length(2 * lmax) > 4096) return GSH_ERR_UNSUPPORTED;
-  if ((lmax + 1 + 63) / 64 > 36) return GSH_ERR_UNSUPPORTED;
+  // Grids up to lMax = 2048 (the reference's constructor has no cap).  The
+  // LDS ring FFT transforms up to 8192 points, so every ring length up to
+  // 4096 runs, power of two or through Bluestein's convolution; past 2048 the
+  // recurrence checkpoint table of one upper index, about (L + 1)^3 bytes,
+  // grows from 8.6 GB (L = 2048) to 69 GB (L = 4096).
+  if (lmax > kMaxGridDegree) return GSH_ERR_UNSUPPORTED;
+  if (lmax > 0 && fft_length(2 * lmax) > 8192) return GSH_ERR_UNSUPPORTED;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return GSH_ERR_NO_DEVICE;
   int dev = 0;
@@ -807,6 +924,8 @@ int gsh_grid_destroy(gsh_grid_t g) {
     (void)hipFree(kv.second.roff);
     if (kv.second.ck) (void)hipFree(kv.second.ck);
     if (kv.second.ckoff) (void)hipFree(kv.second.ckoff);
+    if (kv.second.sseed) (void)hipFree(kv.second.sseed);
+    if (kv.second.skexp) (void)hipFree(kv.second.skexp);
   }
   for (auto& kv : g->worklists) (void)hipFree(kv.second.d);
   (void)hipFree(g->d_cos);
@@ -867,7 +986,11 @@ struct RunOpts {
   int f32 = 0;                   // fields stored in FP32
   int synth_tiles = 0;           // FP64 synthesis: 16-column tiles per workgroup (0: auto)
   int fft = 0;                   // 0: auto, 1: wavefront-per-ring kernel, 2: LDS kernel
-  const void* field2 = nullptr;  // analysis of the pointwise product field * field2
+  int anal_pass = 0;             // split-BF16 analysis degrees per pass: 64, 128 (0: auto)
+  const void* field2 = nullptr;  // analysis of a pointwise expression: second operand
+  int op = 0;                    // gsh::ExprOp applied as the analysis rings load
+  int src_cplx = 0;              // real analysis of Re / Im of a complex field
+  double2 scalar = {0.0, 0.0};   // scalar operand of the expression
 };
 
 bool use_wave_fft(const FFTParams& q, const RunOpts& o) {
@@ -891,6 +1014,16 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
   p.cg_lmax = o.cg_lmax;
   const int ntsb = (g->ntheta() + 255) / 256;
   if (precision == GSH_FP64 || precision == GSH_FP32) {
+    // Orders whose seeds underflow but whose values rise to significance
+    // before L: the synthesis takes its state from the scaled checkpoints.
+    if (d->sseed) {
+      if ((st = ensure_ck(g, d, n, s))) return st;
+      if (d->ck_restart) {
+        p.ck = d->ck;
+        p.ckoff = d->ckoff;
+        p.ck_restart = 1;
+      }
+    }
     const int T = o.synth_tiles == 1 || o.synth_tiles == 2 ? o.synth_tiles : (p.ncol > 16 ? 2 : 1);
     WorkList w;
     if ((st = ensure_synth_worklist(g, 10 + T, lmax, real, (p.ncol + 16 * T - 1) / (16 * T), ntsb,
@@ -986,6 +1119,9 @@ int run_forward(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
   FFTParams q = fft_params(g, lmax, real, batch);
   q.fin = field;
   q.fin2 = o.field2;
+  q.op = o.op;
+  q.src_cplx = o.src_cplx;
+  q.scalar = o.scalar;
   q.fstride = fstride;
   q.f32 = o.f32;
   q.fmap = o.fmap;
@@ -1027,6 +1163,8 @@ int run_forward(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
     }));
     return GSH_OK;
   }
+  // 128-degree passes unless asked for 64 (o.anal_pass; 0 takes the default).
+  const bool wide = (o.anal_pass ? o.anal_pass : kDefaultAnalPass) == gsh::kWideAnalPass;
   {
     // 128-column super-groups of the split-BF16 kernel.
     const int nsg = (ncg16 + 7) / 8;
@@ -1037,13 +1175,13 @@ int run_forward(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
       sg.push_back(mx);
     }
     WorkList w;
-    if ((st = ensure_worklist(g, 1, lmax, n, real, nsg, sg, &w))) return st;
+    if ((st = ensure_worklist(g, wide ? 3 : 1, lmax, n, real, nsg, sg, &w))) return st;
     p.items = w.d;
     p.items_per = w.per;
   }
   const int ns = precision == GSH_BF16X6 ? 3 : 2;
   GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_ANALYSIS, s,
-                [&] { return gsh::launch_leg_anal_bf16(p, ns, s); }));
+                [&] { return gsh::launch_leg_anal_bf16(p, ns, s, wide ? 1 : 0); }));
   return GSH_OK;
 }
 
@@ -1203,20 +1341,25 @@ int gsh_random_coefficients(gsh_grid_t g, int lmax, int n, int value_kind,
   return GSH_OK;
 }
 
-// Analysis of the pointwise product of two fields (the product field of
-// CanonicalComponentFieldBinary.h:203-253 and CanonicalComponentFieldOverloads.h:
-// 207-233, upper index n = N0 + N1), the product formed as the ring FFT loads
-// its rings.
-int gsh_forward_product(gsh_grid_t g, int lmax, int n, int value_kind, int precision, int batch,
-                        const void* field_a, const void* field_b, long field_stride,
-                        void* coeffs, long coeff_stride, int accumulate, void* stream) {
+// Analysis of a pointwise expression of one or two fields, formed as the
+// ring FFT loads its rings (the expression fields of
+// CanonicalComponentFieldBinary.h:17-308 and CanonicalComponentFieldUnary.h:
+// 17-357, built by CanonicalComponentFieldOverloads.h:20-266).  value_kind is
+// the input fields' kind; Re and Im of complex fields are real fields.
+int gsh_forward_expr(gsh_grid_t g, int lmax, int n, int value_kind, int op, int precision,
+                     int batch, const void* field_a, const void* field_b, long field_stride,
+                     double scalar_re, double scalar_im, void* coeffs, long coeff_stride,
+                     int accumulate, void* stream) {
   int st = check_call(g, lmax, n, value_kind, precision, GSH_STORE_FP64, batch);
   if (st) return st;
+  if (op < GSH_EXPR_PRODUCT || op > GSH_EXPR_SHIFT) return GSH_ERR_INVALID_ARGUMENT;
   if (batch == 0) return GSH_OK;
-  if (!coeffs || !field_a || !field_b) return GSH_ERR_INVALID_ARGUMENT;
+  if (!coeffs || !field_a || (gsh::expr_binary(op) && !field_b)) return GSH_ERR_INVALID_ARGUMENT;
+  if (value_kind == GSH_REAL && scalar_im != 0.0) return GSH_ERR_INVALID_ARGUMENT;
   if (lmax == 0) return GSH_ERR_UNSUPPORTED;  // the reference's lMax = 0 branch reads one value
-  const int real = value_kind == GSH_REAL;
-  const long csize = gsh_coefficient_size(lmax, n, value_kind);
+  const bool src_cplx = value_kind == GSH_COMPLEX && (op == GSH_EXPR_REAL || op == GSH_EXPR_IMAG);
+  const int real = value_kind == GSH_REAL || src_cplx;
+  const long csize = gsh_coefficient_size(lmax, n, real ? GSH_REAL : GSH_COMPLEX);
   const long fsize = static_cast<long>(g->ntheta()) * g->nphi();
   if (batch > 1 && (coeff_stride < csize || field_stride < fsize)) return GSH_ERR_INVALID_ARGUMENT;
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -1225,9 +1368,26 @@ int gsh_forward_product(gsh_grid_t g, int lmax, int n, int value_kind, int preci
   NData* d;
   if ((st = ensure_ndata(g, n, &d))) return st;
   RunOpts o;
-  o.field2 = field_b;
+  o.field2 = gsh::expr_binary(op) ? field_b : nullptr;
+  o.op = op;
+  o.src_cplx = src_cplx ? 1 : 0;
+  o.scalar = make_double2(scalar_re, scalar_im);
+  // Re / Im of complex fields: the wavefront FFT reads two complex samples
+  // per real pair; the LDS kernels handle every form.
   return run_forward(g, d, lmax, n, real, precision, batch, field_a, field_stride,
                      static_cast<double*>(coeffs), coeff_stride, accumulate, 0, s, o);
+}
+
+// Analysis of the pointwise product of two fields (the product field of
+// CanonicalComponentFieldBinary.h:203-253 and CanonicalComponentFieldOverloads.h:
+// 207-233, upper index n = N0 + N1), the product formed as the ring FFT loads
+// its rings.
+int gsh_forward_product(gsh_grid_t g, int lmax, int n, int value_kind, int precision, int batch,
+                        const void* field_a, const void* field_b, long field_stride,
+                        void* coeffs, long coeff_stride, int accumulate, void* stream) {
+  return gsh_forward_expr(g, lmax, n, value_kind, GSH_EXPR_PRODUCT, precision, batch, field_a,
+                          field_b, field_stride, 0.0, 0.0, coeffs, coeff_stride, accumulate,
+                          stream);
 }
 
 int gsh_timing_enable(gsh_grid_t g, int enable) {
@@ -1354,6 +1514,7 @@ struct PlanGroup {
   int synth_tiles = 0;  // FP64 synthesis column tiles (0: not tuned)
   int fft_inv = 0;      // 1: wavefront FFT, 2: LDS FFT (0: not tuned)
   int fft_fwd = 0;
+  int anal_pass = 0;    // split-BF16 analysis degrees per pass, 64 or 128 (0: not tuned)
 };
 
 struct GraphKey {
@@ -1457,6 +1618,7 @@ RunOpts group_opts(const gsh_plan_s* P, const PlanGroup& G, int dir) {
   o.f32 = P->storage == GSH_STORE_FP32;
   o.synth_tiles = G.synth_tiles;
   o.fft = dir == 0 ? G.fft_inv : G.fft_fwd;
+  o.anal_pass = dir == 0 ? 0 : G.anal_pass;
   return o;
 }
 
@@ -1504,8 +1666,73 @@ int run_plan(gsh_plan_s* P, int dir, const void* in, long istride, void* out, lo
   return GSH_OK;
 }
 
-// Autotuner: per group, run each candidate (FP64 synthesis tiles x FFT family)
-// once to warm up and once timed with events, and keep the fastest.
+// Tuned choices by launch shape: the key names the grid degree, value kind,
+// precision, storage, direction, upper index, field count and the fields'
+// degrees (hashed, in the group's order), so a later plan of the same shape
+// takes the choices without timing.  Kept for the life of the process and,
+// when the environment variable GSH_TUNE_CACHE names a file, read from and
+// appended to that file (one "key tiles fft pass" line per shape;
+// unreadable lines are skipped).
+std::mutex tune_mu;
+// Choices per shape: synthesis tiles, FFT family, analysis pass length.
+struct TuneChoice {
+  int tiles = 0, fft = 0, pass = 0;
+};
+std::map<std::string, TuneChoice> tune_memo;
+std::string tune_file_loaded;
+
+std::string tune_key(const gsh_plan_s* P, const PlanGroup& G, int dir) {
+  uint64_t h = 1469598103934665603ULL;  // FNV-1a over the degrees
+  for (int l : G.flmax) {
+    h ^= static_cast<uint64_t>(l) & 0xffffffffULL;
+    h *= 1099511628211ULL;
+  }
+  std::ostringstream k;
+  k << "gsh2:L" << P->g->L << ":k" << P->kind << ":p" << P->precision << ":s" << P->storage
+    << ":d" << dir << ":n" << G.n << ":c" << G.count << ":h" << std::hex << h;
+  return k.str();
+}
+
+// Caller holds tune_mu.
+void tune_load_file() {
+  const char* path = std::getenv("GSH_TUNE_CACHE");
+  if (!path || !*path || tune_file_loaded == path) return;
+  tune_file_loaded = path;
+  std::ifstream f(path);
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    std::string key;
+    TuneChoice c{-1, -1, -1};
+    if ((is >> key >> c.tiles >> c.fft >> c.pass) && key.rfind("gsh2:", 0) == 0 &&
+        c.tiles >= 0 && c.tiles <= 2 && c.fft >= 1 && c.fft <= 2 &&
+        (c.pass == 0 || c.pass == gsh::kPass || c.pass == gsh::kWideAnalPass))
+      tune_memo[key] = c;
+  }
+}
+
+bool tune_lookup(const std::string& key, TuneChoice* out) {
+  std::lock_guard<std::mutex> lock(tune_mu);
+  tune_load_file();
+  auto it = tune_memo.find(key);
+  if (it == tune_memo.end()) return false;
+  *out = it->second;
+  return true;
+}
+
+void tune_store(const std::string& key, const TuneChoice& v) {
+  std::lock_guard<std::mutex> lock(tune_mu);
+  tune_memo[key] = v;
+  const char* path = std::getenv("GSH_TUNE_CACHE");
+  if (!path || !*path) return;
+  std::ofstream f(path, std::ios::app);
+  if (f) f << key << " " << v.tiles << " " << v.fft << " " << v.pass << "\n";
+}
+
+// Autotuner: per group, run each candidate (FP64 synthesis tiles, FFT family,
+// and for the split-BF16 analysis 64- or 128-degree passes) once to warm up
+// and once timed with events, and keep the fastest; a shape tuned before
+// (this process, or the GSH_TUNE_CACHE file) is not timed again.
 int tune_plan(gsh_plan_s* P, int dir, const void* in, long istride, void* out, long ostride,
               hipStream_t s) {
   int st;
@@ -1524,18 +1751,34 @@ int tune_plan(gsh_plan_s* P, int dir, const void* in, long istride, void* out, l
   const bool was_timing = P->g->timing;
   P->g->timing = false;
   for (PlanGroup& G : P->groups) {
-    std::vector<std::pair<int, int>> cands;  // (tiles, fft)
+    const std::string key = tune_key(P, G, dir);
+    TuneChoice known;
+    if (tune_lookup(key, &known)) {
+      if (dir == 0) {
+        G.synth_tiles = known.tiles;
+        G.fft_inv = known.fft;
+      } else {
+        G.fft_fwd = known.fft;
+        G.anal_pass = known.pass;
+      }
+      continue;
+    }
+    std::vector<TuneChoice> cands;
     const bool wave_ok = P->g->d_twa != nullptr;
-    const bool tiles2 =
-        dir == 0 && (P->precision == GSH_FP64 || P->precision == GSH_FP32) && G.count > 8;
+    const bool fp = P->precision == GSH_FP64 || P->precision == GSH_FP32;
+    const bool tiles2 = dir == 0 && fp && G.count > 8;
+    const std::vector<int> passes =
+        dir == 1 && !fp ? std::vector<int>{gsh::kPass, gsh::kWideAnalPass} : std::vector<int>{0};
     for (int t : tiles2 ? std::vector<int>{1, 2} : std::vector<int>{0})
-      for (int f : wave_ok ? std::vector<int>{1, 2} : std::vector<int>{2}) cands.emplace_back(t, f);
+      for (int f : wave_ok ? std::vector<int>{1, 2} : std::vector<int>{2})
+        for (int pl : passes) cands.push_back(TuneChoice{t, f, pl});
     float best = 1e30f;
-    std::pair<int, int> pick = cands[0];
+    TuneChoice pick = cands[0];
     for (const auto& c : cands) {
       RunOpts o = group_opts(P, G, dir);
-      o.synth_tiles = c.first;
-      o.fft = c.second;
+      o.synth_tiles = c.tiles;
+      o.fft = c.fft;
+      o.anal_pass = c.pass;
       for (int rep = 0; rep < 2; rep++) {
         if (rep == 1) GSH_HIP(hipEventRecord(e0, s));
         if ((st = run_group(P, G, dir, cin, cout, dir == 0 ? istride : ostride, in, out,
@@ -1554,11 +1797,13 @@ int tune_plan(gsh_plan_s* P, int dir, const void* in, long istride, void* out, l
       }
     }
     if (dir == 0) {
-      G.synth_tiles = pick.first;
-      G.fft_inv = pick.second;
+      G.synth_tiles = pick.tiles;
+      G.fft_inv = pick.fft;
     } else {
-      G.fft_fwd = pick.second;
+      G.fft_fwd = pick.fft;
+      G.anal_pass = pick.pass;
     }
+    tune_store(key, pick);
   }
   P->g->timing = was_timing;
   (void)hipEventDestroy(e0);
@@ -1744,6 +1989,14 @@ int gsh_plan_inverse(gsh_plan_t P, const void* coeffs, long coeff_stride, void* 
 int gsh_plan_forward(gsh_plan_t P, const void* field, long field_stride, void* coeffs,
                      long coeff_stride, int graph, void* stream) {
   return plan_exec(P, 1, field, field_stride, coeffs, coeff_stride, graph, stream);
+}
+
+int gsh_plan_analysis_pass(gsh_plan_t P, int* ngroups, int cap, int* pass) {
+  if (!P || !ngroups || !P->g) return GSH_ERR_INVALID_ARGUMENT;
+  *ngroups = static_cast<int>(P->groups.size());
+  for (int i = 0; i < cap && i < *ngroups; i++)
+    if (pass) pass[i] = P->groups[i].anal_pass;
+  return GSH_OK;
 }
 
 int gsh_plan_info(gsh_plan_t P, int* ngroups, int cap, int* n, int* lmax, int* count,

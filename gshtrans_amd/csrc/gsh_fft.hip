@@ -5,7 +5,7 @@
 // transform of the order bins F_m placed at k = m (m >= 0) and k = N + m
 // (m < 0), and an unnormalised c2c/r2c forward transform of each ring.  Here
 // each workgroup loads `rings` rings of N = 2L points into LDS (64 KiB of
-// complex values), runs a Stockham radix-4 (+ one radix-2) autosort FFT in
+// complex values, 128 KiB for transforms of 8192 points), runs a Stockham radix-4 (+ one radix-2) autosort FFT in
 // place (values go through registers between barriers), and writes the rings
 // back.  A ring length that is not a power of two is transformed with
 // Bluestein's chirp-z identity jk = (j^2 + k^2 - (k-j)^2)/2: chirp multiply,
@@ -19,7 +19,12 @@ namespace gsh {
 namespace {
 
 constexpr int kFFTThreads = 256;
-constexpr int kFFTElems = 4096;  // complex values per workgroup
+// Complex values per workgroup: CAP = 4096 (64 KiB of LDS) holds every ring
+// of up to 4096 points, or its Bluestein convolution, several rings at a
+// time; CAP = 8192 (128 KiB) takes the 8192-point transforms of the longer
+// rings, one at a time.
+constexpr int kFFTElems = 4096;
+constexpr int kFFTElemsMax = 8192;
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -37,7 +42,7 @@ __device__ __forceinline__ double2 cmuli(double2 a, int s) {
 
 // Stockham autosort passes over `rings` rings of length N held in buf
 // (ring q at buf[q*N]).  sign = -1 forward, +1 backward.
-template <int SIGN>
+template <int SIGN, int CAP>
 __device__ void stockham(double2* buf, int N, int rings, const double2* __restrict__ tw) {
   int logN = 31 - __clz(N);
   int Ns = 1;
@@ -45,7 +50,7 @@ __device__ void stockham(double2* buf, int N, int rings, const double2* __restri
   while (logN >= 2) {
     const int quarter = N >> 2;
     const int total = rings * quarter;  // butterflies
-    constexpr int kPer = kFFTElems / 4 / kFFTThreads;
+    constexpr int kPer = CAP / 4 / kFFTThreads;
     double2 v[kPer][4];
     int dst[kPer];
 #pragma unroll
@@ -94,7 +99,7 @@ __device__ void stockham(double2* buf, int N, int rings, const double2* __restri
   if (logN == 1) {
     const int half = N >> 1;
     const int total = rings * half;
-    constexpr int kPer = kFFTElems / 2 / kFFTThreads;
+    constexpr int kPer = CAP / 2 / kFFTThreads;
     double2 v[kPer][2];
     int dst[kPer];
 #pragma unroll
@@ -131,11 +136,11 @@ __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -
 
 // Unnormalised DFT of length N of each ring (ring q at buf[q*M], zero from N
 // to M when Bluestein is used).  SIGN = -1 forward, +1 backward.
-template <int SIGN>
+template <int SIGN, int CAP>
 __device__ void ring_dft(double2* buf, const FFTParams& p) {
   const int N = p.N, M = p.M, R = p.rings;
   if (M == N) {
-    stockham<SIGN>(buf, N, R, p.tw);
+    stockham<SIGN, CAP>(buf, N, R, p.tw);
     return;
   }
   for (int idx = threadIdx.x; idx < R * N; idx += kFFTThreads) {
@@ -144,12 +149,12 @@ __device__ void ring_dft(double2* buf, const FFTParams& p) {
     buf[q * M + j] = cmul(buf[q * M + j], w);
   }
   __syncthreads();
-  stockham<-1>(buf, M, R, p.tw);
+  stockham<-1, CAP>(buf, M, R, p.tw);
   const double2* h = SIGN < 0 ? p.hf : p.hb;
   for (int idx = threadIdx.x; idx < R * M; idx += kFFTThreads)
     buf[idx] = cmul(buf[idx], h[idx & (M - 1)]);
   __syncthreads();
-  stockham<+1>(buf, M, R, p.tw);
+  stockham<+1, CAP>(buf, M, R, p.tw);
   for (int idx = threadIdx.x; idx < R * N; idx += kFFTThreads) {
     const int q = idx / N, j = idx - q * N;
     const double2 w = SIGN < 0 ? p.chirp[j] : cconj(p.chirp[j]);
@@ -159,8 +164,9 @@ __device__ void ring_dft(double2* buf, const FFTParams& p) {
 }
 
 // Synthesis: bins from F, backward FFT, write field rings.
+template <int CAP>
 __global__ __launch_bounds__(kFFTThreads) void fft_synth(FFTParams p) {
-  __shared__ double2 buf[kFFTElems];
+  __shared__ double2 buf[CAP];
   const int theta = blockIdx.x / p.nfg;
   const int fg = blockIdx.x % p.nfg;
   const int N = p.N, M = p.M, R = p.rings;
@@ -197,7 +203,7 @@ __global__ __launch_bounds__(kFFTThreads) void fft_synth(FFTParams p) {
     buf[q * M + k] = X;
   }
   __syncthreads();
-  ring_dft<+1>(buf, p);
+  ring_dft<+1, CAP>(buf, p);
   for (int idx = threadIdx.x; idx < R * N; idx += kFFTThreads) {
     const int q = idx / N;
     const int j = idx - q * N;
@@ -217,8 +223,9 @@ __global__ __launch_bounds__(kFFTThreads) void fft_synth(FFTParams p) {
 }
 
 // Analysis: load rings, forward FFT, scatter weighted bins into F.
+template <int CAP>
 __global__ __launch_bounds__(kFFTThreads) void fft_anal(FFTParams p) {
-  __shared__ double2 buf[kFFTElems];
+  __shared__ double2 buf[CAP];
   const int theta = blockIdx.x / p.nfg;
   const int fg = blockIdx.x % p.nfg;
   const int N = p.N, M = p.M, R = p.rings;
@@ -229,36 +236,31 @@ __global__ __launch_bounds__(kFFTThreads) void fft_anal(FFTParams p) {
     double2 x = make_double2(0.0, 0.0);
     if (f < p.nfield && j < N) {
       const long o = (long)(p.fmap ? p.fmap[f] : f) * p.fstride + (long)theta * N + j;
-      if (p.real && p.f32) {
-        x.x = static_cast<const float*>(p.fin)[o];
-      } else if (p.real) {
-        x.x = static_cast<const double*>(p.fin)[o];
-      } else if (p.f32) {
-        const float2 y = static_cast<const float2*>(p.fin)[o];
-        x = make_double2(y.x, y.y);
+      auto load = [&](const void* b) {
+        if (p.real && p.f32) return make_double2(static_cast<const float*>(b)[o], 0.0);
+        if (p.real) return make_double2(static_cast<const double*>(b)[o], 0.0);
+        if (p.f32) {
+          const float2 y = static_cast<const float2*>(b)[o];
+          return make_double2(y.x, y.y);
+        }
+        return static_cast<const double2*>(b)[o];
+      };
+      if (p.src_cplx) {  // Re or Im of a complex field, transformed as a real ring
+        const double2 z = static_cast<const double2*>(p.fin)[o];
+        x.x = p.op == kExprImag ? z.y : z.x;
       } else {
-        x = static_cast<const double2*>(p.fin)[o];
-      }
-      if (p.fin2) {  // pointwise product with the second field
-        if (p.real) {
-          x.x *= p.f32 ? (double)static_cast<const float*>(p.fin2)[o]
-                       : static_cast<const double*>(p.fin2)[o];
-        } else {
-          double2 y;
-          if (p.f32) {
-            const float2 z = static_cast<const float2*>(p.fin2)[o];
-            y = make_double2(z.x, z.y);
-          } else {
-            y = static_cast<const double2*>(p.fin2)[o];
-          }
-          x = make_double2(x.x * y.x - x.y * y.y, x.x * y.y + x.y * y.x);
+        x = load(p.fin);
+        if (p.op) {  // pointwise expression of the fields
+          const double2 y = expr_binary(p.op) ? load(p.fin2) : make_double2(0.0, 0.0);
+          x = p.real ? make_double2(expr_real(p.op, x.x, y.x, p.scalar.x), 0.0)
+                     : expr_cplx(p.op, x, y, p.scalar);
         }
       }
     }
     buf[idx] = x;
   }
   __syncthreads();
-  ring_dft<-1>(buf, p);
+  ring_dft<-1, CAP>(buf, p);
   const double w = p.wq[theta];
   for (int idx = threadIdx.x; idx < R * p.nslot; idx += kFFTThreads) {
     const int q = idx % R;
@@ -337,10 +339,15 @@ __global__ void lmax0_kernel(int mode, int real, int nfield, const double* in, l
 
 hipError_t launch_fft_synth(const FFTParams& p0, hipStream_t s) {
   FFTParams p = p0;
-  if (p.M > kFFTElems) return hipErrorInvalidValue;
-  p.rings = kFFTElems / p.M;
+  if (p.M > kFFTElemsMax) return hipErrorInvalidValue;
+  const bool big = p.M > kFFTElems;
+  p.rings = (big ? kFFTElemsMax : kFFTElems) / p.M;
   p.nfg = (p.nfield + p.rings - 1) / p.rings;
-  hipLaunchKernelGGL(fft_synth, dim3((long)p.ntheta * p.nfg), dim3(kFFTThreads), 0, s, p);
+  const dim3 grid((long)p.ntheta * p.nfg);
+  if (big)
+    hipLaunchKernelGGL((fft_synth<kFFTElemsMax>), grid, dim3(kFFTThreads), 0, s, p);
+  else
+    hipLaunchKernelGGL((fft_synth<kFFTElems>), grid, dim3(kFFTThreads), 0, s, p);
   return hipGetLastError();
 }
 
@@ -348,10 +355,15 @@ hipError_t launch_fft_anal(const FFTParams& p0, hipStream_t s) {
   FFTParams p = p0;
   p.n_cg = (p.ncol + 15) / 16;
   p.nquad = (p.ntheta + 3) / 4;
-  if (p.M > kFFTElems) return hipErrorInvalidValue;
-  p.rings = kFFTElems / p.M;
+  if (p.M > kFFTElemsMax) return hipErrorInvalidValue;
+  const bool big = p.M > kFFTElems;
+  p.rings = (big ? kFFTElemsMax : kFFTElems) / p.M;
   p.nfg = (p.nfield + p.rings - 1) / p.rings;
-  hipLaunchKernelGGL(fft_anal, dim3((long)p.ntheta * p.nfg), dim3(kFFTThreads), 0, s, p);
+  const dim3 grid((long)p.ntheta * p.nfg);
+  if (big)
+    hipLaunchKernelGGL((fft_anal<kFFTElemsMax>), grid, dim3(kFFTThreads), 0, s, p);
+  else
+    hipLaunchKernelGGL((fft_anal<kFFTElems>), grid, dim3(kFFTThreads), 0, s, p);
   return hipGetLastError();
 }
 

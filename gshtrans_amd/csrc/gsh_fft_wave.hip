@@ -235,6 +235,21 @@ __device__ __forceinline__ double2 ld_pair(const void* b, long i) {
   return *reinterpret_cast<const double2*>(static_cast<const double*>(b) + i);
 }
 
+// Re (op kExprReal) or Im (kExprImag) of a complex sample, and of the two
+// consecutive complex samples i, i + 1 (FP64 storage).
+__device__ __forceinline__ double cplx_part(double2 z, int op) {
+  return op == kExprImag ? z.y : z.x;
+}
+__device__ __forceinline__ double2 cplx_part2(const void* b, long i, int op) {
+  const double2 z0 = static_cast<const double2*>(b)[i];
+  const double2 z1 = static_cast<const double2*>(b)[i + 1];
+  return make_double2(cplx_part(z0, op), cplx_part(z1, op));
+}
+// A real expression of two consecutive samples of each operand.
+__device__ __forceinline__ double2 expr_pair(int op, double2 a, double2 b, double s) {
+  return make_double2(expr_real(op, a.x, b.x, s), expr_real(op, a.y, b.y, s));
+}
+
 __device__ __forceinline__ double2 ld2(const double* p) {
   return *reinterpret_cast<const double2*>(p);
 }
@@ -470,13 +485,22 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
           f1v ? (long)field_index(p.fmap, f0 + 1) * p.fstride + (long)theta * N + 2 * lane : ia;
 #pragma unroll
       for (int a = 0; a < R2; a++) {
-        double2 pa = ld_pair<F32>(p.fin, ia + 128 * a);
-        double2 pb = ld_pair<F32>(p.fin, ib + 128 * a);
-        if (p.fin2) {  // pointwise product of two fields
-          const double2 qa = ld_pair<F32>(p.fin2, ia + 128 * a);
-          const double2 qb = ld_pair<F32>(p.fin2, ib + 128 * a);
-          pa = make_double2(pa.x * qa.x, pa.y * qa.y);
-          pb = make_double2(pb.x * qb.x, pb.y * qb.y);
+        double2 pa, pb;
+        if (p.src_cplx) {  // Re or Im of complex fields (samples 2m, 2m + 1)
+          pa = cplx_part2(p.fin, ia + 128 * a, p.op);
+          pb = cplx_part2(p.fin, ib + 128 * a, p.op);
+        } else {
+          pa = ld_pair<F32>(p.fin, ia + 128 * a);
+          pb = ld_pair<F32>(p.fin, ib + 128 * a);
+          if (p.op) {  // pointwise expression of the fields
+            double2 qa = make_double2(0.0, 0.0), qb = qa;
+            if (expr_binary(p.op)) {
+              qa = ld_pair<F32>(p.fin2, ia + 128 * a);
+              qb = ld_pair<F32>(p.fin2, ib + 128 * a);
+            }
+            pa = expr_pair(p.op, pa, qa, p.scalar.x);
+            pb = expr_pair(p.op, pb, qb, p.scalar.x);
+          }
         }
         if (!f1v) pb = make_double2(0.0, 0.0);
         e[a] = make_double2(pa.x, pb.x);
@@ -499,27 +523,33 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
     const long ia = (long)field_index(p.fmap, f0) * p.fstride + (long)theta * N + lane;
     const long ib =
         REAL && f1v ? (long)field_index(p.fmap, f0 + 1) * p.fstride + (long)theta * N + lane : ia;
-    if (!p.fin2) {
-      if (!REAL) {
+    if (!REAL) {
 #pragma unroll
-        for (int n1 = 0; n1 < R; n1++) v[n1] = ld_cplx<F32>(p.fin, ia + 64 * n1);
-      } else {
-#pragma unroll
-        for (int n1 = 0; n1 < R; n1++) {
-          const double xa = ld_real<F32>(p.fin, ia + 64 * n1);
-          const double xb = ld_real<F32>(p.fin, ib + 64 * n1);
-          v[n1] = make_double2(xa, f1v ? xb : 0.0);
+      for (int n1 = 0; n1 < R; n1++) {
+        double2 x = ld_cplx<F32>(p.fin, ia + 64 * n1);
+        if (p.op) {  // pointwise expression of the fields
+          const double2 y =
+              expr_binary(p.op) ? ld_cplx<F32>(p.fin2, ia + 64 * n1) : make_double2(0.0, 0.0);
+          x = expr_cplx(p.op, x, y, p.scalar);
         }
+        v[n1] = x;
       }
-    } else if (!REAL) {  // pointwise product of two fields
-#pragma unroll
-      for (int n1 = 0; n1 < R; n1++)
-        v[n1] = cmul(ld_cplx<F32>(p.fin, ia + 64 * n1), ld_cplx<F32>(p.fin2, ia + 64 * n1));
     } else {
 #pragma unroll
       for (int n1 = 0; n1 < R; n1++) {
-        const double xa = ld_real<F32>(p.fin, ia + 64 * n1) * ld_real<F32>(p.fin2, ia + 64 * n1);
-        const double xb = ld_real<F32>(p.fin, ib + 64 * n1) * ld_real<F32>(p.fin2, ib + 64 * n1);
+        double xa, xb;
+        if (p.src_cplx) {  // Re or Im of complex fields
+          xa = cplx_part(ld_cplx<F32>(p.fin, ia + 64 * n1), p.op);
+          xb = cplx_part(ld_cplx<F32>(p.fin, ib + 64 * n1), p.op);
+        } else {
+          xa = ld_real<F32>(p.fin, ia + 64 * n1);
+          xb = ld_real<F32>(p.fin, ib + 64 * n1);
+          if (p.op) {  // pointwise expression of the fields
+            const bool two = expr_binary(p.op);
+            xa = expr_real(p.op, xa, two ? ld_real<F32>(p.fin2, ia + 64 * n1) : 0.0, p.scalar.x);
+            xb = expr_real(p.op, xb, two ? ld_real<F32>(p.fin2, ib + 64 * n1) : 0.0, p.scalar.x);
+          }
+        }
         v[n1] = make_double2(xa, f1v ? xb : 0.0);
       }
     }

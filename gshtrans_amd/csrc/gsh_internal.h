@@ -22,6 +22,8 @@ constexpr int kSynthTailRows = 2;
 // degrees (the split-BF16 analysis starts its recurrences at each of them).
 constexpr int kPass = 64;
 constexpr int kCk = 16;
+// Passes of the wide split-BF16 analysis (leg_anal_bf16_w): 128 degrees.
+constexpr int kWideAnalPass = 128;
 static_assert(kPass % kCk == 0, "passes start at checkpoints");
 
 // Parameters of one Legendre-contraction launch (synthesis, analysis or
@@ -60,6 +62,16 @@ struct LegParams {
   // per-grid / per-n tables
   const double* cosv;   // cos(theta_t), t < ntheta
   const double* seeds;  // Ortho-normalised d^{l0}_{m n}(theta_t) at [(m+L)*ntheta + t]
+  // Seeds below the normal double range, in extended range (rec_checkpoints
+  // only): where skexp[i] = k > 0 the seed of entry i is sseed[i] * 2^-k.
+  // Null when every seed of the upper index is a normal double or zero.
+  const double* sseed;
+  const int* skexp;
+  int* ck_flag;     // rec_checkpoints: set to 1 when a lane that started scaled
+                    // reaches a state of magnitude 2^-60 or more before L
+  int ck_restart;   // FP64 synthesis: take the state from the checkpoint
+                    // table every kPass degrees (the table's scaled states
+                    // carry the orders whose seeds underflow)
   // Recurrence coefficients {a, b, c, 0} per (m, l):
   //   e_l = (a cos - b) e_{l-1} - c e_{l-2}
   // rec[roff[m + L] + (l - l0)], l0 <= l <= L + kRecPad, zero beyond L.
@@ -147,6 +159,65 @@ __host__ __device__ inline long fa_index(int slot, int col, int theta, int n_cg,
          (col & 15);
 }
 
+// Pointwise field expressions that the analysis ring FFTs form as they load
+// a ring (gsh_forward_expr): codes of include/gshtrans_amd.h's
+// gsh_field_expr.  a and b are samples of the two input fields, s the
+// scalar (its real part for real fields).
+enum ExprOp {
+  kExprNone = 0,
+  kExprProduct = 1,     // a b
+  kExprSum = 2,         // a + b
+  kExprDifference = 3,  // a - b
+  kExprQuotient = 4,    // a / b
+  kExprNegate = 5,      // -a
+  kExprConj = 6,        // conj(a)
+  kExprReal = 7,        // Re a (a real field)
+  kExprImag = 8,        // Im a (a real field)
+  kExprScale = 9,       // s a
+  kExprShift = 10       // a + s
+};
+__host__ __device__ inline bool expr_binary(int op) {
+  return op >= kExprProduct && op <= kExprQuotient;
+}
+// Complex samples.  The quotient scales by the larger part of b (Smith's
+// method), as complex division without overflow of |b|^2 does.
+__device__ inline double2 expr_cplx(int op, double2 a, double2 b, double2 s) {
+  switch (op) {
+    case kExprProduct: return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    case kExprSum: return make_double2(a.x + b.x, a.y + b.y);
+    case kExprDifference: return make_double2(a.x - b.x, a.y - b.y);
+    case kExprQuotient:
+      if (fabs(b.x) >= fabs(b.y)) {
+        const double r = b.y / b.x, d = b.x + b.y * r;
+        return make_double2((a.x + a.y * r) / d, (a.y - a.x * r) / d);
+      } else {
+        const double r = b.x / b.y, d = b.x * r + b.y;
+        return make_double2((a.x * r + a.y) / d, (a.y * r - a.x) / d);
+      }
+    case kExprNegate: return make_double2(-a.x, -a.y);
+    case kExprConj: return make_double2(a.x, -a.y);
+    case kExprReal: return make_double2(a.x, 0.0);
+    case kExprImag: return make_double2(a.y, 0.0);
+    case kExprScale: return make_double2(s.x * a.x - s.y * a.y, s.x * a.y + s.y * a.x);
+    case kExprShift: return make_double2(a.x + s.x, a.y + s.y);
+    default: return a;
+  }
+}
+// Real samples (conj and Re are the identity, Im is zero).
+__device__ inline double expr_real(int op, double a, double b, double s) {
+  switch (op) {
+    case kExprProduct: return a * b;
+    case kExprSum: return a + b;
+    case kExprDifference: return a - b;
+    case kExprQuotient: return a / b;
+    case kExprNegate: return -a;
+    case kExprImag: return 0.0;
+    case kExprScale: return s * a;
+    case kExprShift: return a + s;
+    default: return a;
+  }
+}
+
 // Parameters of one ring-FFT launch.
 struct FFTParams {
   int N;            // ring length 2L
@@ -168,8 +239,12 @@ struct FFTParams {
   double* F;          // bins: FA (analysis; float elements when fa32) or F
   int fa32;           // analysis: FA in FP32 (split-BF16 modes)
   const void* fin;    // analysis input field
-  const void* fin2;   // analysis of a pointwise product: the second factor
-                      // (same layout as fin), multiplied in as the rings load; or null
+  const void* fin2;   // analysis of a pointwise expression of two fields: the
+                      // second operand (same layout as fin), or null
+  int op;             // ExprOp applied to each loaded sample (kExprNone: none)
+  int src_cplx;       // real analysis of Re / Im of a complex field: fin is
+                      // complex (kExprReal / kExprImag), the rings real
+  double2 scalar;     // kExprScale / kExprShift operand
   void* fout;         // synthesis output field
   long fstride;       // elements between fields
   int f32;            // fields stored in FP32 (float / complex float)
@@ -199,7 +274,8 @@ hipError_t launch_leg_synth_bf16(const LegParams& p, int ns, hipStream_t s);
 // partial sums, and the sums over chunks into F after it.
 hipError_t launch_tail_dvals(const LegParams& p, hipStream_t s);
 hipError_t launch_tail_reduce(const LegParams& p, hipStream_t s);
-hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s);
+// wide: 128-degree passes (leg_anal_bf16_w; work list of family 3) instead of 64.
+hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s, int wide = 0);
 
 // FP32 <-> FP64 coefficient staging of batch plans (gsh_batch.hip); strides
 // and sizes in complex elements.

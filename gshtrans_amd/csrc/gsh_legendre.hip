@@ -204,7 +204,22 @@ __global__ __launch_bounds__(256) void leg_synth_f64(LegParams p) {
   double B0[T], B1[T];
   loadB(l0, B0);
   uint4 rq = fetch(l0);
+  // With ck_restart the state is taken from the checkpoint table every kPass
+  // degrees (checkpoint (l - l0) / kCk at degree l <= L), loaded a pass ahead.
+  const int nck = (p.L - l0 + kCk) / kCk;  // checkpoints 1 .. nck - 1 exist
+  const double2* ckm = p.ck_restart ? p.ck + ld_uniform(p.ckoff, m + p.L) * (long)p.ntheta + tc
+                                    : nullptr;
+  double2 ckn = make_double2(0.0, 0.0);
+  if (ckm && kPass / kCk < nck) ckn = ckm[(long)(kPass / kCk - 1) * p.ntheta];
   for (int l = l0; l <= lwg; l += 8) {
+    if (ckm && l > l0 && ((l - l0) & (kPass - 1)) == 0) {
+      const int k = (l - l0) / kCk;
+      if (k < nck) {
+        e0 = tv ? ckn.x : 0.0;
+        e1 = tv ? ckn.y : 0.0;
+        if (k + kPass / kCk < nck) ckn = ckm[(long)(k + kPass / kCk - 1) * p.ntheta];
+      }
+    }
     if (lane < 16) rsw[wave][lane] = rq;
     wave_sync();
     rq = fetch(l + 8);
@@ -429,6 +444,15 @@ hipError_t launch_anal(const LegParams& p, hipStream_t s, bool f32m) {
 // (e_l, e_{l-1}) at l = l0 + kCk k, k = 1 .. nck - 1, with the same
 // arithmetic as the transform kernels.  The coefficients are staged in LDS
 // 64 degrees at a time (all threads of the workgroup share the order).
+//
+// Scaling against underflow: a seed below the normal range (far from the
+// turning point, high degree) starts scaled, e = sseed = seed * 2^k; the
+// recurrence is linear and homogeneous, so the scaled state runs the same
+// steps, and whenever |e_l| passes 2^600 both values are divided by 2^600
+// (exactly) and k drops by as much, until k = 0.  The stored states are the
+// unscaled ones, e * 2^-k: zero while they are below the double range, and
+// accurate once the order's values rise toward significance.  Lanes with
+// k = 0 take exactly the steps of the transform kernels.
 __global__ __launch_bounds__(256) void rec_checkpoints(LegParams p) {
   const int mi = blockIdx.y;
   const int m = mi - p.L;
@@ -438,6 +462,10 @@ __global__ __launch_bounds__(256) void rec_checkpoints(LegParams p) {
   const int tc = min(t, nt - 1);
   const double cs = p.cosv[tc];
   double e0 = p.seeds[(long)mi * nt + tc], e1 = 0.0;
+  int kk = p.skexp ? p.skexp[(long)mi * nt + tc] : 0;
+  if (kk > 0) e0 = p.sseed[(long)mi * nt + tc];
+  const bool scaled = kk > 0;
+  bool rose = false;  // a scaled lane reached 2^-60
   const int nck = (p.L - l0 + kCk) / kCk;
   __shared__ uint4 rs[128];
   const uint4* recq = reinterpret_cast<const uint4*>(p.rec + (ld_uniform(p.roff, mi) - l0));
@@ -454,9 +482,21 @@ __global__ __launch_bounds__(256) void rec_checkpoints(LegParams p) {
     }
     const d4* rl = reinterpret_cast<const d4*>(rs) + j0;
 #pragma unroll
-    for (int j = 0; j < kCk; j++) rec_step(rl[j], cs, e0, e1);
-    if (t < nt) out[(long)(k - 1) * nt + t] = make_double2(e0, e1);
+    for (int j = 0; j < kCk; j++) {
+      rec_step(rl[j], cs, e0, e1);
+      if (kk > 0 && fabs(e0) > 0x1p+600) {
+        const int r = min(kk, 600);
+        e0 = ldexp(e0, -r);
+        e1 = ldexp(e1, -r);
+        kk -= r;
+      }
+    }
+    const double s0 = kk > 0 ? ldexp(e0, -kk) : e0;
+    const double s1 = kk > 0 ? ldexp(e1, -kk) : e1;
+    rose = rose || (scaled && fabs(s0) >= 0x1p-60);
+    if (t < nt) out[(long)(k - 1) * nt + t] = make_double2(s0, s1);
   }
+  if (p.ck_flag && t < nt && rose) atomicOr(p.ck_flag, 1);
 }
 
 template <bool F32M>

@@ -30,6 +30,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
 typedef const __attribute__((address_space(4))) d4 rec_t;
 
 __device__ __forceinline__ rec_t* rec_ptr(const double4* rec, long off) {
@@ -58,22 +59,20 @@ __device__ __forceinline__ long coef_index(int l, int m, int nabs, int real) {
               : (long)l * l - (long)nabs * nabs + m + l;
 }
 
-__device__ __forceinline__ unsigned pack2(__bf16 a, __bf16 b) {
-  bf16x2 v = {a, b};
-  return __builtin_bit_cast(unsigned, v);
-}
-
-// Split two doubles (consecutive K entries) into NS bf16 planes; plane s of
-// the pair is returned packed in one dword (element 0 in the low half).
+// Split two floats (consecutive K entries) into NS bf16 planes; plane s of
+// the pair is returned packed in one dword (element 0 in the low half).  One
+// packed round-to-nearest conversion per plane; the residuals subtract the
+// planes' values, read back by placing each bf16 in a float's high half.
 template <int NS>
 __device__ __forceinline__ void split_pair_f(float f0, float f1, unsigned* out) {
 #pragma unroll
   for (int s = 0; s < NS; s++) {
-    const __bf16 b0 = (__bf16)f0, b1 = (__bf16)f1;
-    out[s] = pack2(b0, b1);
+    const bf16x2 b = __builtin_convertvector((f2v){f0, f1}, bf16x2);
+    const unsigned u = __builtin_bit_cast(unsigned, b);
+    out[s] = u;
     if (s + 1 < NS) {
-      f0 -= (float)b0;
-      f1 -= (float)b1;
+      f0 -= __builtin_bit_cast(float, u << 16);
+      f1 -= __builtin_bit_cast(float, u & 0xffff0000u);
     }
   }
 }
@@ -682,6 +681,211 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
 }
 
 // ---------------------------------------------------------------------------
+// Analysis over 128-degree passes (kWidePass): the same contraction as
+// leg_anal_bf16 with eight degree tiles per wave, so each split B operand
+// (the VALU-heavy part of a K chunk) feeds twice as many MFMAs.  Colatitudes
+// are walked in groups of 64 (two 32-colatitude K chunks) so that the bf16
+// planes of 128 degrees still fit the LDS double buffer.  Wave w produces
+// the 16 degrees lp + 16 w .. + 15 of each group (one colatitude per lane)
+// from checkpoint 8 pass + w, and multiplies all 8 degree tiles with column
+// tile w in two halves of 4.  Chunk sums are flushed to FP64 in pairs, as
+// in leg_anal_bf16 and in the same order, so the results are the same bits.
+// ---------------------------------------------------------------------------
+constexpr int kWidePass = kWideAnalPass;
+
+template <int NS, int REAL>
+__global__ __launch_bounds__(512) void leg_anal_bf16_w(LegParams p) {
+  static_assert(kCk == 16 && kWidePass == 8 * kCk, "8 degree tiles of 16 per pass");
+  // Rows of 64 colatitudes padded to 72 (144 bytes, 36 banks apart): the 16
+  // rows of a 16-byte A-operand read start at 16 different multiples of 4
+  // banks and cover all 64 banks once.
+  __shared__ __bf16 Dl[2][NS][kWidePass][72];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const uint2 it = work_item(p.items, p.items_per);
+  if (it.x == ~0u) return;
+  const int mw = it.x;
+  const int pass = it.y & 0xffff;
+  const int csg = it.y >> 16;
+  const int m = order_of(mw, REAL);
+  const int slot = m - p.m_first;
+  const int l0 = max(abs(m), p.nabs);
+  const int nt = p.ntheta;
+  const int ngrp = (nt + 63) >> 6;
+  const int lp = l0 + kWidePass * pass;  // first degree of the pass
+  // Recurrence coefficients of the pass (entries lp+1 .. lp+128, 4 KiB).
+  // Entries up to L + kRecPad exist: kRecPad covers the 128 entries only
+  // from a pass start at most L - 63, so the last pass's tail reads clamp.
+  __shared__ uint4 rsa[2 * kWidePass];
+  const uint4* recq = reinterpret_cast<const uint4*>(p.rec + (ld_uniform(p.roff, m + p.L) - l0));
+  const int tid = threadIdx.x;
+  if (tid < 2 * kWidePass) {
+    const int e = min(lp + 1 + (tid >> 1), p.L + kRecPad);  // entry of this uint4
+    rsa[tid] = recq[2 * e + (tid & 1)];
+  }
+
+  // Producer share: degrees lp + 16 w .. + 15 of every colatitude of a group,
+  // from checkpoint kq (the seed for kq = 0; past L clamped to the last).
+  const int nck = (p.L - l0 + kCk) / kCk;  // checkpoints 1 .. nck-1 exist
+  const int kq = min((kWidePass / kCk) * pass + wave, nck - 1);
+  const double2* ckp = kq == 0 ? p.ck : p.ck + (ld_uniform(p.ckoff, m + p.L) + kq - 1) * (long)nt;
+  const double* sdp = p.seeds + (long)(m + p.L) * nt;
+  double nx_seed, nx_cs;
+  double2 nx_ck;
+  auto prefetch = [&](int g) {
+    const int tc = min(64 * g + lane, nt - 1);
+    nx_seed = sdp[tc];
+    nx_ck = ckp[tc];
+    nx_cs = p.cosv[tc];
+  };
+  double pe0, pe1, pcs;
+  auto begin = [&]() {
+    pcs = nx_cs;
+    pe0 = kq == 0 ? nx_seed : nx_ck.x;
+    pe1 = kq == 0 ? 0.0 : nx_ck.y;
+  };
+  // Half `part` (8 degrees) of this wave's production for a group.
+  auto produce8 = [&](int buf, int part) {
+    const d4* rl = reinterpret_cast<const d4*>(&rsa[0]) + 16 * wave;  // entry lp+16w+1+j
+#pragma unroll
+    for (int k = 8 * part; k < 8 * part + 8; k += 2) {
+      const double v0 = pe0;
+      rec_step(rl[k], pcs, pe0, pe1);
+      const double v1 = pe0;
+      rec_step(rl[k + 1], pcs, pe0, pe1);
+      unsigned o[NS];
+      split_pair<NS>(v0, v1, o);
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        const bf16x2 pr = __builtin_bit_cast(bf16x2, o[s]);
+        Dl[buf][s][16 * wave + k][lane] = pr[0];
+        Dl[buf][s][16 * wave + k + 1][lane] = pr[1];
+      }
+    }
+  };
+
+  // Consumer: column tile w, B straight from the FP32 bins as in
+  // leg_anal_bf16 (colatitudes from ntheta on zeroed at the split).
+  struct RawB {
+    float x[8];
+  };
+  const int tile = csg * 8 + wave;
+  const float* fb =
+      p.FA32 + (((long)slot * p.n_cg + min(tile, p.n_cg - 1)) * p.nquad + 2 * (lane >> 4)) * 64 +
+      (lane & 15);
+  auto loadB = [&](int kc, RawB& X) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) X.x[e] = fb[(long)kc * 512 + 16 * e];
+  };
+  auto splitB = [&](const RawB& X, uint4 (&B)[1][NS], int kc, bool last) {
+    unsigned pl[NS][4];
+    const int th = 32 * kc + 8 * (lane >> 4);
+#pragma unroll
+    for (int e2 = 0; e2 < 4; e2++) {
+      float x0 = X.x[2 * e2], x1 = X.x[2 * e2 + 1];
+      if (last) {
+        x0 = th + 2 * e2 < nt ? x0 : 0.0f;
+        x1 = th + 2 * e2 + 1 < nt ? x1 : 0.0f;
+      }
+      unsigned o[NS];
+      split_pair_f<NS>(x0, x1, o);
+#pragma unroll
+      for (int s = 0; s < NS; s++) pl[s][e2] = o[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) B[0][s] = make_uint4(pl[s][0], pl[s][1], pl[s][2], pl[s][3]);
+  };
+
+  d4 acc[8][1];
+#pragma unroll
+  for (int i = 0; i < 8; i++) acc[i][0] = d4{0.0, 0.0, 0.0, 0.0};
+  // Pairs of chunk sums: an odd chunk keeps its predecessor's sums in pp,
+  // an even one adds the pair in FP32 and flushes it to FP64.
+  f4 cp[8], pp[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) cp[i] = pp[i] = f4{0.f, 0.f, 0.f, 0.f};
+  auto step = [&](int buf, int kc, const RawB& X, bool last, bool odd) {
+    uint4 B[1][NS];
+    splitB(X, B, kc, last);
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+      uint4 A[4][NS];
+#pragma unroll
+      for (int lt = 0; lt < 4; lt++)
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+          A[lt][s] = *reinterpret_cast<const uint4*>(
+              &Dl[buf][s][(4 * half + lt) * 16 + (lane & 15)][(kc & 1) * 32 + 8 * (lane >> 4)]);
+      f4 c[4][1];
+      split_mma<NS, 4, 1>(A, B, c);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int t = 4 * half + i;
+        if (odd)
+          pp[t] = cp[t];
+        else
+          flush(acc[t][0], pp[t] + cp[t]);
+        cp[t] = c[i][0];
+      }
+    }
+  };
+
+  prefetch(0);
+  __syncthreads();  // rsa
+  begin();
+  produce8(0, 0);
+  produce8(0, 1);
+  prefetch(1);
+  __syncthreads();
+  // B operands one chunk ahead: chunk c in X[c & 1].
+  const int kcl = p.nkc - 1;
+  RawB X[2];
+  loadB(0, X[0]);
+  for (int g = 0; g + 1 < ngrp; g++) {
+    const int buf = g & 1;
+    const int kc0 = 2 * g;
+    begin();
+    prefetch(g + 2);
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      loadB(min(kc0 + j + 1, kcl), X[(j + 1) & 1]);
+      produce8(buf ^ 1, j);
+      step(buf, kc0 + j, X[j], false, j & 1);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+  }
+  {
+    // Last group: chunks 2 (ngrp-1) .. nkc-1 (one or two).
+    const int buf = (ngrp - 1) & 1;
+    const int kc0 = 2 * (ngrp - 1);
+    if (kc0 + 1 < p.nkc) loadB(kc0 + 1, X[1]);
+    step(buf, kc0, X[0], true, false);
+    if (kc0 + 1 < p.nkc) step(buf, kc0 + 1, X[1], true, true);
+    const bool lodd = (p.nkc - 1) & 1;
+#pragma unroll
+    for (int i = 0; i < 8; i++) flush(acc[i][0], lodd ? pp[i] + cp[i] : cp[i]);
+  }
+
+  const int col = tile * 16 + (lane & 15);
+  if (col >= p.ncol) return;
+  const int lf = field_lmax(p, col >> 1);
+  const long cbase = (long)field_index(p.fmap, col >> 1) * p.cstride;
+#pragma unroll
+  for (int lt = 0; lt < 8; lt++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int l = lp + 16 * lt + 4 * (lane >> 4) + r;
+      if (l > lf) continue;
+      const long ci = (cbase + coef_index(l, m, p.nabs, REAL)) * 2 + (col & 1);
+      double v = acc[lt][0][r];
+      if (p.accumulate) v += p.cout[ci];
+      if (p.zero_last && lf == p.L && l == lf && m == lf) v = 0.0;
+      p.cout[ci] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Synthesis of the last colatitudes t0 .. ntheta - 1 (t0 = ntheta -
 // tail_rows) when ntheta is only a row or two past a multiple of 256.  A
 // leg_synth_bf16 item holding one row would still walk every degree chunk,
@@ -801,9 +1005,24 @@ hipError_t launch_anal_bf16_t(const LegParams& p0, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s) {
+template <int NS>
+hipError_t launch_anal_bf16_w(const LegParams& p0, hipStream_t s) {
+  LegParams p = p0;
+  p.n_cg = (p.ncol + 15) / 16;
+  p.nquad = (p.ntheta + 3) / 4;
+  p.nkc = (p.ntheta + 31) / 32;
+  const dim3 grid(8 * p.items_per);  // work list: (order, 128-degree pass, column super-group)
+  if (p.real)
+    hipLaunchKernelGGL((leg_anal_bf16_w<NS, 1>), grid, dim3(512), 0, s, p);
+  else
+    hipLaunchKernelGGL((leg_anal_bf16_w<NS, 0>), grid, dim3(512), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s, int wide) {
   if (!p.ck || !p.ckoff) return hipErrorInvalidValue;  // checkpoint table required
   if (!p.FA32) return hipErrorInvalidValue;             // FP32 bins required
+  if (wide) return ns == 3 ? launch_anal_bf16_w<3>(p, s) : launch_anal_bf16_w<2>(p, s);
   return ns == 3 ? launch_anal_bf16_t<3>(p, s) : launch_anal_bf16_t<2>(p, s);
 }
 

@@ -3,8 +3,10 @@
 // GSHTrans/src/CanonicalComponentField/CanonicalComponentField.h:39-125, and
 // CanonicalComponentExpansion, GSHTrans/src/CanonicalComponentExpansion/
 // CanonicalComponentExpansion.h:21-78): host vectors in the reference's
-// layouts whose Data() feeds the grid's transforms.  Element access uses
-// operator()(i, j) where the reference uses the C++23 operator[](i, j).
+// layouts whose Data() feeds the grid's transforms.  Element access is the
+// reference's f[iTheta, iPhi] and u[l, m] (C++23 multi-argument operator[],
+// CanonicalComponentField.h:62-71, CanonicalComponentExpansion.h:63-66), or
+// f(iTheta, iPhi) and u(l, m) for C++20 callers.
 #ifndef GSHTRANS_AMD_CANONICAL_COMPONENT_FIELD_H
 #define GSHTRANS_AMD_CANONICAL_COMPONENT_FIELD_H
 
@@ -41,6 +43,10 @@ class CanonicalComponentField {
   Int NumberOfCoLatitudes() const { return _grid->NumberOfCoLatitudes(); }
   Int NumberOfLongitudes() const { return _grid->NumberOfLongitudes(); }
   Int Size() const { return static_cast<Int>(_data.size()); }
+#ifdef __cpp_multidimensional_subscript
+  Scalar operator[](Int iTheta, Int iPhi) const { return _data[Index(iTheta, iPhi)]; }
+  Scalar& operator[](Int iTheta, Int iPhi) { return _data[Index(iTheta, iPhi)]; }
+#endif
   Scalar operator()(Int iTheta, Int iPhi) const { return _data[Index(iTheta, iPhi)]; }
   Scalar& operator()(Int iTheta, Int iPhi) { return _data[Index(iTheta, iPhi)]; }
   auto Data() { return std::views::all(_data); }
@@ -80,6 +86,10 @@ class CanonicalComponentExpansion
 
   static constexpr Int UpperIndex() { return N_; }
   const Grid_& Grid() const { return *_grid; }
+#ifdef __cpp_multidimensional_subscript
+  Complex operator[](Int l, Int m) const { return _data[this->Index(l, m)]; }
+  Complex& operator[](Int l, Int m) { return _data[this->Index(l, m)]; }
+#endif
   Complex operator()(Int l, Int m) const { return _data[this->Index(l, m)]; }
   Complex& operator()(Int l, Int m) { return _data[this->Index(l, m)]; }
   auto Data() { return std::views::all(_data); }

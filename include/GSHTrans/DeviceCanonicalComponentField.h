@@ -99,6 +99,131 @@ class DeviceCanonicalComponentField {
   detail::DeviceBuffer _buf;
 };
 
+// A pointwise expression of device-resident fields, as the reference builds
+// expression fields from fields (CanonicalComponentFieldOverloads.h:20-266:
+// sums, differences, products, quotients, negation, conj / real / imag,
+// scalar multiples and shifts).  It holds the operands only;
+// GaussLegendreGrid::ForwardTransformation analyses it with the expression
+// applied as the ring FFT loads its rings (gsh_forward_expr), so no
+// expression field is formed in memory.  N_ and Value_ are the expression's
+// upper index and value type (real for real / imag of complex fields).
+template <std::ptrdiff_t N_, typename Grid_, RealOrComplexValued Value_>
+class DeviceFieldExpression {
+ public:
+  static constexpr bool device_expression = true;
+  using Int = std::ptrdiff_t;
+  using Value = Value_;
+  using Real = typename Grid_::Real;
+  using Complex = typename Grid_::Complex;
+  using Scalar = std::conditional_t<std::same_as<Value, RealValued>, Real, Complex>;
+
+  DeviceFieldExpression(int op, const Grid_& grid, int input_kind, const void* a,
+                        const void* b = nullptr, Complex s = Complex{0})
+      : _op{op}, _kind{input_kind}, _grid{&grid}, _a{a}, _b{b}, _s{s} {}
+  static constexpr Int UpperIndex() { return N_; }
+  const Grid_& Grid() const { return *_grid; }
+  int Operation() const { return _op; }
+  int InputKind() const { return _kind; }  // GSH_REAL or GSH_COMPLEX operands
+  const void* First() const { return _a; }
+  const void* Second() const { return _b; }
+  Complex ScalarOperand() const { return _s; }
+
+ private:
+  int _op, _kind;
+  const Grid_* _grid;
+  const void* _a;
+  const void* _b;
+  Complex _s;
+};
+
+namespace detail {
+template <RealOrComplexValued V>
+constexpr int kind_of = std::same_as<V, RealValued> ? GSH_REAL : GSH_COMPLEX;
+}  // namespace detail
+
+// CanonicalComponentFieldMultiply (upper index N0 + N1).
+template <std::ptrdiff_t N0, std::ptrdiff_t N1, typename G, RealOrComplexValued V>
+auto operator*(const DeviceCanonicalComponentField<N0, G, V>& a,
+               const DeviceCanonicalComponentField<N1, G, V>& b) {
+  return DeviceFieldExpression<N0 + N1, G, V>(GSH_EXPR_PRODUCT, a.Grid(), detail::kind_of<V>,
+                                              a.DeviceData(), b.DeviceData());
+}
+// CanonicalComponentFieldAdd / Subtract (equal upper indices).
+template <std::ptrdiff_t N, typename G, RealOrComplexValued V>
+auto operator+(const DeviceCanonicalComponentField<N, G, V>& a,
+               const DeviceCanonicalComponentField<N, G, V>& b) {
+  return DeviceFieldExpression<N, G, V>(GSH_EXPR_SUM, a.Grid(), detail::kind_of<V>,
+                                        a.DeviceData(), b.DeviceData());
+}
+template <std::ptrdiff_t N, typename G, RealOrComplexValued V>
+auto operator-(const DeviceCanonicalComponentField<N, G, V>& a,
+               const DeviceCanonicalComponentField<N, G, V>& b) {
+  return DeviceFieldExpression<N, G, V>(GSH_EXPR_DIFFERENCE, a.Grid(), detail::kind_of<V>,
+                                        a.DeviceData(), b.DeviceData());
+}
+// CanonicalComponentFieldDivide (a divisor of upper index 0).
+template <std::ptrdiff_t N, typename G, RealOrComplexValued V>
+auto operator/(const DeviceCanonicalComponentField<N, G, V>& a,
+               const DeviceCanonicalComponentField<0, G, V>& b) {
+  return DeviceFieldExpression<N, G, V>(GSH_EXPR_QUOTIENT, a.Grid(), detail::kind_of<V>,
+                                        a.DeviceData(), b.DeviceData());
+}
+// Negation, conj, real, imag (CanonicalComponentFieldUnary.h).
+template <std::ptrdiff_t N, typename G, RealOrComplexValued V>
+auto operator-(const DeviceCanonicalComponentField<N, G, V>& a) {
+  return DeviceFieldExpression<N, G, V>(GSH_EXPR_NEGATE, a.Grid(), detail::kind_of<V>,
+                                        a.DeviceData());
+}
+template <std::ptrdiff_t N, typename G, RealOrComplexValued V>
+auto conj(const DeviceCanonicalComponentField<N, G, V>& a) {
+  return DeviceFieldExpression<N, G, V>(GSH_EXPR_CONJ, a.Grid(), detail::kind_of<V>,
+                                        a.DeviceData());
+}
+template <std::ptrdiff_t N, typename G, RealOrComplexValued V>
+auto real(const DeviceCanonicalComponentField<N, G, V>& a) {
+  return DeviceFieldExpression<N, G, RealValued>(GSH_EXPR_REAL, a.Grid(), detail::kind_of<V>,
+                                                 a.DeviceData());
+}
+template <std::ptrdiff_t N, typename G, RealOrComplexValued V>
+auto imag(const DeviceCanonicalComponentField<N, G, V>& a) {
+  return DeviceFieldExpression<N, G, RealValued>(GSH_EXPR_IMAG, a.Grid(), detail::kind_of<V>,
+                                                 a.DeviceData());
+}
+// Scalar multiples (any upper index) and shifts (upper index 0), the
+// CanonicalComponentFieldUnaryWithScalar forms.
+template <std::ptrdiff_t N, typename G, RealOrComplexValued V>
+auto operator*(const DeviceCanonicalComponentField<N, G, V>& a,
+               typename DeviceCanonicalComponentField<N, G, V>::Scalar s) {
+  return DeviceFieldExpression<N, G, V>(GSH_EXPR_SCALE, a.Grid(), detail::kind_of<V>,
+                                        a.DeviceData(), nullptr, typename G::Complex(s));
+}
+template <std::ptrdiff_t N, typename G, RealOrComplexValued V>
+auto operator*(typename DeviceCanonicalComponentField<N, G, V>::Scalar s,
+               const DeviceCanonicalComponentField<N, G, V>& a) {
+  return a * s;
+}
+template <std::ptrdiff_t N, typename G, RealOrComplexValued V>
+auto operator/(const DeviceCanonicalComponentField<N, G, V>& a,
+               typename DeviceCanonicalComponentField<N, G, V>::Scalar s) {
+  return a * (typename DeviceCanonicalComponentField<N, G, V>::Scalar(1) / s);
+}
+template <typename G, RealOrComplexValued V>
+auto operator+(const DeviceCanonicalComponentField<0, G, V>& a,
+               typename DeviceCanonicalComponentField<0, G, V>::Scalar s) {
+  return DeviceFieldExpression<0, G, V>(GSH_EXPR_SHIFT, a.Grid(), detail::kind_of<V>,
+                                        a.DeviceData(), nullptr, typename G::Complex(s));
+}
+template <typename G, RealOrComplexValued V>
+auto operator+(typename DeviceCanonicalComponentField<0, G, V>::Scalar s,
+               const DeviceCanonicalComponentField<0, G, V>& a) {
+  return a + s;
+}
+template <typename G, RealOrComplexValued V>
+auto operator-(const DeviceCanonicalComponentField<0, G, V>& a,
+               typename DeviceCanonicalComponentField<0, G, V>::Scalar s) {
+  return a + (-s);
+}
+
 template <std::ptrdiff_t N, typename Grid>
 using DeviceRealCanonicalComponentField = DeviceCanonicalComponentField<N, Grid, RealValued>;
 template <std::ptrdiff_t N, typename Grid>

@@ -35,25 +35,9 @@
 #include <vector>
 
 #include "../gshtrans_amd.h"
+#include "Indexing.h"
 
 namespace GSHTrans {
-
-// Index-range, normalisation and value tags (GSHTrans/src/Concepts.h:24-54).
-struct All {};
-struct NonNegative {};
-struct Single {};
-struct Ortho {};
-struct FourPi {};
-struct RealValued {};
-struct ComplexValued {};
-
-template <typename T>
-concept IndexRange =
-    std::same_as<T, All> or std::same_as<T, NonNegative> or std::same_as<T, Single>;
-template <typename T>
-concept OrderIndexRange = std::same_as<T, All> or std::same_as<T, NonNegative>;
-template <typename T>
-concept RealOrComplexValued = std::same_as<T, RealValued> or std::same_as<T, ComplexValued>;
 
 template <typename T>
 struct IsComplex : std::false_type {};
@@ -72,48 +56,6 @@ inline void Check(int status, const char* where) {
   if (status != GSH_OK) throw Error(status, where);
 }
 
-// GSHIndices(lMax, lMax, n) offsets (GSHTrans/src/Indexing.h:104-150).
-template <OrderIndexRange MRange>
-class GSHIndices {
- public:
-  using Int = std::ptrdiff_t;
-  GSHIndices(Int lMax, Int mMax, Int n) : _lMax{lMax}, _mMax{std::min(lMax, mMax)}, _n{n} {}
-  Int MinDegree() const { return std::abs(_n); }
-  Int MaxDegree() const { return _lMax; }
-  Int MaxOrder() const { return _mMax; }
-  // GSHIndices::OffsetForDegree (Indexing.h:104-131): degrees below
-  // min(lMax, mMax) hold every order, those above 2 mMax + 1 (All) or
-  // mMax + 1 (NonNegative).
-  Int OffsetForDegree(Int l) const {
-    const Int na = std::abs(_n), mm = _mMax;
-    if constexpr (std::same_as<MRange, All>) {
-      if (mm >= na)
-        return l <= mm ? l * l - na * na
-                       : (mm + 1) * (mm + 1) - na * na + (l - 1 - mm) * (2 * mm + 1);
-      return (l - na) * (2 * mm + 1);
-    } else {
-      if (mm >= na)
-        return l <= mm ? l * (l + 1) / 2 - na * (na + 1) / 2
-                       : (mm + 1) * (mm + 2) / 2 - na * (na + 1) / 2 + (l - 1 - mm) * (mm + 1);
-      return (l - na) * (mm + 1);
-    }
-  }
-  Int Size() const {
-    const Int top = std::min(_lMax, _mMax);
-    return OffsetForDegree(_lMax) + (std::same_as<MRange, All> ? 2 * top + 1 : top + 1);
-  }
-  // Index(l, m) (Indexing.h:54-65, 143-146).
-  Int Index(Int l, Int m) const {
-    if constexpr (std::same_as<MRange, All>)
-      return OffsetForDegree(l) + m + std::min(l, _mMax);
-    else
-      return OffsetForDegree(l) + m;
-  }
-
- private:
-  Int _lMax, _mMax, _n;
-};
-
 // A field or coefficient array held in device memory
 // (DeviceCanonicalComponentField / DeviceCanonicalComponentExpansion in
 // DeviceCanonicalComponentField.h): the transforms take its device pointer
@@ -125,6 +67,18 @@ concept DeviceResident = requires(T& t, const T& ct) {
   { t.DeviceData() } -> std::same_as<typename T::Scalar*>;
   { ct.DeviceData() } -> std::same_as<const typename T::Scalar*>;
   { ct.Size() } -> std::convertible_to<std::ptrdiff_t>;
+};
+
+// A pointwise expression of device-resident fields (DeviceFieldExpression in
+// DeviceCanonicalComponentField.h), analysed without being formed.
+template <typename T>
+concept DeviceExpression = requires(const T& e) {
+  requires T::device_expression;
+  typename T::Scalar;
+  { e.Operation() } -> std::convertible_to<int>;
+  { e.InputKind() } -> std::convertible_to<int>;
+  { e.First() } -> std::convertible_to<const void*>;
+  { e.Second() } -> std::convertible_to<const void*>;
 };
 
 template <std::floating_point Real_ = double, OrderIndexRange MRange_ = All,
@@ -264,6 +218,23 @@ class GaussLegendreGrid {
                       real ? GSH_REAL : GSH_COMPLEX, precision, GSH_STORE_FP64, 1,
                       in.DeviceData(), 0, out.DeviceData(), 0, stream),
           "InverseTransformation");
+  }
+  // ForwardTransformation of an expression of device-resident fields
+  // (a * b, a + b, conj(a), real(a), s * a, ...): the expression is formed as
+  // the analysis ring FFT loads its rings, and the result accumulates into
+  // `out` as the reference's analysis does.
+  template <DeviceExpression Expr, DeviceResident Out>
+    requires std::same_as<typename Out::Scalar, Complex>
+  void ForwardTransformation(Int lMax, Int n, const Expr& e, Out& out,
+                             int precision = GSH_FP64, void* stream = nullptr) const {
+    constexpr bool real = std::same_as<typename Expr::Scalar, Real>;
+    if (out.Size() != (real ? RealCoefficientSize(lMax, n) : ComplexCoefficientSize(lMax, n)))
+      throw Error(GSH_ERR_INVALID_ARGUMENT, "ForwardTransformation");
+    const Complex s = e.ScalarOperand();
+    Check(gsh_forward_expr(_h.get(), static_cast<int>(lMax), static_cast<int>(n), e.InputKind(),
+                           e.Operation(), precision, 1, e.First(), e.Second(), 0, s.real(),
+                           s.imag(), out.DeviceData(), 0, 1, stream),
+          "ForwardTransformation");
   }
   void Synchronize(void* stream = nullptr) const {
     Check(gsh_synchronize(_h.get(), stream), "Synchronize");

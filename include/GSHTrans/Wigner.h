@@ -10,10 +10,19 @@
 //   AngleRange  Single (one angle) or Multiple
 //   Storage     ColumnMajor (blocks n outer) or RowMajor (theta outer)
 //
-// Block access is operator()(n, iTheta) (the reference's C++23 operator[]
-// with two arguments), returning a view with operator()(l, m).  MaxOrder(l)
-// returns min(l, mMax) (the reference's std::max there makes Orders(l) wrong
-// for l < mMax; its own views use the minimum, Indexing.h:24-27).
+// Block access as in the reference (Wigner.h:240-265, C++23):
+//   d[n, iTheta]   the (n, theta_i) block, a ConstGSHView: d[n, iTheta][l] is
+//                  degree l's view (Orders(), begin()/end(), [m]) and
+//                  d[n, iTheta][l, m] the value
+//   d[n]           one angle, several upper indices: the block of n
+//   d[iTheta]      one upper index, several angles: the block of theta_i
+//   d[l]           one angle and one upper index: degree l's view
+// operator() takes the same arguments for C++20 callers.  The one-index
+// forms with NRange = Single use the table's upper index nMax (the
+// reference passes n = 0 there, which is the same block only for nMax = 0).
+// MaxOrder(l) returns min(l, mMax) (the reference's std::max there makes
+// Orders(l) wrong for l < mMax; its own views use the minimum,
+// Indexing.h:24-27).
 #ifndef GSHTRANS_AMD_WIGNER_H
 #define GSHTRANS_AMD_WIGNER_H
 
@@ -23,26 +32,13 @@
 #include <vector>
 
 #include "GaussLegendreGrid.h"
+#include "Views.h"
 
 namespace GSHTrans {
 
 struct Multiple {};
 struct ColumnMajor {};
 struct RowMajor {};
-
-// One (n, theta_i) block: GSHIndices<MRange>(lMax, mMax, n) of values.
-template <typename Real, OrderIndexRange MRange>
-class WignerBlock : public GSHIndices<MRange> {
- public:
-  using Int = std::ptrdiff_t;
-  WignerBlock(Int lMax, Int mMax, Int n, const Real* data)
-      : GSHIndices<MRange>(lMax, mMax, n), _data{data} {}
-  Real operator()(Int l, Int m) const { return _data[this->Index(l, m)]; }
-  const Real* Data() const { return _data; }
-
- private:
-  const Real* _data;
-};
 
 template <std::floating_point Real_ = double, typename Norm_ = Ortho,
           OrderIndexRange MRange_ = All, IndexRange NRange_ = Single,
@@ -133,8 +129,9 @@ class Wigner {
   Int NumberOfAngles() const { return _nTheta; }
   auto AngleIndices() const { return std::views::iota(Int{0}, _nTheta); }
 
-  // Block of (n, theta_i) (Wigner.h:253-278).
-  WignerBlock<Real, MRange> operator()(Int n, Int iTheta) const {
+  // Block of (n, theta_i) (Wigner.h:240-265).
+  using Block = ConstGSHView<Real, MRange>;
+  Block operator()(Int n, Int iTheta) const {
     const long o = gsh_wigner_offset(static_cast<int>(_lMax), static_cast<int>(_mMax),
                                      static_cast<int>(MinUpperIndex()), static_cast<int>(_nMax),
                                      static_cast<int>(_nTheta), Mr(),
@@ -142,13 +139,33 @@ class Wigner {
                                                                         : GSH_ROW_MAJOR,
                                      static_cast<int>(n), static_cast<int>(iTheta));
     if (o < 0) throw Error(GSH_ERR_INVALID_ARGUMENT, "Wigner block");
-    return WignerBlock<Real, MRange>(_lMax, _mMax, n, _data.data() + o);
+    return Block(_lMax, _mMax, n, _data.data() + o);
   }
-  WignerBlock<Real, MRange> operator()(Int k) const
+  // One index: the block of n (one angle), of theta_k (one upper index).
+  Block operator()(Int k) const
     requires(std::same_as<AngleRange, Single> != std::same_as<NRange, Single>)
   {
-    if constexpr (std::same_as<AngleRange, Single>) return operator()(k, 0);  // block of n
-    return operator()(_nMax, k);                                              // block of theta_k
+    if constexpr (std::same_as<AngleRange, Single>) return operator()(k, 0);
+    return operator()(_nMax, k);
+  }
+  // One angle and one upper index: the view of degree l.
+  ConstGSHSubView<Real, MRange> operator()(Int l) const
+    requires(std::same_as<AngleRange, Single> && std::same_as<NRange, Single>)
+  {
+    return operator()(_nMax, 0)[l];
+  }
+#ifdef __cpp_multidimensional_subscript
+  Block operator[](Int n, Int iTheta) const { return operator()(n, iTheta); }
+#endif
+  Block operator[](Int k) const
+    requires(std::same_as<AngleRange, Single> != std::same_as<NRange, Single>)
+  {
+    return operator()(k);
+  }
+  ConstGSHSubView<Real, MRange> operator[](Int l) const
+    requires(std::same_as<AngleRange, Single> && std::same_as<NRange, Single>)
+  {
+    return operator()(l);
   }
   const std::vector<Real>& Data() const { return _data; }
 

@@ -28,7 +28,7 @@
  *                          synthesis, used for least-squares gradients
  *   gsh_plan_*             no reference counterpart: batches of fields with
  *                          uneven lMax and upper index in one plan (graphs,
- *                          autotuned tiles)
+ *                          autotuned tiles, FFT family and analysis passes)
  *   gsh_timing_enable /    no reference counterpart (the reference has no
  *   gsh_timing_read        timing code): HIP events around each kernel
  *                          launch, on the launch stream, for benchmarks
@@ -41,6 +41,10 @@
  *   gsh_forward_product    ForwardTransformation of the product field of
  *                          CanonicalComponentFieldBinary.h:203-253
  *                          (CanonicalComponentFieldOverloads.h:207-233)
+ *   gsh_forward_expr       ForwardTransformation of the expression fields of
+ *                          CanonicalComponentFieldOverloads.h:20-266 (sums,
+ *                          differences, quotients, conj / real / imag,
+ *                          negation, scalar multiples and shifts, products)
  *   gsh_wigner_table       Wigner<...>(lMax, mMax, nMax, theta) table build,
  *   (_host, _size,         Wigner.h:99-286 (ComputeAll :305-314, offsets
  *    _offset)              :116-132, 280-286), computed on the GPU
@@ -147,6 +151,33 @@ int gsh_forward(gsh_grid_t grid, int lmax, int n, int value_kind, int precision,
 int gsh_forward_product(gsh_grid_t grid, int lmax, int n, int value_kind, int precision,
                         int batch, const void* field_a, const void* field_b, long field_stride,
                         void* coeffs, long coeff_stride, int accumulate, void* stream);
+/* Analysis of a pointwise expression of the fields a (and b), formed as the
+ * ring FFT loads its rings, so no expression field is written: coeffs (+)=
+ * A_n(e) for the expression e of CanonicalComponentFieldOverloads.h:20-266
+ * (classes in CanonicalComponentFieldUnary.h / CanonicalComponentFieldBinary.h).
+ * value_kind is the kind of a and b; e is real when they are, or for
+ * GSH_EXPR_REAL / GSH_EXPR_IMAG (then coeffs take the NonNegative layout),
+ * complex otherwise.  n is e's upper index: N0 + N1 for products, N0 for the
+ * rest (the reference's types require N0 == N1 for sums and differences, a
+ * divisor of upper index 0, and N == 0 for adding a scalar).  The scalar
+ * (scalar_re + i scalar_im; real for real fields) is the operand of
+ * GSH_EXPR_SCALE and GSH_EXPR_SHIFT.  FP64 storage; lmax >= 1. */
+enum gsh_field_expr {
+  GSH_EXPR_PRODUCT = 1,    /* a * b   CanonicalComponentFieldMultiply */
+  GSH_EXPR_SUM = 2,        /* a + b   CanonicalComponentFieldAdd      */
+  GSH_EXPR_DIFFERENCE = 3, /* a - b   CanonicalComponentFieldSubtract */
+  GSH_EXPR_QUOTIENT = 4,   /* a / b   CanonicalComponentFieldDivide   */
+  GSH_EXPR_NEGATE = 5,     /* -a      operator-, Overloads.h:23-32    */
+  GSH_EXPR_CONJ = 6,       /* conj(a) CanonicalComponentFieldConj     */
+  GSH_EXPR_REAL = 7,       /* real(a) CanonicalComponentFieldReal     */
+  GSH_EXPR_IMAG = 8,       /* imag(a) CanonicalComponentFieldImag     */
+  GSH_EXPR_SCALE = 9,      /* s * a   UnaryWithScalar, multiplies     */
+  GSH_EXPR_SHIFT = 10      /* a + s   UnaryWithScalar, plus           */
+};
+int gsh_forward_expr(gsh_grid_t grid, int lmax, int n, int value_kind, int op, int precision,
+                     int batch, const void* field_a, const void* field_b, long field_stride,
+                     double scalar_re, double scalar_im, void* coeffs, long coeff_stride,
+                     int accumulate, void* stream);
 /* Integrate (ScalarFieldOverloads.h:288-301) of `batch` fields, or with
  * field_b the L2 inner product Integrate(conj(a) b) (:303-316), with the
  * grid's quadrature weights w_i 2 pi / nphi (GridBase.h:60-66): out[f] is
@@ -209,6 +240,12 @@ int gsh_plan_forward(gsh_plan_t plan, const void* field, long field_stride, void
  * FFT family of synthesis / analysis (1 wavefront-per-ring, 2 LDS). */
 int gsh_plan_info(gsh_plan_t plan, int* ngroups, int cap, int* n, int* lmax, int* count,
                   int* synth_tiles, int* fft_inverse, int* fft_forward);
+/* The tuned degrees per pass of each group's split-BF16 analysis (64 or
+ * 128; 0 for FP64 / FP32 plans and before the first analysis).  Tuned
+ * choices are kept per launch shape for the life of the process and, when
+ * the environment variable GSH_TUNE_CACHE names a file, read from and
+ * appended to it, so later plans of a shape skip the timing. */
+int gsh_plan_analysis_pass(gsh_plan_t plan, int* ngroups, int cap, int* pass);
 
 /* ---- per-kernel timing ------------------------------------------------- */
 /* In the split-BF16 synthesis of a grid whose colatitude count is one or two

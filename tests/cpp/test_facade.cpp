@@ -1,6 +1,7 @@
 // C++ facade test: the reference's own checks written against
 // include/GSHTrans (GaussLegendreGrid, CanonicalComponentField,
-// CanonicalComponentExpansion) running on the GPU through the C ABI.
+// CanonicalComponentExpansion, Wigner and its views) running on the GPU
+// through the C ABI; C++23, as the reference (multi-argument operator[]).
 //   * accessors of the grid (GaussLegendreGrid.h:86-103, GridBase.h:17-113);
 //   * the coefficient round trip of tests/CheckCoeff2Coeff.h:74-83 for every
 //     upper index, complex and real fields, bar 5e4 eps;
@@ -9,13 +10,15 @@
 //     transforms without host staging, bit-identical to the host-staged
 //     calls, in FP64 and in the split-BF16 mode.
 // Prints one line per check and "OK" at the end; exits non-zero on failure.
-#include <GSHTrans/Core>
+#include <GSHTrans/All>
 
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <concepts>
 #include <cstdio>
 #include <limits>
+#include <numeric>
 #include <vector>
 
 using namespace GSHTrans;
@@ -45,7 +48,36 @@ static double round_trip(const G& grid, int lMax, int n, bool real, unsigned see
   return err;
 }
 
-int main() {
+// The analysis of a device field expression (formed as the ring FFT loads its
+// rings) against the analysis of the same expression formed on the host and
+// uploaded: relative L2 of the coefficients.
+template <std::ptrdiff_t N, RealOrComplexValued V, typename Expr, typename F>
+static double expr_error(const Grid& grid, int L, const Expr& e, F value) {
+  CanonicalComponentField<N, Grid, V> h(grid);
+  for (int i = 0; i < grid.NumberOfCoLatitudes(); i++)
+    for (int j = 0; j < grid.NumberOfLongitudes(); j++) h[i, j] = value(i, j);
+  DeviceCanonicalComponentField<N, Grid, V> dh(h);
+  DeviceCanonicalComponentExpansion<N, Grid, V> out(grid), ref(grid);
+  grid.ForwardTransformation(L, N, e, out);
+  grid.ForwardTransformation(L, N, dh, ref);
+  grid.Synchronize();
+  const auto o = out.ToHost();
+  const auto r = ref.ToHost();
+  double num = 0, den = 0;
+  for (std::size_t k = 0; k < o.Data().size(); k++) {
+    num += std::norm(o.Data()[k] - r.Data()[k]);
+    den += std::norm(r.Data()[k]);
+  }
+  return std::sqrt(num / den);
+}
+
+int main(int argc, char** argv) {
+  // argv[1]: file for the device containers' inputs and outputs (raw doubles),
+  // which tests/test_gpu_facade.py holds to the oracle.
+  std::FILE* dump = argc > 1 ? std::fopen(argv[1], "wb") : nullptr;
+  auto put = [&](const void* p, std::size_t bytes) {
+    if (dump) std::fwrite(p, 1, bytes, dump);
+  };
   const double eps = std::numeric_limits<double>::epsilon();
   const int L = 32;
   Grid grid(L, 2);
@@ -95,7 +127,7 @@ int main() {
   grid.ForwardTransformation(L, 1, f.Data(), v.Data());
   double e = 0;
   for (int l = 1; l <= L; l++)
-    for (int m = -l; m <= l; m++) e = std::max(e, std::abs(v(l, m) - u(l, m)));
+    for (int m = -l; m <= l; m++) e = std::max(e, std::abs(v[l, m] - u[l, m]));
   expect(e < 5e4 * eps, "CanonicalComponentExpansion / Field round trip");
   // Device-resident containers: no host staging, same kernels, same bits.
   {
@@ -109,11 +141,11 @@ int main() {
     const auto vh = dv.ToHost();
     bool same = true;
     for (int i = 0; i < grid.NumberOfCoLatitudes(); i++)
-      for (int j = 0; j < grid.NumberOfLongitudes(); j++) same = same && fh(i, j) == f(i, j);
+      for (int j = 0; j < grid.NumberOfLongitudes(); j++) same = same && fh[i, j] == f[i, j];
     expect(same, "device synthesis equals the host-staged synthesis bit for bit");
     same = true;
     for (int l = 1; l <= L; l++)
-      for (int m = -l; m <= l; m++) same = same && vh(l, m) == v(l, m);
+      for (int m = -l; m <= l; m++) same = same && vh[l, m] == v[l, m];
     expect(same, "device analysis equals the host-staged analysis bit for bit");
     // Split-BF16 synthesis on the device against the FP64 field.
     DeviceComplexCanonicalComponentField<1, Grid> dm(grid);
@@ -123,8 +155,8 @@ int main() {
     double num = 0, den = 0;
     for (int i = 0; i < grid.NumberOfCoLatitudes(); i++)
       for (int j = 0; j < grid.NumberOfLongitudes(); j++) {
-        num += std::norm(mh(i, j) - f(i, j));
-        den += std::norm(f(i, j));
+        num += std::norm(mh[i, j] - f[i, j]);
+        den += std::norm(f[i, j]);
       }
     std::printf("  bf16x6 device synthesis relative L2 %.2e\n", std::sqrt(num / den));
     expect(std::sqrt(num / den) < 1e-7, "bf16x6 device synthesis within 1e-7 of FP64");
@@ -136,8 +168,8 @@ int main() {
     num = den = 0;
     for (int l = 1; l <= L; l++)
       for (int m = -l; m <= l; m++) {
-        num += std::norm(wh(l, m) - v(l, m));
-        den += std::norm(v(l, m));
+        num += std::norm(wh[l, m] - v[l, m]);
+        den += std::norm(v[l, m]);
       }
     std::printf("  bf16x6 device analysis relative L2 %.2e\n", std::sqrt(num / den));
     expect(std::sqrt(num / den) < 1e-7, "bf16x6 device analysis within 1e-7 of FP64");
@@ -153,8 +185,18 @@ int main() {
     const auto rv = drv.ToHost();
     double er = 0;
     for (int l = 0; l <= L; l++)
-      for (int m = 0; m <= l; m++) er = std::max(er, std::abs(rv(l, m) - ru(l, m)));
+      for (int m = 0; m <= l; m++) er = std::max(er, std::abs(rv[l, m] - ru[l, m]));
     expect(er < 5e4 * eps, "device real field round trip");
+    // Inputs and results of the device containers, in order: complex n = 1
+    // coefficients, synthesis, analysis; real n = 0 coefficients, synthesis,
+    // analysis.
+    const auto rf = drf.ToHost();
+    put(u.Data().data(), u.Data().size() * sizeof(u.Data()[0]));
+    put(fh.Data().data(), fh.Data().size() * sizeof(fh.Data()[0]));
+    put(vh.Data().data(), vh.Data().size() * sizeof(vh.Data()[0]));
+    put(ru.Data().data(), ru.Data().size() * sizeof(ru.Data()[0]));
+    put(rf.Data().data(), rf.Data().size() * sizeof(rf.Data()[0]));
+    put(rv.Data().data(), rv.Data().size() * sizeof(rv.Data()[0]));
     // Size mismatches raise instead of reading past the arrays.
     bool raised = false;
     try {
@@ -163,6 +205,53 @@ int main() {
       raised = err.status == GSH_ERR_INVALID_ARGUMENT;
     }
     expect(raised, "device arrays of the wrong size raise");
+  }
+  // Device field expressions (the reference's expression fields,
+  // CanonicalComponentFieldOverloads.h:20-266) analysed without being formed.
+  {
+    using C = std::complex<double>;
+    ComplexCanonicalComponentExpansion<1, Grid> ua(grid), ub(grid);
+    ComplexCanonicalComponentExpansion<0, Grid> uc(grid);
+    grid.RandomComplexCoefficient(L, 1, ua.Data(), 101);
+    grid.RandomComplexCoefficient(L, 1, ub.Data(), 102);
+    grid.RandomComplexCoefficient(L, 0, uc.Data(), 103);
+    uc[0, 0] = 12.0;  // the divisor stays away from zero
+    DeviceComplexCanonicalComponentExpansion<1, Grid> dua(ua), dub(ub);
+    DeviceComplexCanonicalComponentExpansion<0, Grid> duc(uc);
+    DeviceComplexCanonicalComponentField<1, Grid> fa(grid), fb(grid);
+    DeviceComplexCanonicalComponentField<0, Grid> fc(grid);
+    grid.InverseTransformation(L, 1, dua, fa);
+    grid.InverseTransformation(L, 1, dub, fb);
+    grid.InverseTransformation(L, 0, duc, fc);
+    grid.Synchronize();
+    const auto ha = fa.ToHost(), hb = fb.ToHost();
+    const auto hc = fc.ToHost();
+    const C s(0.25, -1.5);
+    double worst = 0;
+    auto note = [&](const char* what, double e) {
+      std::printf("  expression %s: %.2e\n", what, e);
+      worst = std::max(worst, e);
+    };
+    note("a * b", expr_error<2, ComplexValued>(grid, L, fa * fb,
+                                               [&](int i, int j) { return ha[i, j] * hb[i, j]; }));
+    note("a + b", expr_error<1, ComplexValued>(grid, L, fa + fb,
+                                               [&](int i, int j) { return ha[i, j] + hb[i, j]; }));
+    note("a - b", expr_error<1, ComplexValued>(grid, L, fa - fb,
+                                               [&](int i, int j) { return ha[i, j] - hb[i, j]; }));
+    note("a / c", expr_error<1, ComplexValued>(grid, L, fa / fc,
+                                               [&](int i, int j) { return ha[i, j] / hc[i, j]; }));
+    note("-a", expr_error<1, ComplexValued>(grid, L, -fa, [&](int i, int j) { return -ha[i, j]; }));
+    note("conj(a)", expr_error<1, ComplexValued>(grid, L, conj(fa),
+                                                 [&](int i, int j) { return std::conj(ha[i, j]); }));
+    note("real(a)", expr_error<1, RealValued>(grid, L, real(fa),
+                                              [&](int i, int j) { return ha[i, j].real(); }));
+    note("imag(a)", expr_error<1, RealValued>(grid, L, imag(fa),
+                                              [&](int i, int j) { return ha[i, j].imag(); }));
+    note("s * a", expr_error<1, ComplexValued>(grid, L, s * fa,
+                                               [&](int i, int j) { return s * ha[i, j]; }));
+    note("c + s", expr_error<0, ComplexValued>(grid, L, fc + s,
+                                               [&](int i, int j) { return hc[i, j] + s; }));
+    expect(worst < 1e-13, "device field expressions equal the analysis of the formed field");
   }
   // Wigner tables (Wigner.h:99-286) computed on the GPU: Gauss-Legendre
   // orthonormality of the Ortho values, FourPi = Ortho / (sqrt(2l+1) /
@@ -181,7 +270,7 @@ int main() {
         for (int l1 = std::max(std::abs(m), std::abs(n)); l1 <= L; l1++)
           for (int l2 = l1; l2 <= L; l2++) {
             double s = 0;
-            for (int i = 0; i <= L; i++) s += wt[i] * w(n, i)(l1, m) * w(n, i)(l2, m);
+            for (int i = 0; i <= L; i++) s += wt[i] * w[n, i][l1, m] * w[n, i][l2, m];
             orth = std::max(orth, std::abs(2 * M_PI * s - (l1 == l2 ? 1.0 : 0.0)));
           }
     for (int n = -2; n <= 2; n++)
@@ -189,8 +278,8 @@ int main() {
         for (int l = std::abs(n); l <= L; l++)
           for (int m = -l; m <= l; m++) {
             const double f = std::sqrt(2.0 * l + 1) / (2 * std::sqrt(M_PI));
-            rel = std::max(rel, std::abs(w4(n, i)(l, m) * f - w(n, i)(l, m)));
-            if (n >= 0 && m >= 0) same = same && wp(n, i)(l, m) == w(n, i)(l, m);
+            rel = std::max(rel, std::abs(w4[n, i][l, m] * f - w[n, i][l, m]));
+            if (n >= 0 && m >= 0) same = same && wp[n, i][l][m] == w[n, i][l][m];
           }
     std::printf("  Wigner orthonormality %.2e, FourPi vs Ortho %.2e\n", orth, rel);
     expect(orth < 1e-13, "Wigner Ortho table orthonormal on the Gauss-Legendre nodes");
@@ -200,20 +289,71 @@ int main() {
     double d1 = 0;
     for (int l = 1; l <= L; l++)
       for (int m = -std::min(l, 5); m <= std::min(l, 5); m++)
-        d1 = std::max(d1, std::abs(w1(1, 0)(l, m) - w(1, 4)(l, m)));
+        d1 = std::max(d1, std::abs(w1[l][m] - w[1, 4][l, m]));
     std::printf("  single-angle table vs full table %.2e; MaxOrder(3) %ld, MaxOrder(9) %ld\n", d1,
                 static_cast<long>(w1.MaxOrder(3)), static_cast<long>(w1.MaxOrder(9)));
     expect(d1 == 0 && w1.MaxOrder(3) == 3 && w1.MaxOrder(9) == 5,
            "single-angle Wigner with mMax < lMax equals the full table's orders");
   }
+  // The reference's view interface in the call shapes of its own Wigner
+  // tests: degree views d[l] with Orders() and d[l][m] of a one-angle table
+  // built by class template argument deduction (tests/CheckLegendre.h:26-35),
+  // compared with std::sph_legendre (n = 0: Ortho d^l_{m0}(theta) is
+  // Y_lm(theta, 0), Condon-Shortley phase included) for m >= 0, where the
+  // library function is defined; and upper-index blocks d[n] whose degree
+  // views d1[l].begin() / end() feed std::inner_product, the orthogonality
+  // sum_m d^l_{mn} d^l_{mn'} = delta_{nn'} of tests/CheckAdditionTheorem.h:
+  // 28-44, with the same bars (1e5 eps absolute, 1e3 eps).
+  {
+    const double theta = 2.0313;
+    auto d = Wigner(300, 300, 0, theta);
+    static_assert(std::same_as<decltype(d), Wigner<double, Ortho, All, Single, Single, ColumnMajor>>);
+    double err = 0;
+    long count = 0;
+    for (auto l : d.Degrees(0))
+      for (auto m : d[l].Orders()) {
+        if (m < 0) continue;
+        err = std::max(err, std::abs(d[l][m] - std::sph_legendre(l, m, theta)));
+        count++;
+      }
+    std::printf("  d[l][m] vs std::sph_legendre, %ld values: %.2e\n", count, err);
+    expect(count == 301 * 302 / 2 && err < 1e5 * eps, "Wigner degree views against sph_legendre");
+    auto dd = Wigner<double, FourPi, All, All>(40, 40, 40, theta);
+    double worst = 0;
+    for (auto n : dd.UpperIndices()) {
+      auto d1 = dd[n];
+      for (auto np : dd.UpperIndices()) {
+        auto d2 = dd[np];
+        for (auto l = std::max(std::abs(n), std::abs(np)); l <= 40; l++) {
+          auto sum = std::inner_product(d1[l].begin(), d1[l].end(), d2[l].begin(), 0.0);
+          if (n == np) --sum;
+          worst = std::max(worst, std::abs(sum));
+        }
+      }
+    }
+    std::printf("  addition theorem over d[n][l] views: %.2e\n", worst);
+    expect(worst < 1e3 * eps, "Wigner upper-index views satisfy the addition theorem");
+    // d[n, iTheta][l, m] = d[n, iTheta][l][m]; GSHView spans the block.
+    std::vector<double> th2{0.3, 1.1, 2.9};
+    auto dm = Wigner<double, Ortho, All, All, Multiple, RowMajor>(12, 7, 2, th2);
+    bool views = true;
+    for (auto n : dm.UpperIndices())
+      for (int i = 0; i < 3; i++) {
+        auto blk = dm[n, i];
+        views = views && blk.end() - blk.begin() == blk.Size();
+        for (auto l : blk.Degrees())
+          for (auto m : blk[l].Orders()) views = views && blk[l, m] == blk[l][m];
+      }
+    expect(views, "Wigner block views: [l, m] equals [l][m], begin/end span the block");
+  }
   // A single mode: Y_{3,2} (n = 0) synthesised then read back.
   ComplexCanonicalComponentExpansion<0, Grid> y(grid);
-  y(3, 2) = 1.0;
+  y[3, 2] = 1.0;
   ComplexCanonicalComponentField<0, Grid> fy(grid);
   grid.InverseTransformation(L, 0, y.Data(), fy.Data());
   ComplexCanonicalComponentExpansion<0, Grid> y2(grid);
   grid.ForwardTransformation(L, 0, fy.Data(), y2.Data());
-  expect(std::abs(y2(3, 2) - 1.0) < 1e-13 && std::abs(y2(3, 1)) < 1e-13, "single mode");
+  expect(std::abs(y2[3, 2] - 1.0) < 1e-13 && std::abs(y2[3, 1]) < 1e-13, "single mode");
   // The library's errors surface as exceptions (the reference asserts).
   bool threw = false;
   try {
@@ -224,6 +364,7 @@ int main() {
     threw = err.status == GSH_ERR_UPPER_INDEX;
   }
   expect(threw, "upper index outside the grid raises");
+  if (dump) std::fclose(dump);
   if (failures) {
     std::printf("%d FAILED\n", failures);
     return 1;

@@ -108,12 +108,29 @@ def test_random_coefficients_masks_and_reproducible():
 
 def test_cpp_facade_compiles_and_reports_errors():
     """include/GSHTrans (the C++ facade mirroring the reference's
-    GaussLegendreGrid / CanonicalComponentField API) compiles as C++20; without
-    a device its grid constructor raises the library's NO_DEVICE status."""
+    GaussLegendreGrid / CanonicalComponentField / Wigner API) compiles as
+    C++23 with the reference's call shapes (tests/cpp/test_facade.cpp, built
+    with the compiler gshtrans_amd.build uses), and its umbrella header also
+    compiles as C++20, where the two-argument subscripts are operator();
+    without a device the grid constructor raises the library's NO_DEVICE
+    status."""
     import subprocess
+    from gshtrans_amd.build import cxx23
+    inc = os.path.join(ROOT, "include")
     src = os.path.join(ROOT, "tests", "cpp", "test_facade.cpp")
-    r = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-Wall", "-I",
-                        os.path.join(ROOT, "include"), src], capture_output=True, text=True)
+    r = subprocess.run([cxx23(), "-std=c++23", "-fsyntax-only", "-Wall", "-I", inc, src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    snippet = (
+        "#include <GSHTrans/All>\n"
+        "using namespace GSHTrans;\n"
+        "double f(const Wigner<double, Ortho, All, All, Multiple>& d,\n"
+        "         ComplexCanonicalComponentExpansion<0, GaussLegendreGrid<double, All, All>>& u) {\n"
+        "  u(1, 0) = 1.0;\n"
+        "  return d(0, 0)(1, 0) + d(0, 0)[1][0] + u(1, 0).real();\n"
+        "}\n")
+    r = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-Wall", "-I", inc, "-x", "c++",
+                        "-"], input=snippet, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     binary = os.path.join(ROOT, "tests", "cpp", "test_facade")
     if gpu_available() or not os.path.exists(binary):

@@ -205,3 +205,80 @@ def test_plan_outlives_grid(torch):
     with pytest.raises(G.GSHError):
         plan.inverse(c)
     plan.close()
+
+
+def test_tuned_choices_are_cached_and_persisted(torch, tmp_path, monkeypatch):
+    """Tuned choices are kept per launch shape: with GSH_TUNE_CACHE naming a
+    file, the first plan of a shape appends its choices there, a second plan
+    of the same shape takes them without timing (the file does not grow) and
+    reports the same choices, and its results equal the first plan's."""
+    cache = tmp_path / "tune.txt"
+    cache.write_text("not a tuning line\n")  # unreadable lines are skipped
+    monkeypatch.setenv("GSH_TUNE_CACHE", str(cache))
+    L = 96
+    lmax = np.array([96, 80, 71, 40, 33, 12, 90, 64, 55] * 2)
+    n = np.array([0, 1, -1, 2, 0, 0, 1, -2, 2] * 2)
+    g = G.GaussLegendreGrid(L, 2)
+    p1 = G.BatchPlan(g, lmax, n)
+    c = make_coeffs(torch, lmax, n, False, p1.cmax, SEED0 + 500)
+    f1 = p1.inverse(c)
+    a1 = p1.forward(f1)
+    torch.cuda.synchronize()
+    lines = [x for x in cache.read_text().splitlines() if x.startswith("gsh2:")]
+    say(f"[tune cache] {len(lines)} shapes written")
+    assert len(lines) == 2 * len(p1.info())  # every group, both directions
+    p2 = G.BatchPlan(g, lmax, n)
+    f2 = p2.inverse(c)
+    a2 = p2.forward(f2)
+    torch.cuda.synchronize()
+    assert cache.read_text().count("gsh2:") == len(lines)
+    assert p2.info() == p1.info()
+    assert torch.equal(f1, f2) and torch.equal(a1, a2)
+    p1.close()
+    p2.close()
+
+
+def test_split_bf16_analysis_pass_choice(torch, tmp_path, monkeypatch):
+    """The split-BF16 analysis runs in 64- or 128-degree passes, whichever the
+    tuner timed faster per group; both pass lengths accumulate the same
+    products in the same order, so a plan's result does not depend on the
+    choice.  A second cache file that pins every shape to the other pass
+    length (read when GSH_TUNE_CACHE names a new file; its lines replace the
+    process's choices) forces it, and the results stay bit-identical."""
+    L = 512
+    lmax = np.array([512, 400, 512, 300])
+    n = np.array([0, 1, -1, 2])
+    g = G.GaussLegendreGrid(L, 2)
+    fp = G.BatchPlan(g, lmax, n)
+    c = make_coeffs(torch, lmax, n, False, fp.cmax, SEED0 + 600)
+    f = fp.inverse(c)
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("GSH_TUNE_CACHE", str(cache))
+    p1 = G.BatchPlan(g, lmax, n, precision="bf16x6")
+    a1 = p1.forward(f)
+    torch.cuda.synchronize()
+    passes = [d["analysis_pass"] for d in p1.info()]
+    say(f"[pass] tuned analysis passes {passes}")
+    assert all(p in (64, 128) for p in passes)
+    lines = [x.split() for x in cache.read_text().splitlines() if x.startswith("gsh2:")]
+    flipped = []
+    for key, t, ff, pl in lines:
+        if int(pl) in (64, 128):
+            pl = "128" if pl == "64" else "64"
+        flipped.append(" ".join([key, t, ff, pl]))
+    cache2 = tmp_path / "tune2.txt"
+    cache2.write_text("\n".join(flipped) + "\n")
+    monkeypatch.setenv("GSH_TUNE_CACHE", str(cache2))
+    p2 = G.BatchPlan(g, lmax, n, precision="bf16x6")
+    a2 = p2.forward(f)
+    torch.cuda.synchronize()
+    passes2 = [d["analysis_pass"] for d in p2.info()]
+    say(f"[pass] forced analysis passes {passes2}")
+    assert all(a + b == 192 for a, b in zip(passes, passes2))
+    assert torch.equal(a1, a2)
+    fp64 = fp.forward(f)
+    e = float(torch.linalg.vector_norm(a1 - fp64) / torch.linalg.vector_norm(fp64))
+    say(f"[pass] bf16x6 vs FP64 {e:.2e}")
+    assert e < 1e-7
+    for p in (fp, p1, p2):
+        p.close()

@@ -296,3 +296,50 @@ def test_wavefront_ring_fft_batches(torch, L, lmax, n, real, batch):
         c_ref = og.forward(lmax, n, f[b], real)
         assert rel_l2(c2[b], c_ref) <= 1e-12, b
     assert float(np.max(np.abs(c2 - cs))) < 5e4 * EPS
+
+
+@pytest.mark.parametrize("L,n,real", [(1100, 1, False), (1100, 0, True)])
+def test_long_bluestein_rings(torch, L, n, real):
+    """lMax past 1024 with a ring length that is not a power of two (2200
+    points): the LDS ring FFT's 8192-point Bluestein convolution, against the
+    oracle (<= 1e-12) and through the round trip."""
+    say(f"[long rings] L={L} n={n} real={real}")
+    g = G.GaussLegendreGrid(L, 1)
+    og = oracle.OracleGrid(L, 1, table=False)
+    cs = G.random_coefficients(L, n, real=real, seed=SEED0 + 70, batch=2)
+    f = host(g.inverse(L, n, dev(torch, cs), real=real))
+    c2 = host(g.forward(L, n, dev(torch, f), real=real))
+    f_ref = og.inverse(L, n, cs[1], real)
+    e_syn = rel_l2(f[1], f_ref)
+    e_ana = rel_l2(c2[1], og.forward(L, n, f[1], real))
+    e_rt = float(np.max(np.abs(c2 - cs)))
+    say(f"    synthesis {e_syn:.2e} analysis {e_ana:.2e} round trip {e_rt:.2e}")
+    assert e_syn <= 1e-12 and e_ana <= 1e-12 and e_rt < 5e4 * EPS
+    g.close()
+    og.close()
+
+
+def test_L2048_grid(torch):
+    """The largest grid (lMax = 2048, 4096-point rings in the LDS FFT, a
+    colatitude count one past a multiple of 256): the FP64 round trip of
+    complex n = 2 and real n = 0 fields, the split-BF16 synthesis and
+    analysis within 1e-7 of FP64, and the refusal of lMax = 2049."""
+    L = 2048
+    g = G.GaussLegendreGrid(L, 2)
+    for n, real in ((2, False), (0, True)):
+        say(f"[L2048] n={n} real={real}")
+        cs = G.random_coefficients(L, n, real=real, seed=SEED0 + 80 + n, batch=2)
+        dc = dev(torch, cs)
+        f = g.inverse(L, n, dc, real=real)
+        c2 = g.forward(L, n, f, real=real)
+        e_rt = float(np.max(np.abs(host(c2) - cs)))
+        fm = g.inverse(L, n, dc, real=real, precision="bf16x6")
+        cm = g.forward(L, n, f, real=real, precision="bf16x6")
+        e_syn = rel_l2(host(fm), host(f))
+        e_ana = rel_l2(host(cm), host(c2))
+        say(f"    round trip {e_rt:.2e}; bf16x6 synthesis {e_syn:.2e} analysis {e_ana:.2e}")
+        assert e_rt < 5e4 * EPS and e_syn <= 1e-7 and e_ana <= 1e-7
+    g.close()
+    with pytest.raises(G.GSHError) as e:
+        G.GaussLegendreGrid(L + 1, 0)
+    assert e.value.status == 5
