@@ -583,12 +583,13 @@ int ensure_synth_worklist(gsh_grid_s* g, int family, int lmax, int real, int ngr
     return GSH_OK;
   }
   const int nslot = real ? lmax + 1 : 2 * lmax + 1;
-  const int m_first = real ? 0 : -lmax;
   std::vector<std::pair<int, int>> units;  // (weight, mw * ngroups + grp)
   units.reserve((size_t)nslot * ngroups);
+  // mw is the kernels' order index: complex orders interleave as
+  // 0, -1, 1, -2, 2, ... (order_of_host).
   for (int mw = 0; mw < nslot; mw++)
     for (int grp = 0; grp < ngroups; grp++)
-      units.emplace_back(lmax - std::abs(mw + m_first) + 1, mw * ngroups + grp);
+      units.emplace_back(lmax - std::abs(order_of_host(mw, real)) + 1, mw * ngroups + grp);
   std::stable_sort(units.begin(), units.end(),
                    [](const auto& a, const auto& b) { return a.first > b.first; });
   std::vector<uint2> xcd[8];

@@ -37,17 +37,16 @@ __device__ __forceinline__ rec_t* rec_ptr(const double4* rec, long off) {
   return (rec_t*)(rec) + off;
 }
 
+// Recurrence coefficients of one degree, read with a scalar load: every lane
+// of a wave asks for the same entry, so the values arrive in SGPRs and the
+// recurrence takes them as scalar operands (no LDS traffic, no VGPRs).
+__device__ __forceinline__ d4 rec_entry(rec_t* rq, int e) { return rq[e]; }
+
 __device__ __forceinline__ void rec_step(const d4 q, double cs, double& e0, double& e1) {
   const double t = fma(q.x, cs, -q.y);
   const double en = fma(t, e0, -(q.z * e1));
   e1 = e0;
   e0 = en;
-}
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ int order_of(int mw, int real) {
@@ -471,12 +470,9 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   const int nt = p.ntheta;
   const int ngrp = (nt + 127) >> 7;
   const int lp = l0 + kPass * pass;  // first degree of the pass
-  // Recurrence coefficients of the pass (entries lp+1 .. lp+64, 2 KiB),
-  // staged in LDS.  Entries up to L + kRecPad exist.
-  __shared__ uint4 rsa[128];
-  const uint4* recq = reinterpret_cast<const uint4*>(p.rec + (ld_uniform(p.roff, m + p.L) - l0));
-  const int tid = threadIdx.x;
-  if (tid < 128) rsa[tid] = recq[2 * (lp + 1) + tid];
+  // Recurrence coefficients of the pass: entries lp+1 .. lp+64 (entry l for
+  // degree l; entries up to L + kRecPad exist).
+  rec_t* rq = rec_ptr(p.rec, ld_uniform(p.roff, m + p.L) - l0);
 
   // Producer share of this wave: block half bh, degree quarter q.  Its
   // recurrence starts at degree lp + 16 q from checkpoint kq (the seed for
@@ -511,13 +507,13 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
     pe1 = kq == 0 ? 0.0 : nx_ck.y;
   };
   auto produce4 = [&](int buf, int part) {
-    const d4* rl = reinterpret_cast<const d4*>(&rsa[0]) + 16 * q;  // entry lp+16q+1+j
+    const int eq = lp + 16 * q + 1;  // entry of step 0, wave-uniform
 #pragma unroll
     for (int k = 4 * part; k < 4 * part + 4; k += 2) {
       const double v0 = pe0;
-      rec_step(rl[k], pcs, pe0, pe1);
+      rec_step(rec_entry(rq, eq + k), pcs, pe0, pe1);
       const double v1 = pe0;
-      rec_step(rl[k + 1], pcs, pe0, pe1);
+      rec_step(rec_entry(rq, eq + k + 1), pcs, pe0, pe1);
       unsigned o[NS];
       split_pair<NS>(v0, v1, o);
 #pragma unroll
@@ -607,7 +603,6 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
   };
 
   prefetch(0);
-  __syncthreads();  // rsa
   begin();
 #pragma unroll
   for (int part = 0; part < 4; part++) produce4(0, part);
@@ -713,16 +708,11 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_w(LegParams p) {
   const int nt = p.ntheta;
   const int ngrp = (nt + 63) >> 6;
   const int lp = l0 + kWidePass * pass;  // first degree of the pass
-  // Recurrence coefficients of the pass (entries lp+1 .. lp+128, 4 KiB).
-  // Entries up to L + kRecPad exist: kRecPad covers the 128 entries only
-  // from a pass start at most L - 63, so the last pass's tail reads clamp.
-  __shared__ uint4 rsa[2 * kWidePass];
-  const uint4* recq = reinterpret_cast<const uint4*>(p.rec + (ld_uniform(p.roff, m + p.L) - l0));
-  const int tid = threadIdx.x;
-  if (tid < 2 * kWidePass) {
-    const int e = min(lp + 1 + (tid >> 1), p.L + kRecPad);  // entry of this uint4
-    rsa[tid] = recq[2 * e + (tid & 1)];
-  }
+  // Recurrence coefficients of the pass: entries lp+1 .. lp+128 (entry l for
+  // degree l).  Entries up to L + kRecPad exist: kRecPad covers the 128
+  // entries only from a pass start at most L - 63, so the last pass's tail
+  // reads clamp.
+  rec_t* rq = rec_ptr(p.rec, ld_uniform(p.roff, m + p.L) - l0);
 
   // Producer share: degrees lp + 16 w .. + 15 of every colatitude of a group,
   // from checkpoint kq (the seed for kq = 0; past L clamped to the last).
@@ -746,13 +736,13 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_w(LegParams p) {
   };
   // Half `part` (8 degrees) of this wave's production for a group.
   auto produce8 = [&](int buf, int part) {
-    const d4* rl = reinterpret_cast<const d4*>(&rsa[0]) + 16 * wave;  // entry lp+16w+1+j
+    const int ew = lp + 16 * wave + 1;  // entry of step 0, wave-uniform
 #pragma unroll
     for (int k = 8 * part; k < 8 * part + 8; k += 2) {
       const double v0 = pe0;
-      rec_step(rl[k], pcs, pe0, pe1);
+      rec_step(rec_entry(rq, min(ew + k, p.L + kRecPad)), pcs, pe0, pe1);
       const double v1 = pe0;
-      rec_step(rl[k + 1], pcs, pe0, pe1);
+      rec_step(rec_entry(rq, min(ew + k + 1, p.L + kRecPad)), pcs, pe0, pe1);
       unsigned o[NS];
       split_pair<NS>(v0, v1, o);
 #pragma unroll
@@ -831,7 +821,6 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_w(LegParams p) {
   };
 
   prefetch(0);
-  __syncthreads();  // rsa
   begin();
   produce8(0, 0);
   produce8(0, 1);
