@@ -1014,6 +1014,16 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
   p.f_lmax = o.f_lmax;
   p.cg_lmax = o.cg_lmax;
   const int ntsb = (g->ntheta() + 255) / 256;
+  FFTParams q = fft_params(g, lmax, real, batch);
+  q.fout = field;
+  q.fstride = fstride;
+  q.f32 = o.f32;
+  q.fmap = o.fmap;
+  // The split-BF16 synthesis hands its sums to the wavefront ring FFT in
+  // FP32 (half the bytes of F written and read; the products are FP32-
+  // accurate already).  The LDS ring FFT reads FP64 bins.
+  const bool f32bins = precision != GSH_FP64 && precision != GSH_FP32 && use_wave_fft(q, o);
+  p.f32out = q.fa32 = f32bins ? 1 : 0;
   if (precision == GSH_FP64 || precision == GSH_FP32) {
     // Orders whose seeds underflow but whose values rise to significance
     // before L: the synthesis takes its state from the scaled checkpoints.
@@ -1095,11 +1105,6 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
       return e;
     }));
   }
-  FFTParams q = fft_params(g, lmax, real, batch);
-  q.fout = field;
-  q.fstride = fstride;
-  q.f32 = o.f32;
-  q.fmap = o.fmap;
   GSH_HIP(timed(g, GSH_KERNEL_FFT_SYNTHESIS, s, [&] {
     return use_wave_fft(q, o) ? gsh::launch_fft_synth_wave(q, s) : gsh::launch_fft_synth(q, s);
   }));

@@ -340,6 +340,7 @@ __global__ void lmax0_kernel(int mode, int real, int nfield, const double* in, l
 hipError_t launch_fft_synth(const FFTParams& p0, hipStream_t s) {
   FFTParams p = p0;
   if (p.M > kFFTElemsMax) return hipErrorInvalidValue;
+  if (p.fa32) return hipErrorInvalidValue;  // FP32 bins: wavefront kernels only
   const bool big = p.M > kFFTElems;
   p.rings = (big ? kFFTElemsMax : kFFTElems) / p.M;
   p.nfg = (p.nfield + p.rings - 1) / p.rings;

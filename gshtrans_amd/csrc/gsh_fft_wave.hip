@@ -21,6 +21,8 @@
 // are XOR-swizzled so that every 16-lane phase of a 16-byte access touches
 // all 64 banks once.  Real fields are transformed two at a time as the
 // complex ring x_a + i x_b.
+#include <type_traits>
+
 #include "gsh_internal.h"
 
 namespace gsh {
@@ -281,7 +283,7 @@ __device__ __forceinline__ int xch(int c, int b) { return b ^ (2 * c); }
 //    k < N/2 and the conjugates of slot N - k for k > N/2 (c2r symmetry);
 //    slot N/2 (lane 0, register R/2) is loaded by the ring itself.  c2r
 //    ignores Im of bins 0 and N/2.
-template <int R, int REAL, int F32>
+template <int R, int REAL, int F32, int FA32>
 __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_synth_w(FFTParams p) {
   constexpr int W = waves_for(R);
   constexpr int LPW = lds_per_wave(R);
@@ -306,60 +308,84 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_synth_w(FFTParams p)
   const int c = tid & 7, bq = tid >> 3;
   const int cf = (fg * W + c) * (REAL ? 2 : 1);
   const bool cv = cf < p.nfield, cv1 = REAL && cf + 1 < p.nfield;
-  const double* Fc = p.F + f_index(theta, 0, 2 * cf, p.nslot, ncg16);
+  // Bin (re, im) at element i of F: FP64, or FP32 (FA32: the same indices in
+  // a float array) from the split-BF16 synthesis.  FP32 bins are loaded
+  // unconditionally (an absent bin reads the in-bounds element fc and is
+  // zeroed when stored to LDS) and stay raw until then, so that no load
+  // waits for a conversion and all of them are in flight together.
+  using Bin = typename std::conditional<FA32 != 0, float2, double2>::type;
+  const long fc = f_index(theta, 0, 2 * cf, p.nslot, ncg16);
+  auto ldF = [&](bool ok, long i) -> Bin {
+    if constexpr (FA32) {
+      return *reinterpret_cast<const float2*>(reinterpret_cast<const float*>(p.F) + (ok ? i : fc));
+    } else {
+      double2 x = make_double2(0.0, 0.0);
+      if (ok) x = ld2(p.F + i);
+      return x;
+    }
+  };
+  // A loaded bin as stored: FP32 bins that were not asked for are zeros.
+  auto d2 = [](bool ok, const Bin& x) {
+    if constexpr (FA32)
+      return ok ? make_double2((double)x.x, (double)x.y) : make_double2(0.0, 0.0);
+    else
+      return x;
+  };
   double2 v[R];
   if (!REAL) {
-    double2 st[H][CPT];
+    Bin st[H][CPT];
+    bool sok[H][CPT];
 #pragma unroll
     for (int r = 0; r < H; r++)
 #pragma unroll
       for (int i = 0; i < CPT; i++) {
         const int k = r * LPW + 64 * i + bq;
         const bool lo = k <= lmax;
-        st[r][i] = make_double2(0.0, 0.0);
-        if (cv && (lo || k >= N - lmax)) st[r][i] = ld2(Fc + 16 * (long)(lo ? k - m_first : k - N - m_first));
+        sok[r][i] = cv && (lo || k >= N - lmax);
+        st[r][i] = ldF(sok[r][i], fc + 16 * (long)(lo ? k - m_first : k - N - m_first));
       }
     constexpr int RH = (N / 2) / LPW, IH = ((N / 2) % LPW) / 64;  // bin N/2
     const bool both = cv && bq == 0 && N / 2 <= lmax;
-    double2 xm = make_double2(0.0, 0.0);
-    if (both) xm = ld2(Fc + 16 * (long)(-N / 2 - m_first));
+    const Bin xm = ldF(both, fc + 16 * (long)(-N / 2 - m_first));
 #pragma unroll
     for (int r = 0; r < H; r++) {
       if (r > 0) __syncthreads();  // round r - 1 read back
-      if (r == RH) st[RH][IH] = cadd(st[RH][IH], xm);
 #pragma unroll
-      for (int i = 0; i < CPT; i++) lds_all[c * LPW + xch(c, 64 * i + bq)] = st[r][i];
+      for (int i = 0; i < CPT; i++) {
+        double2 x = d2(sok[r][i], st[r][i]);
+        if (r == RH && i == IH) x = cadd(x, d2(both, xm));
+        lds_all[c * LPW + xch(c, 64 * i + bq)] = x;
+      }
       __syncthreads();
 #pragma unroll
       for (int j = 0; j < CPT; j++)
         v[r * CPT + j] = lds_all[wave * LPW + xch(wave, 64 * j + lane)];
     }
   } else {
-    double2 sa[H][CPH], sb[H][CPH];
+    Bin sa[H][CPH], sb[H][CPH];
+    bool aok[H][CPH], bok[H][CPH];
 #pragma unroll
     for (int r = 0; r < H; r++)
 #pragma unroll
       for (int i = 0; i < CPH; i++) {
         const int kk = r * BH + 64 * i + bq;
-        sa[r][i] = make_double2(0.0, 0.0);
-        sb[r][i] = make_double2(0.0, 0.0);
-        if (cv && kk <= lmax) sa[r][i] = ld2(Fc + 16 * (long)kk);
-        if (cv1 && kk <= lmax) sb[r][i] = ld2(Fc + 16 * (long)kk + 2);
+        aok[r][i] = cv && kk <= lmax;
+        bok[r][i] = cv1 && kk <= lmax;
+        sa[r][i] = ldF(aok[r][i], fc + 16 * (long)kk);
+        sb[r][i] = ldF(bok[r][i], fc + 16 * (long)kk + 2);
       }
     // Slot N/2 of this ring's pair (register R/2, lane 0).
-    double2 na = make_double2(0.0, 0.0), nb = make_double2(0.0, 0.0);
-    if (fv && lane == 0 && N / 2 <= lmax) {
-      const double* Fw = p.F + f_index(theta, 0, 2 * f0, p.nslot, ncg16);
-      na = ld2(Fw + 16 * (long)(N / 2));
-      if (f1v) nb = ld2(Fw + 16 * (long)(N / 2) + 2);
-    }
+    const long fw = f_index(theta, 0, 2 * f0, p.nslot, ncg16);
+    const bool nok = fv && lane == 0 && N / 2 <= lmax;
+    const Bin na = ldF(nok, fw + 16 * (long)(N / 2));
+    const Bin nb = ldF(nok && f1v, fw + 16 * (long)(N / 2) + 2);
 #pragma unroll
     for (int r = 0; r < H; r++) {
       if (r > 0) __syncthreads();
 #pragma unroll
       for (int i = 0; i < CPH; i++) {
-        lds_all[c * LPW + xch(c, 64 * i + bq)] = sa[r][i];
-        lds_all[c * LPW + BH + xch(c, 64 * i + bq)] = sb[r][i];
+        lds_all[c * LPW + xch(c, 64 * i + bq)] = d2(aok[r][i], sa[r][i]);
+        lds_all[c * LPW + BH + xch(c, 64 * i + bq)] = d2(bok[r][i], sb[r][i]);
       }
       __syncthreads();
       const double2* lw = lds_all + wave * LPW;
@@ -387,7 +413,7 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_synth_w(FFTParams p)
         v[n1] = in ? X : keep;
       }
     }
-    if (lane == 0) v[R / 2] = make_double2(na.x, nb.x);
+    if (lane == 0) v[R / 2] = make_double2(d2(nok, na).x, d2(nok && f1v, nb).x);
   }
   // Each wave now works in its own LDS region only.
   wave_fft<R, +1>(v, lds_all + wave * LPW, p.twa, p.tw, lane);
@@ -634,14 +660,23 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
 template <int R>
 hipError_t launch_synth_r(const FFTParams& p, hipStream_t s) {
   const dim3 grid(p.nfg, p.ntheta), block(64 * waves_for(R));
-  if (p.real && p.f32)
-    hipLaunchKernelGGL((fft_synth_w<R, 1, 1>), grid, block, 0, s, p);
+  if (p.fa32) {
+    if (p.real && p.f32)
+      hipLaunchKernelGGL((fft_synth_w<R, 1, 1, 1>), grid, block, 0, s, p);
+    else if (p.real)
+      hipLaunchKernelGGL((fft_synth_w<R, 1, 0, 1>), grid, block, 0, s, p);
+    else if (p.f32)
+      hipLaunchKernelGGL((fft_synth_w<R, 0, 1, 1>), grid, block, 0, s, p);
+    else
+      hipLaunchKernelGGL((fft_synth_w<R, 0, 0, 1>), grid, block, 0, s, p);
+  } else if (p.real && p.f32)
+    hipLaunchKernelGGL((fft_synth_w<R, 1, 1, 0>), grid, block, 0, s, p);
   else if (p.real)
-    hipLaunchKernelGGL((fft_synth_w<R, 1, 0>), grid, block, 0, s, p);
+    hipLaunchKernelGGL((fft_synth_w<R, 1, 0, 0>), grid, block, 0, s, p);
   else if (p.f32)
-    hipLaunchKernelGGL((fft_synth_w<R, 0, 1>), grid, block, 0, s, p);
+    hipLaunchKernelGGL((fft_synth_w<R, 0, 1, 0>), grid, block, 0, s, p);
   else
-    hipLaunchKernelGGL((fft_synth_w<R, 0, 0>), grid, block, 0, s, p);
+    hipLaunchKernelGGL((fft_synth_w<R, 0, 0, 0>), grid, block, 0, s, p);
   return hipGetLastError();
 }
 

@@ -59,6 +59,8 @@ struct LegParams {
   long cstride;
   // F work buffer
   double* F;
+  int f32out;     // split-BF16 synthesis: F written as float elements (same
+                  // indices; the wavefront ring FFT reads them)
   // per-grid / per-n tables
   const double* cosv;   // cos(theta_t), t < ntheta
   const double* seeds;  // Ortho-normalised d^{l0}_{m n}(theta_t) at [(m+L)*ntheta + t]
@@ -236,8 +238,10 @@ struct FFTParams {
   const double* wq;   // analysis weights w_t * 2 pi / N (all 1 for the adjoint)
   int adjoint;        // adjoint of the synthesis: real fields take the c2r
                       // bin weights (2 for 0 < k < N/2, Re only at 0, N/2)
-  double* F;          // bins: FA (analysis; float elements when fa32) or F
-  int fa32;           // analysis: FA in FP32 (split-BF16 modes)
+  double* F;          // bins: FA (analysis) or F (synthesis); float elements
+                      // when fa32
+  int fa32;           // bins in FP32 (split-BF16 modes; the synthesis only
+                      // through the wavefront kernels)
   const void* fin;    // analysis input field
   const void* fin2;   // analysis of a pointwise expression of two fields: the
                       // second operand (same layout as fin), or null

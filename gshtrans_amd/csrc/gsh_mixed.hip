@@ -426,12 +426,22 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
   for (int t = 0; t < 2; t++) {
     const int col = (tile0 + t) * 16 + (lane & 15);
     if (col >= p.ncol) continue;
-    double* fp = p.F + f_index(row0, slot, col, p.nslot, ncg16);
+    const long fi = f_index(row0, slot, col, p.nslot, ncg16);
+    if (p.f32out) {
+      float* fp = reinterpret_cast<float*>(p.F) + fi;
 #pragma unroll
-    for (int g = 0; g < 8; g++)
+      for (int g = 0; g < 8; g++)
 #pragma unroll
-      for (int r = 0; r < 4; r++)
-        if (row0 + g * 16 + r < nt) fp[(g * 16 + r) * rs] = acc[g][t][r];
+        for (int r = 0; r < 4; r++)
+          if (row0 + g * 16 + r < nt) fp[(g * 16 + r) * rs] = (float)acc[g][t][r];
+    } else {
+      double* fp = p.F + fi;
+#pragma unroll
+      for (int g = 0; g < 8; g++)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          if (row0 + g * 16 + r < nt) fp[(g * 16 + r) * rs] = acc[g][t][r];
+    }
   }
 }
 
@@ -930,7 +940,11 @@ __global__ __launch_bounds__(256) void tail_reduce(LegParams p) {
   for (int col = threadIdx.x; col < p.ncol; col += 256) {
     double sum = 0.0;
     for (int c = 0; c < nchunk; c++) sum += pp[c * cstride + col];
-    p.F[f_index(t, slot, col, p.nslot, ncg16)] = sum;
+    const long fi = f_index(t, slot, col, p.nslot, ncg16);
+    if (p.f32out)
+      reinterpret_cast<float*>(p.F)[fi] = (float)sum;
+    else
+      p.F[fi] = sum;
   }
 }
 

@@ -78,8 +78,11 @@ def test_mixed_synthesis_last_rows(torch, prec, L, n, real, batch):
     its MFMA items: the coefficient split adds their partial sums per block
     from the FP64 coefficients and the FP64 recurrence values, and a last
     kernel sums the blocks.  The whole field meets the mode's bar, and those
-    rows equal the FP64 transform to 1e-12 in either mode (a wrong degree,
-    order, column or block shows far above that)."""
+    rows equal the FP64 transform to 1e-12 in either mode when the sums reach
+    the ring FFT in FP64 (the LDS ring FFT, here L = 257), and to one FP32
+    rounding of the bins (2^-23) when they reach the wavefront ring FFT in
+    FP32 (L = 256, 512, 1024); a wrong degree, order, column or block shows
+    far above either."""
     say(f"[mixed synthesis, last rows] {prec} L={L} n={n} real={real} batch={batch}")
     g = grid(L)
     c = torch.from_numpy(G.random_coefficients(L, n, real=real, seed=SEED0 + 7 * L + n,
@@ -91,10 +94,11 @@ def test_mixed_synthesis_last_rows(torch, prec, L, n, real, batch):
     assert e.max() <= BARS[prec]
     nt = L + 1
     assert 1 <= nt % 256 <= 2
+    row_bar = 2.0 ** -23 if 2 * L in (512, 1024, 2048) else 1e-12
     for t in range(nt // 256 * 256, nt):
         e = per_field_rel(torch, fmx[:, t:t + 1], f64[:, t:t + 1])
-        say(f"    row {t}: max {e.max():.2e}")
-        assert e.max() <= 1e-12
+        say(f"    row {t}: max {e.max():.2e} (bar {row_bar:.1e})")
+        assert e.max() <= row_bar
     # The row before them comes from the MFMA items.
     t = nt // 256 * 256 - 1
     assert per_field_rel(torch, fmx[:, t:t + 1], f64[:, t:t + 1]).max() > 1e-12
