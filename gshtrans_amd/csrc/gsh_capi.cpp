@@ -619,12 +619,14 @@ int ensure_synth_worklist(gsh_grid_s* g, int family, int lmax, int real, int ngr
   return GSH_OK;
 }
 
-// Work list of the coefficient split: items (slot, chunk | tile << 16)
-// chunk-major (all orders' chunk 0, then chunk 1, ...), padded to a multiple
-// of 32 (8 XCDs x 4 waves per workgroup); per = items / 32.
 // Work list of coef_split: items (group gs of the 8 slots 8 gs .. 8 gs + 7,
-// chunk | tile << 16), chunks innermost, so that consecutive items -- which
-// run on one XCD -- share the degrees their windows overlap in.
+// chunk | tile << 16).  Groups go in bands of kSplitBand neighbours, and
+// within a band the list runs tile, then chunk, then group: consecutive
+// items -- which run on one XCD -- are neighbouring order groups at the same
+// degrees, whose 128-byte pieces of the coefficient rows (8 orders of one
+// degree and field, not line-aligned) share cache lines, and the next chunk
+// of a group, whose window overlaps the previous one by 8 degrees, follows
+// a band later.
 int ensure_split_worklist(gsh_grid_s* g, int lmax, int n, int real, int ntile, WorkList* out) {
   auto key = std::make_tuple(31, lmax, n, real, std::vector<int>{ntile});
   auto it = g->worklists.find(key);
@@ -634,17 +636,25 @@ int ensure_split_worklist(gsh_grid_s* g, int lmax, int n, int real, int ntile, W
   }
   const int nslot = real ? lmax + 1 : 2 * lmax + 1;
   const int m_first = real ? 0 : -lmax;
-  std::vector<uint2> items;
-  for (int gs = 0; 8 * gs < nslot; gs++) {
-    int nch = 0;  // chunks of the group's order with the most
+  const int ngs = (nslot + 7) / 8;
+  std::vector<int> nch(ngs, 0);  // chunks of each group's order with the most
+  for (int gs = 0; gs < ngs; gs++)
     for (int sl = 8 * gs; sl < std::min(8 * gs + 8, nslot); sl++) {
       const int l0 = std::max(std::abs(sl + m_first), std::abs(n));
-      nch = std::max(nch, (lmax - l0 + 32) / 32);
+      nch[gs] = std::max(nch[gs], (lmax - l0 + 32) / 32);
     }
+  constexpr int kSplitBand = 4;
+  std::vector<uint2> items;
+  for (int b0 = 0; b0 < ngs; b0 += kSplitBand) {
+    const int b1 = std::min(b0 + kSplitBand, ngs);
+    int cmax = 0;
+    for (int gs = b0; gs < b1; gs++) cmax = std::max(cmax, nch[gs]);
     for (int t = 0; t < ntile; t++)
-      for (int c = 0; c < nch; c++)
-        items.push_back(make_uint2(static_cast<unsigned>(gs),
-                                   static_cast<unsigned>(c) | (static_cast<unsigned>(t) << 16)));
+      for (int c = 0; c < cmax; c++)
+        for (int gs = b0; gs < b1; gs++)
+          if (c < nch[gs])
+            items.push_back(make_uint2(static_cast<unsigned>(gs),
+                                       static_cast<unsigned>(c) | (static_cast<unsigned>(t) << 16)));
   }
   while (items.size() % 8 != 0 || items.empty()) items.push_back(make_uint2(~0u, 0u));
   WorkList w;
