@@ -47,15 +47,18 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "transform_kernels_lmax1024_b64_co
 # Kernel instantiations of each work family per mode (real and complex
 # batch), as regular expressions over the summary's kernel names.
 FAMILY_KERNELS = {
+    # split-BF16 analysis: the 128-degree-pass kernel the single calls use
     "legendre_analysis": {"fp64": r"leg_anal_f64<4, false>", "fp32": r"leg_anal_f64<4, true>",
-                          "bf16x3": r"leg_anal_bf16<2, \d>", "bf16x6": r"leg_anal_bf16<3, \d>"},
+                          "bf16x3": r"leg_anal_bf16_w<2, \d>", "bf16x6": r"leg_anal_bf16_w<3, \d>"},
     "legendre_synthesis": {"fp64": r"leg_synth_f64<2, \d, false>",
                            "fp32": r"leg_synth_f64<2, \d, true>",
                            "bf16x3": r"leg_synth_bf16<2, \d>", "bf16x6": r"leg_synth_bf16<3, \d>"},
     # FA in FP64 (fp64, fp32) or FP32 (split-BF16 modes): last template argument
     "fft_analysis": {"fp64": r"fft_anal_w<32, \d, 0, 0>", "fp32": r"fft_anal_w<32, \d, 0, 0>",
                      "bf16x3": r"fft_anal_w<32, \d, 0, 1>", "bf16x6": r"fft_anal_w<32, \d, 0, 1>"},
-    "fft_synthesis": r"fft_synth_w<32, \d, 0>",
+    # F in FP64 (fp64, fp32) or FP32 (split-BF16 modes): last template argument
+    "fft_synthesis": {"fp64": r"fft_synth_w<32, \d, 0, 0>", "fp32": r"fft_synth_w<32, \d, 0, 0>",
+                      "bf16x3": r"fft_synth_w<32, \d, 0, 1>", "bf16x6": r"fft_synth_w<32, \d, 0, 1>"},
     "coef_split": {"bf16x3": r"coef_split<2, \d>", "bf16x6": r"coef_split<3, \d>"},
 }
 HEADLINE = "bf16x6"            # <= 1e-7 relative L2 against FP64

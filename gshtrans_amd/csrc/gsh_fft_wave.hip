@@ -310,14 +310,16 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_synth_w(FFTParams p)
   const bool cv = cf < p.nfield, cv1 = REAL && cf + 1 < p.nfield;
   // Bin (re, im) at element i of F: FP64, or FP32 (FA32: the same indices in
   // a float array) from the split-BF16 synthesis.  FP32 bins are loaded
-  // unconditionally (an absent bin reads the in-bounds element fc and is
-  // zeroed when stored to LDS) and stay raw until then, so that no load
-  // waits for a conversion and all of them are in flight together.
+  // unconditionally (an absent bin reads element f00, slot 0 and column 0 of
+  // this colatitude, and is zeroed when stored to LDS) and stay raw until
+  // then, so that no load waits for a conversion and all of them are in
+  // flight together.
   using Bin = typename std::conditional<FA32 != 0, float2, double2>::type;
   const long fc = f_index(theta, 0, 2 * cf, p.nslot, ncg16);
+  const long f00 = f_index(theta, 0, 0, p.nslot, ncg16);
   auto ldF = [&](bool ok, long i) -> Bin {
     if constexpr (FA32) {
-      return *reinterpret_cast<const float2*>(reinterpret_cast<const float*>(p.F) + (ok ? i : fc));
+      return *reinterpret_cast<const float2*>(reinterpret_cast<const float*>(p.F) + (ok ? i : f00));
     } else {
       double2 x = make_double2(0.0, 0.0);
       if (ok) x = ld2(p.F + i);
