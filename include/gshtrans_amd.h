@@ -104,7 +104,9 @@ enum gsh_range { GSH_ALL = 0, GSH_NONNEGATIVE = 1, GSH_SINGLE = 2 };
 enum gsh_value_kind { GSH_REAL = 0, GSH_COMPLEX = 1 };
 
 /* Arithmetic of the Legendre contraction.  The Wigner recurrence, its seeds,
- * the quadrature weights and the FFT are FP64 in every mode. */
+ * the quadrature weights and the FFT arithmetic are FP64 in every mode; the
+ * split-BF16 modes hand the order bins between the Legendre sums and the
+ * wavefront ring FFT (ring lengths 512, 1024, 2048) over in FP32. */
 enum gsh_precision {
   GSH_FP64 = 0,   /* FP64 MFMA, the reference's arithmetic                 */
   GSH_FP32 = 1,   /* FP32 MFMA on FP32-rounded operands                     */
