@@ -1179,7 +1179,11 @@ int run_forward(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
     }));
     return GSH_OK;
   }
-  // 128-degree passes unless asked for 64 (o.anal_pass; 0 takes the default).
+  // 128-degree passes unless asked for 64 (o.anal_pass; 0 takes the
+  // default).  Upper index 0 runs over the northern colatitudes only (the
+  // Gauss-Legendre nodes are symmetric about the equator and d^l_{m0} has
+  // parity (-1)^(l+m) under theta -> pi - theta).
+  const bool sym = n == 0;
   const bool wide = (o.anal_pass ? o.anal_pass : kDefaultAnalPass) == gsh::kWideAnalPass;
   {
     // 128-column super-groups of the split-BF16 kernel.
@@ -1197,7 +1201,7 @@ int run_forward(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
   }
   const int ns = precision == GSH_BF16X6 ? 3 : 2;
   GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_ANALYSIS, s,
-                [&] { return gsh::launch_leg_anal_bf16(p, ns, s, wide ? 1 : 0); }));
+                [&] { return gsh::launch_leg_anal_bf16(p, ns, s, (wide ? 1 : 0) + (sym ? 2 : 0)); }));
   return GSH_OK;
 }
 

@@ -279,7 +279,9 @@ hipError_t launch_leg_synth_bf16(const LegParams& p, int ns, hipStream_t s);
 hipError_t launch_tail_dvals(const LegParams& p, hipStream_t s);
 hipError_t launch_tail_reduce(const LegParams& p, hipStream_t s);
 // wide: 128-degree passes (leg_anal_bf16_w; work list of family 3) instead of 64.
-hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s, int wide = 0);
+// form: 0 passes of 64 degrees, 1 passes of 128 degrees; 2 and 3 the same
+// over the northern colatitudes (upper index 0, symmetric grids).
+hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s, int form = 0);
 
 // FP32 <-> FP64 coefficient staging of batch plans (gsh_batch.hip); strides
 // and sizes in complex elements.

@@ -686,6 +686,272 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
 }
 
 // ---------------------------------------------------------------------------
+// Analysis for upper index 0 over the northern colatitudes.  Gauss-Legendre
+// nodes are symmetric about the equator and d^l_{m0}(pi - theta) =
+// (-1)^(l+m) d^l_{m0}(theta), so sum_t d_l(theta_t) FA(t) runs over the
+// northern t (the equator once) against FA(t) + FA(nt-1-t) for even l + m and
+// FA(t) - FA(nt-1-t) for odd l + m: half the colatitudes, recurrence values
+// and MFMAs of leg_anal_bf16, whose items, production and chunk order this
+// kernel keeps.  The degrees of a pass are stored in LDS by parity class
+// (tiles 0, 1: l + m even; tiles 2, 3: odd), each class multiplied with its
+// own B operand; the sums FA(t) +- FA(nt-1-t) of the FP32 bins round once to
+// FP32 before the split, and each chunk's sums are flushed to FP64 on their
+// own, as in leg_anal_bf16_ws (the same bits).
+// ---------------------------------------------------------------------------
+template <int NS, int REAL>
+__global__ __launch_bounds__(512) void leg_anal_bf16_s(LegParams p) {
+  static_assert(kPass == 64 && kCk == 16, "4 degree quarters of 16 per pass");
+  // Rows of 128 colatitudes padded to 136 (272 bytes): the 16 lanes of a
+  // 16-byte A-operand read hit 16 consecutive rows, which then start 4 banks
+  // apart and cover all 64 banks once (256-byte rows would all map to the
+  // same 4 banks).
+  __shared__ __bf16 Dl[2][NS][64][136];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  // Work item: (order, pass, column super-group).
+  const uint2 it = work_item(p.items, p.items_per);
+  if (it.x == ~0u) return;
+  const int mw = it.x;
+  const int pass = it.y & 0xffff;
+  const int csg = it.y >> 16;
+  const int m = order_of(mw, REAL);
+  const int slot = m - p.m_first;
+  const int l0 = max(abs(m), p.nabs);
+  const int nt = p.ntheta;
+  const int nts = (nt + 1) >> 1;      // northern colatitudes, the equator included
+  const int ngrp = (nts + 127) >> 7;
+  const int nkc = (nts + 31) >> 5;    // 32-colatitude K chunks of the north
+  const int lp = l0 + kPass * pass;  // first degree of the pass
+  const int par0 = (lp + m) & 1;     // parity of l + m at the pass's first degree
+  // Recurrence coefficients of the pass: entries lp+1 .. lp+64 (entry l for
+  // degree l; entries up to L + kRecPad exist).
+  rec_t* rq = rec_ptr(p.rec, ld_uniform(p.roff, m + p.L) - l0);
+
+  // Producer share of this wave: block half bh, degree quarter q.  Its
+  // recurrence starts at degree lp + 16 q from checkpoint kq (the seed for
+  // kq = 0).  Quarters past L (only garbage rows, never written) clamp to
+  // the last checkpoint so every load stays in the table.
+  const int bh = wave & 1, q = wave >> 1;
+  const int nck = (p.L - l0 + kCk) / kCk;  // checkpoints 1 .. nck-1 exist
+  const int kq = min((kPass / kCk) * pass + q, nck - 1);
+  const double2* ckp = kq == 0 ? p.ck : p.ck + (ld_uniform(p.ckoff, m + p.L) + kq - 1) * (long)nt;
+  const double* sdp = p.seeds + (long)(m + p.L) * nt;
+  // The state and cosine of group g's colatitude are loaded into registers
+  // one group before they are used, after the previous production (so the
+  // loads land in registers just freed and nothing waits on them until the
+  // next production).  Colatitudes past ntheta load the last one's values:
+  // their products meet zero B operands, and the values stay finite.
+  double nx_seed, nx_cs;
+  double2 nx_ck;
+  auto prefetch = [&](int g) {
+    const int tc = min((2 * g + bh) * 64 + lane, nts - 1);
+    nx_seed = sdp[tc];
+    nx_ck = ckp[tc];
+    nx_cs = p.cosv[tc];
+  };
+  // Production of this wave's 16 degrees x 64 colatitudes of a group into
+  // buffer buf, in four parts of 4 degrees (one per K chunk of the group
+  // multiplied meanwhile, so the scheduler can interleave their VALU work
+  // with that chunk's MFMAs).  begin() takes the prefetched state.
+  double pe0, pe1, pcs;
+  auto begin = [&]() {
+    pcs = nx_cs;
+    pe0 = kq == 0 ? nx_seed : nx_ck.x;
+    pe1 = kq == 0 ? 0.0 : nx_ck.y;
+  };
+  // Degree lp + d sits in LDS row 32 c + d / 2 of its parity class c
+  // ((d + par0) % 2; class 0: l + m even): tiles 0, 1 hold class 0, tiles
+  // 2, 3 class 1.
+  auto row_of = [&](int d) { return 32 * ((d + par0) & 1) + (d >> 1); };
+  auto produce4 = [&](int buf, int part) {
+    const int eq = lp + 16 * q + 1;  // entry of step 0, wave-uniform
+#pragma unroll
+    for (int k = 4 * part; k < 4 * part + 4; k += 2) {
+      const double v0 = pe0;
+      rec_step(rec_entry(rq, eq + k), pcs, pe0, pe1);
+      const double v1 = pe0;
+      rec_step(rec_entry(rq, eq + k + 1), pcs, pe0, pe1);
+      unsigned o[NS];
+      split_pair<NS>(v0, v1, o);
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        const bf16x2 pr = __builtin_bit_cast(bf16x2, o[s]);
+        Dl[buf][s][row_of(16 * q + k)][bh * 64 + lane] = pr[0];
+        Dl[buf][s][row_of(16 * q + k + 1)][bh * 64 + lane] = pr[1];
+      }
+    }
+  };
+
+  // Consumer role.
+  // Wave w multiplies all 4 degree tiles with column tile w, so every B
+  // operand block is loaded by one wave of the workgroup (the A operands
+  // come from LDS).  B comes straight from the FP32 bins FA32: lane (q4, c) =
+  // (lane / 16, lane % 16) of K chunk kc takes colatitudes 32 kc + 8 q4 + e
+  // (e = 0..7) of column 16 tile + c, i.e. FA32 entries 16 e past its base
+  // in quad 8 kc + 2 q4 (each e 64 contiguous bytes per 16 lanes), split
+  // into bf16 planes just before the chunk's MFMAs.  A tile past the last
+  // column group reads the last one: its outputs are never stored, and a
+  // column only meets its own B values, so nothing else needs those columns
+  // zero.  Reads of the last block run up to 8 quads past it (the FA buffer
+  // is padded for that); colatitudes from ntheta on are zeroed at the split
+  // of the last group's chunks.  Callers pass chunks below nkc only.
+  struct RawB {
+    float x[8];
+  };
+  const int tile = csg * 8 + wave;
+  const float* ft = p.FA32 + ((long)slot * p.n_cg + min(tile, p.n_cg - 1)) * p.nquad * 64 + (lane & 15);
+  const float* fb = ft + 2 * (lane >> 4) * 64;
+  // Chunk kc's colatitudes t = 32 kc + 8 (lane / 16) + e (X) and their
+  // mirrors nt - 1 - t (Y; a mirror below 0, past the north, reads t = 0).
+  auto loadB = [&](int kc, RawB& X, RawB& Y) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      X.x[e] = fb[(long)kc * 512 + 16 * e];
+      const int tm = max(nt - 1 - (32 * kc + 8 * (lane >> 4) + e), 0);
+      Y.x[e] = ft[(long)(tm >> 2) * 64 + (tm & 3) * 16];
+    }
+  };
+  // B operands of a chunk: FA(t) + FA(nt - 1 - t) (sign +1) or
+  // FA(t) - FA(nt - 1 - t) (sign -1) for northern t below the equator, FA(t)
+  // (+1) and 0 (-1) at the equator, 0 past the north.
+  auto combine = [&](const RawB& X, const RawB& Y, int kc, float sign, RawB& Z) {
+    const int th = 32 * kc + 8 * (lane >> 4);
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int t = th + e;
+      const float y = 2 * t == nt - 1 ? (sign > 0.f ? 0.0f : X.x[e]) : Y.x[e];
+      Z.x[e] = t < nts ? X.x[e] + sign * y : 0.0f;
+    }
+  };
+  auto splitB = [&](const RawB& X, uint4 (&B)[1][NS], int kc, bool last) {
+    unsigned pl[NS][4];
+    const int th = 32 * kc + 8 * (lane >> 4);  // colatitude of e = 0
+#pragma unroll
+    for (int e2 = 0; e2 < 4; e2++) {
+      float x0 = X.x[2 * e2], x1 = X.x[2 * e2 + 1];
+      if (last) {
+        x0 = th + 2 * e2 < nt ? x0 : 0.0f;
+        x1 = th + 2 * e2 + 1 < nt ? x1 : 0.0f;
+      }
+      unsigned o[NS];
+      split_pair_f<NS>(x0, x1, o);
+#pragma unroll
+      for (int s = 0; s < NS; s++) pl[s][e2] = o[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) B[0][s] = make_uint4(pl[s][0], pl[s][1], pl[s][2], pl[s][3]);
+  };
+
+  d4 acc[4][1];
+#pragma unroll
+  for (int i = 0; i < 4; i++) acc[i][0] = d4{0.0, 0.0, 0.0, 0.0};
+  // Each chunk's sums (cp) are flushed to FP64 after the next chunk's MFMAs
+  // are issued, one chunk at a time (no pairs), in the order of
+  // leg_anal_bf16_ws, so both pass lengths give the same bits.
+  f4 cp[4][1];
+#pragma unroll
+  for (int i = 0; i < 4; i++) cp[i][0] = f4{0.f, 0.f, 0.f, 0.f};
+  auto step = [&](int buf, int kc, const RawB& X, const RawB& Y) {
+    f4 c[4][1];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {  // parity class h: tiles 2 h, 2 h + 1
+      RawB Z;
+      combine(X, Y, kc, h == 0 ? 1.0f : -1.0f, Z);
+      uint4 B[1][NS];
+      splitB(Z, B, kc, false);
+      uint4 A[2][NS];
+#pragma unroll
+      for (int lt = 0; lt < 2; lt++)
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+          A[lt][s] = *reinterpret_cast<const uint4*>(
+              &Dl[buf][s][(2 * h + lt) * 16 + (lane & 15)][(kc & 3) * 32 + 8 * (lane >> 4)]);
+      f4 ch[2][1];
+      split_mma<NS, 2, 1>(A, B, ch);
+      c[2 * h][0] = ch[0][0];
+      c[2 * h + 1][0] = ch[1][0];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      flush(acc[i][0], cp[i][0]);
+      cp[i][0] = c[i][0];
+    }
+  };
+
+  prefetch(0);
+  begin();
+#pragma unroll
+  for (int part = 0; part < 4; part++) produce4(0, part);
+  prefetch(1);
+  __syncthreads();
+  // B operands: chunk c sits in buffer X[c & 3], loaded kAhead chunks before
+  // its MFMAs (into the buffer chunk c - 4 + kAhead used).  Loads clamp to
+  // the last chunk nkc - 1; what they bring past it is never multiplied.
+  // Every group but the last has four chunks below nkc (nt > 128 (ngrp-1)),
+  // and the last group's first chunk is below nkc too; only the last group's
+  // chunks hold colatitudes past ntheta.  The scheduling barriers keep each
+  // chunk's loads, production part and MFMAs together.
+  constexpr int kAhead = 3;
+  const int kcl = nkc - 1;
+  RawB X[4], Y[4];
+#pragma unroll
+  for (int j = 0; j < kAhead; j++) loadB(min(j, kcl), X[j], Y[j]);
+  for (int g = 0; g + 1 < ngrp; g++) {
+    const int buf = g & 1;
+    const int kc0 = 4 * g;
+    begin();
+    prefetch(g + 2);  // a whole group ahead of its use
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      loadB(min(kc0 + j + kAhead, kcl), X[(j + kAhead) & 3], Y[(j + kAhead) & 3]);
+      produce4(buf ^ 1, j);
+      step(buf, kc0 + j, X[j], Y[j]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+  }
+  {
+    // Last group: chunks 4 (ngrp-1) .. nkc-1.
+    const int buf = (ngrp - 1) & 1;
+    const int kc0 = 4 * (ngrp - 1);
+    step(buf, kc0, X[0], Y[0]);
+    if (kc0 + 1 < nkc) {
+      step(buf, kc0 + 1, X[1], Y[1]);
+      if (kc0 + 2 < nkc) {
+        step(buf, kc0 + 2, X[2], Y[2]);
+        if (kc0 + 3 < nkc) {
+          loadB(kc0 + 3, X[3], Y[3]);
+          step(buf, kc0 + 3, X[3], Y[3]);
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) flush(acc[i][0], cp[i][0]);
+  }
+
+  // Row 16 lt + 4 (lane / 16) + r of class lt / 2 is degree lp + d, d = 2 i
+  // + (lt / 2 + par0) % 2 with i its index in the class; column 16 tile +
+  // lane % 16.
+  const int col = tile * 16 + (lane & 15);
+  if (col >= p.ncol) return;
+  const int lf = field_lmax(p, col >> 1);
+  const long cbase = (long)field_index(p.fmap, col >> 1) * p.cstride;
+#pragma unroll
+  for (int lt = 0; lt < 4; lt++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = 16 * (lt & 1) + 4 * (lane >> 4) + r;
+      const int l = lp + 2 * i + (((lt >> 1) + par0) & 1);
+      if (l > lf) continue;
+      const long ci = (cbase + coef_index(l, m, p.nabs, REAL)) * 2 + (col & 1);
+      double v = acc[lt][0][r];
+      if (p.accumulate) v += p.cout[ci];
+      if (p.zero_last && lf == p.L && l == lf && m == lf) v = 0.0;
+      p.cout[ci] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Analysis over 128-degree passes (kWidePass): the same contraction as
 // leg_anal_bf16 with eight degree tiles per wave, so each split B operand
 // (the VALU-heavy part of a K chunk) feeds twice as many MFMAs.  Colatitudes
@@ -885,6 +1151,234 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_w(LegParams p) {
 }
 
 // ---------------------------------------------------------------------------
+// Analysis for upper index 0 over the northern colatitudes, 128-degree
+// passes.  Gauss-Legendre nodes are symmetric about the equator and
+// d^l_{m0}(pi - theta) = (-1)^(l+m) d^l_{m0}(theta), so sum_t d_l(theta_t)
+// FA(t) runs over the northern t (the equator once) against FA(t) +
+// FA(nt-1-t) for even l + m and FA(t) - FA(nt-1-t) for odd l + m: half the
+// colatitudes, recurrence values, A-operand reads and MFMAs of
+// leg_anal_bf16_w, whose items, production and chunk order this kernel
+// keeps.  The degrees of a pass sit in LDS by parity class (tiles 0-3:
+// l + m even, 4-7: odd), each class multiplied with its own B operand; the
+// sums FA(t) +- FA(nt-1-t) of the FP32 bins round once to FP32 before the
+// split, and each chunk's sums are flushed to FP64 on their own (no pairs).
+// ---------------------------------------------------------------------------
+template <int NS, int REAL>
+__global__ __launch_bounds__(512) void leg_anal_bf16_ws(LegParams p) {
+  static_assert(kCk == 16 && kWidePass == 8 * kCk, "8 degree tiles of 16 per pass");
+  // Rows of 64 colatitudes padded to 72 (144 bytes, 36 banks apart): the 16
+  // rows of a 16-byte A-operand read start at 16 different multiples of 4
+  // banks and cover all 64 banks once.
+  __shared__ __bf16 Dl[2][NS][kWidePass][72];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const uint2 it = work_item(p.items, p.items_per);
+  if (it.x == ~0u) return;
+  const int mw = it.x;
+  const int pass = it.y & 0xffff;
+  const int csg = it.y >> 16;
+  const int m = order_of(mw, REAL);
+  const int slot = m - p.m_first;
+  const int l0 = max(abs(m), p.nabs);
+  const int nt = p.ntheta;
+  const int nts = (nt + 1) >> 1;  // northern colatitudes, the equator included
+  const int ngrp = (nts + 63) >> 6;
+  const int nkc = (nts + 31) >> 5;
+  const int lp = l0 + kWidePass * pass;  // first degree of the pass
+  const int par0 = (lp + m) & 1;         // parity of l + m at the pass's first degree
+  // Recurrence coefficients of the pass: entries lp+1 .. lp+128 (entry l for
+  // degree l).  Entries up to L + kRecPad exist: kRecPad covers the 128
+  // entries only from a pass start at most L - 63, so the last pass's tail
+  // reads clamp.
+  rec_t* rq = rec_ptr(p.rec, ld_uniform(p.roff, m + p.L) - l0);
+
+  // Producer share: degrees lp + 16 w .. + 15 of every colatitude of a group,
+  // from checkpoint kq (the seed for kq = 0; past L clamped to the last).
+  const int nck = (p.L - l0 + kCk) / kCk;  // checkpoints 1 .. nck-1 exist
+  const int kq = min((kWidePass / kCk) * pass + wave, nck - 1);
+  const double2* ckp = kq == 0 ? p.ck : p.ck + (ld_uniform(p.ckoff, m + p.L) + kq - 1) * (long)nt;
+  const double* sdp = p.seeds + (long)(m + p.L) * nt;
+  double nx_seed, nx_cs;
+  double2 nx_ck;
+  auto prefetch = [&](int g) {
+    const int tc = min(64 * g + lane, nts - 1);
+    nx_seed = sdp[tc];
+    nx_ck = ckp[tc];
+    nx_cs = p.cosv[tc];
+  };
+  double pe0, pe1, pcs;
+  auto begin = [&]() {
+    pcs = nx_cs;
+    pe0 = kq == 0 ? nx_seed : nx_ck.x;
+    pe1 = kq == 0 ? 0.0 : nx_ck.y;
+  };
+  // Degree lp + d sits in LDS row 64 c + d / 2 of its parity class c
+  // ((d + par0) % 2; class 0: l + m even): tiles 0-3 hold class 0, tiles
+  // 4-7 class 1.
+  auto row_of = [&](int d) { return 64 * ((d + par0) & 1) + (d >> 1); };
+  // Half `part` (8 degrees) of this wave's production for a group.
+  auto produce8 = [&](int buf, int part) {
+    const int ew = lp + 16 * wave + 1;  // entry of step 0, wave-uniform
+#pragma unroll
+    for (int k = 8 * part; k < 8 * part + 8; k += 2) {
+      const double v0 = pe0;
+      rec_step(rec_entry(rq, min(ew + k, p.L + kRecPad)), pcs, pe0, pe1);
+      const double v1 = pe0;
+      rec_step(rec_entry(rq, min(ew + k + 1, p.L + kRecPad)), pcs, pe0, pe1);
+      unsigned o[NS];
+      split_pair<NS>(v0, v1, o);
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        const bf16x2 pr = __builtin_bit_cast(bf16x2, o[s]);
+        Dl[buf][s][row_of(16 * wave + k)][lane] = pr[0];
+        Dl[buf][s][row_of(16 * wave + k + 1)][lane] = pr[1];
+      }
+    }
+  };
+
+  // Consumer: column tile w, B straight from the FP32 bins as in
+  // leg_anal_bf16 (colatitudes from ntheta on zeroed at the split).
+  struct RawB {
+    float x[8];
+  };
+  const int tile = csg * 8 + wave;
+  const float* ft = p.FA32 + ((long)slot * p.n_cg + min(tile, p.n_cg - 1)) * p.nquad * 64 + (lane & 15);
+  const float* fb = ft + 2 * (lane >> 4) * 64;
+  // Chunk kc's colatitudes t = 32 kc + 8 (lane / 16) + e (X) and their
+  // mirrors nt - 1 - t (Y; a mirror below 0, past the north, reads t = 0).
+  auto loadB = [&](int kc, RawB& X, RawB& Y) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      X.x[e] = fb[(long)kc * 512 + 16 * e];
+      const int tm = max(nt - 1 - (32 * kc + 8 * (lane >> 4) + e), 0);
+      Y.x[e] = ft[(long)(tm >> 2) * 64 + (tm & 3) * 16];
+    }
+  };
+  // B operands of a chunk: FA(t) + FA(nt - 1 - t) (sign +1) or
+  // FA(t) - FA(nt - 1 - t) (sign -1) for northern t below the equator, FA(t)
+  // (+1) and 0 (-1) at the equator, 0 past the north.
+  auto combine = [&](const RawB& X, const RawB& Y, int kc, float sign, RawB& Z) {
+    const int th = 32 * kc + 8 * (lane >> 4);
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int t = th + e;
+      const float y = 2 * t == nt - 1 ? (sign > 0.f ? 0.0f : X.x[e]) : Y.x[e];
+      Z.x[e] = t < nts ? X.x[e] + sign * y : 0.0f;
+    }
+  };
+  auto splitB = [&](const RawB& X, uint4 (&B)[1][NS], int kc, bool last) {
+    unsigned pl[NS][4];
+    const int th = 32 * kc + 8 * (lane >> 4);
+#pragma unroll
+    for (int e2 = 0; e2 < 4; e2++) {
+      float x0 = X.x[2 * e2], x1 = X.x[2 * e2 + 1];
+      if (last) {
+        x0 = th + 2 * e2 < nt ? x0 : 0.0f;
+        x1 = th + 2 * e2 + 1 < nt ? x1 : 0.0f;
+      }
+      unsigned o[NS];
+      split_pair_f<NS>(x0, x1, o);
+#pragma unroll
+      for (int s = 0; s < NS; s++) pl[s][e2] = o[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) B[0][s] = make_uint4(pl[s][0], pl[s][1], pl[s][2], pl[s][3]);
+  };
+
+  d4 acc[8][1];
+#pragma unroll
+  for (int i = 0; i < 8; i++) acc[i][0] = d4{0.0, 0.0, 0.0, 0.0};
+  // Each chunk's sums (cp) are flushed to FP64 after the next chunk's MFMAs
+  // are issued (no pairs: the registers of a second set hold the mirror
+  // rows' bins instead).
+  f4 cp[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) cp[i] = f4{0.f, 0.f, 0.f, 0.f};
+  auto step = [&](int buf, int kc, const RawB& X, const RawB& Y) {
+#pragma unroll
+    for (int half = 0; half < 2; half++) {  // parity class half: tiles 4 half .. + 3
+      uint4 B[1][NS];
+      {
+        RawB Z;
+        combine(X, Y, kc, half == 0 ? 1.0f : -1.0f, Z);
+        splitB(Z, B, kc, false);
+      }
+      uint4 A[4][NS];
+#pragma unroll
+      for (int lt = 0; lt < 4; lt++)
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+          A[lt][s] = *reinterpret_cast<const uint4*>(
+              &Dl[buf][s][(4 * half + lt) * 16 + (lane & 15)][(kc & 1) * 32 + 8 * (lane >> 4)]);
+      f4 c[4][1];
+      split_mma<NS, 4, 1>(A, B, c);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int t = 4 * half + i;
+        flush(acc[t][0], cp[t]);
+        cp[t] = c[i][0];
+      }
+    }
+  };
+
+  prefetch(0);
+  begin();
+  produce8(0, 0);
+  produce8(0, 1);
+  prefetch(1);
+  __syncthreads();
+  // B operands one chunk ahead: chunk c in X[c & 1], its mirrors in Y[c & 1].
+  const int kcl = nkc - 1;
+  RawB X[2], Y[2];
+  loadB(0, X[0], Y[0]);
+  for (int g = 0; g + 1 < ngrp; g++) {
+    const int buf = g & 1;
+    const int kc0 = 2 * g;
+    begin();
+    prefetch(g + 2);
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      loadB(min(kc0 + j + 1, kcl), X[(j + 1) & 1], Y[(j + 1) & 1]);
+      produce8(buf ^ 1, j);
+      step(buf, kc0 + j, X[j], Y[j]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+  }
+  {
+    // Last group: chunks 2 (ngrp-1) .. nkc-1 (one or two).
+    const int buf = (ngrp - 1) & 1;
+    const int kc0 = 2 * (ngrp - 1);
+    if (kc0 + 1 < nkc) loadB(kc0 + 1, X[1], Y[1]);
+    step(buf, kc0, X[0], Y[0]);
+    if (kc0 + 1 < nkc) step(buf, kc0 + 1, X[1], Y[1]);
+#pragma unroll
+    for (int i = 0; i < 8; i++) flush(acc[i][0], cp[i]);
+  }
+
+  const int col = tile * 16 + (lane & 15);
+  if (col >= p.ncol) return;
+  const int lf = field_lmax(p, col >> 1);
+  const long cbase = (long)field_index(p.fmap, col >> 1) * p.cstride;
+  // Row 16 lt + 4 (lane / 16) + r of class lt / 4 is degree lp + d, d =
+  // 2 i + (lt / 4 + par0) % 2 with i its index in the class.
+#pragma unroll
+  for (int lt = 0; lt < 8; lt++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = 16 * (lt & 3) + 4 * (lane >> 4) + r;
+      const int l = lp + 2 * i + (((lt >> 2) + par0) & 1);
+      if (l > lf) continue;
+      const long ci = (cbase + coef_index(l, m, p.nabs, REAL)) * 2 + (col & 1);
+      double v = acc[lt][0][r];
+      if (p.accumulate) v += p.cout[ci];
+      if (p.zero_last && lf == p.L && l == lf && m == lf) v = 0.0;
+      p.cout[ci] = v;
+    }
+}
+
+
+// ---------------------------------------------------------------------------
 // Synthesis of the last colatitudes t0 .. ntheta - 1 (t0 = ntheta -
 // tail_rows) when ntheta is only a row or two past a multiple of 256.  A
 // leg_synth_bf16 item holding one row would still walk every degree chunk,
@@ -995,13 +1489,17 @@ hipError_t launch_tail_reduce(const LegParams& p, hipStream_t s) {
 }
 
 template <int NS>
-hipError_t launch_anal_bf16_t(const LegParams& p0, hipStream_t s) {
+hipError_t launch_anal_bf16_t(const LegParams& p0, hipStream_t s, bool sym) {
   LegParams p = p0;
   p.n_cg = (p.ncol + 15) / 16;  // the layout FA
   p.nquad = (p.ntheta + 3) / 4;
   p.nkc = (p.ntheta + 31) / 32;
   const dim3 grid(8 * p.items_per);  // work list: (order, pass, column super-group)
-  if (p.real)
+  if (sym && p.real)
+    hipLaunchKernelGGL((leg_anal_bf16_s<NS, 1>), grid, dim3(512), 0, s, p);
+  else if (sym)
+    hipLaunchKernelGGL((leg_anal_bf16_s<NS, 0>), grid, dim3(512), 0, s, p);
+  else if (p.real)
     hipLaunchKernelGGL((leg_anal_bf16<NS, 1>), grid, dim3(512), 0, s, p);
   else
     hipLaunchKernelGGL((leg_anal_bf16<NS, 0>), grid, dim3(512), 0, s, p);
@@ -1009,24 +1507,30 @@ hipError_t launch_anal_bf16_t(const LegParams& p0, hipStream_t s) {
 }
 
 template <int NS>
-hipError_t launch_anal_bf16_w(const LegParams& p0, hipStream_t s) {
+hipError_t launch_anal_bf16_w(const LegParams& p0, hipStream_t s, bool sym) {
   LegParams p = p0;
   p.n_cg = (p.ncol + 15) / 16;
   p.nquad = (p.ntheta + 3) / 4;
   p.nkc = (p.ntheta + 31) / 32;
   const dim3 grid(8 * p.items_per);  // work list: (order, 128-degree pass, column super-group)
-  if (p.real)
+  if (sym && p.real)
+    hipLaunchKernelGGL((leg_anal_bf16_ws<NS, 1>), grid, dim3(512), 0, s, p);
+  else if (sym)
+    hipLaunchKernelGGL((leg_anal_bf16_ws<NS, 0>), grid, dim3(512), 0, s, p);
+  else if (p.real)
     hipLaunchKernelGGL((leg_anal_bf16_w<NS, 1>), grid, dim3(512), 0, s, p);
   else
     hipLaunchKernelGGL((leg_anal_bf16_w<NS, 0>), grid, dim3(512), 0, s, p);
   return hipGetLastError();
 }
 
-hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s, int wide) {
+hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s, int form) {
   if (!p.ck || !p.ckoff) return hipErrorInvalidValue;  // checkpoint table required
   if (!p.FA32) return hipErrorInvalidValue;             // FP32 bins required
-  if (wide) return ns == 3 ? launch_anal_bf16_w<3>(p, s) : launch_anal_bf16_w<2>(p, s);
-  return ns == 3 ? launch_anal_bf16_t<3>(p, s) : launch_anal_bf16_t<2>(p, s);
+  if (form >= 2 && p.nabs != 0) return hipErrorInvalidValue;  // symmetry of n = 0 only
+  if (form == 1 || form == 3)
+    return ns == 3 ? launch_anal_bf16_w<3>(p, s, form == 3) : launch_anal_bf16_w<2>(p, s, form == 3);
+  return ns == 3 ? launch_anal_bf16_t<3>(p, s, form == 2) : launch_anal_bf16_t<2>(p, s, form == 2);
 }
 
 }  // namespace gsh

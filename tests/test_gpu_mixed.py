@@ -109,7 +109,9 @@ def test_mixed_synthesis_last_rows(torch, prec, L, n, real, batch):
                                             (512, 2, False, 8), (300, -1, False, 2),
                                             (1024, 0, True, 16), (1024, 2, False, 16),
                                             (100, 1, True, 70), (24, 1, False, 4),
-                                            (1024, 0, True, 64), (1024, 2, False, 64)])
+                                            (1024, 0, True, 64), (1024, 2, False, 64),
+                                            (63, 0, True, 4), (255, 0, False, 3),
+                                            (512, 0, False, 20)])
 def test_mixed_analysis_error(torch, prec, L, n, real, batch):
     say(f"[mixed analysis] {prec} L={L} n={n} real={real} batch={batch}")
     g = grid(L)
@@ -125,6 +127,30 @@ def test_mixed_analysis_error(torch, prec, L, n, real, batch):
         f"round trip max {rt.max():.2e}")
     assert e.max() <= BARS[prec]
     assert rt.max() <= 2 * BARS[prec]
+
+
+@pytest.mark.parametrize("prec", ["bf16x6", "bf16x3"])
+@pytest.mark.parametrize("L,real", [(64, True), (64, False), (63, True), (1024, True)])
+def test_mixed_analysis_general_field_n0(torch, prec, L, real):
+    """Upper index 0 is analysed over the northern colatitudes against the sum
+    and difference of each row and its mirror.  A field of independent
+    random samples (not a synthesized, band-limited one) makes the northern
+    and southern rows unrelated: the split-BF16 analysis must still meet the
+    mode's bar against FP64, for odd (equator row) and even row counts."""
+    say(f"[mixed analysis, random field n=0] {prec} L={L} real={real}")
+    g = grid(L)
+    gen = torch.Generator(device="cuda").manual_seed(SEED0 + L + real)
+    shape = (5, L + 1, 2 * L)
+    if real:
+        f = torch.randn(shape, dtype=torch.float64, device="cuda", generator=gen)
+    else:
+        f = torch.complex(torch.randn(shape, dtype=torch.float64, device="cuda", generator=gen),
+                          torch.randn(shape, dtype=torch.float64, device="cuda", generator=gen))
+    a64 = g.forward(L, 0, f, real=real)
+    amx = g.forward(L, 0, f, real=real, precision=prec)
+    e = per_field_rel(torch, amx, a64)
+    say(f"    analysis max {e.max():.2e}")
+    assert e.max() <= BARS[prec]
 
 
 def test_mixed_accumulate_and_zero_last(torch):
