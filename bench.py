@@ -47,19 +47,21 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "transform_kernels_lmax1024_b64_co
 # Kernel instantiations of each work family per mode (real and complex
 # batch), as regular expressions over the summary's kernel names.
 FAMILY_KERNELS = {
-    # split-BF16 analysis: the 128-degree-pass kernel the single calls use
+    # split-BF16 Legendre kernels: the n=0 batch runs the equatorially
+    # symmetric variants (_ws, _s), the n=2 batch the general ones
     "legendre_analysis": {"fp64": r"leg_anal_f64<4, false>", "fp32": r"leg_anal_f64<4, true>",
-                          "bf16x3": r"leg_anal_bf16_w<2, \d>", "bf16x6": r"leg_anal_bf16_w<3, \d>"},
+                          "bf16x3": r"leg_anal_bf16_ws?<2, \d>", "bf16x6": r"leg_anal_bf16_ws?<3, \d>"},
     "legendre_synthesis": {"fp64": r"leg_synth_f64<2, \d, false>",
                            "fp32": r"leg_synth_f64<2, \d, true>",
-                           "bf16x3": r"leg_synth_bf16<2, \d>", "bf16x6": r"leg_synth_bf16<3, \d>"},
+                           "bf16x3": r"leg_synth_bf16(_s)?<2, \d>",
+                           "bf16x6": r"leg_synth_bf16(_s)?<3, \d>"},
     # FA in FP64 (fp64, fp32) or FP32 (split-BF16 modes): last template argument
     "fft_analysis": {"fp64": r"fft_anal_w<32, \d, 0, 0>", "fp32": r"fft_anal_w<32, \d, 0, 0>",
                      "bf16x3": r"fft_anal_w<32, \d, 0, 1>", "bf16x6": r"fft_anal_w<32, \d, 0, 1>"},
     # F in FP64 (fp64, fp32) or FP32 (split-BF16 modes): last template argument
     "fft_synthesis": {"fp64": r"fft_synth_w<32, \d, 0, 0>", "fp32": r"fft_synth_w<32, \d, 0, 0>",
                       "bf16x3": r"fft_synth_w<32, \d, 0, 1>", "bf16x6": r"fft_synth_w<32, \d, 0, 1>"},
-    "coef_split": {"bf16x3": r"coef_split<2, \d>", "bf16x6": r"coef_split<3, \d>"},
+    "coef_split": {"bf16x3": r"coef_split<2, \d, \d>", "bf16x6": r"coef_split<3, \d, \d>"},
 }
 HEADLINE = "bf16x6"            # <= 1e-7 relative L2 against FP64
 REPORTED = ["bf16x3", "fp32"]  # reported beside it (throughput and errors)
@@ -71,15 +73,21 @@ def log(*a):
 
 def algorithmic_counts():
     """Per-field algorithmic work: Legendre flops 4*ntheta*C (complex) or
-    4*ntheta*R (real); ring-FFT HBM bytes (field read/write + F bins)."""
+    4*ntheta*R (real); ring-FFT HBM bytes (field read/write + F bins).
+
+    Also the Legendre flops the kernels issue: the n=0 kernels fold each
+    colatitude with its mirror (d^l_{m0}(pi - theta) = (-1)^(l+m) d^l_{m0}(theta)
+    on the equatorially symmetric Gauss-Legendre nodes), so they multiply over
+    the ceil(ntheta/2) northern colatitudes only."""
     import gshtrans_amd as G
     nt, N = L + 1, 2 * L
     R0 = G.coefficient_size(L, 0, real=True)
     C2 = G.coefficient_size(L, 2, real=False)
     leg = {"real": 4.0 * nt * R0, "complex": 4.0 * nt * C2}
+    issued = {"real": 4.0 * ((nt + 1) // 2) * R0, "complex": 4.0 * nt * C2}
     fft_bytes = {"real": nt * N * 8 + nt * (L + 1) * 16,
                  "complex": nt * N * 16 + nt * (2 * L + 1) * 16}
-    return leg, fft_bytes
+    return leg, fft_bytes, issued
 
 
 def run_gpu(args):
@@ -214,7 +222,7 @@ def run_gpu(args):
 
 
 def roofline(results, headline):
-    leg, fftb = algorithmic_counts()
+    leg, fftb, issued = algorithmic_counts()
     kt = results[headline]["kernels"]
     # per-step work: real batch + complex batch, one launch each per kernel
     work = {
@@ -229,12 +237,16 @@ def roofline(results, headline):
     steps = launches / 2.0
     total = per_step * steps
     sec = ms / 1e3
+    issued_frac = None
     if bound == "mfma":
-        # MFMA flops actually issued for the algorithmic products: each
-        # logical product costs PRODUCTS[mode] MFMA products.
+        # MFMA flops for the algorithmic products: each logical product costs
+        # PRODUCTS[mode] MFMA products.
         achieved = total * PRODUCTS[headline] / sec / 1e12
         peak = PEAK_TFLOPS[headline]
         unit = "TFLOP/s"
+        # the same over the products the kernels issue (n=0: northern half)
+        done = BATCH * (issued["real"] + issued["complex"]) * steps
+        issued_frac = done * PRODUCTS[headline] / sec / 1e12 / peak
     else:
         achieved = total / sec / 1e9
         peak = HBM_GBPS
@@ -242,7 +254,8 @@ def roofline(results, headline):
     per_kernel = {k: {"avg_launch_ms": v[0] / max(v[1], 1), "launches": v[1]}
                   for k, v in kt.items()}
     return {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-            "frac": achieved / peak, "traffic": traffic_per_launch(name, headline),
+            "frac": achieved / peak, "issued_frac": issued_frac,
+            "traffic": traffic_per_launch(name, headline),
             "avg_launch_ms": ms / max(launches, 1), "per_kernel": per_kernel}
 
 

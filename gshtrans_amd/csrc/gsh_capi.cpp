@@ -439,8 +439,10 @@ int ensure_work2(gsh_grid_s* g, size_t bytes) {
 
 // Chunk offsets of the split coefficient layout: order m (slot m - m_first)
 // owns ceil((lmax - l0 + 1) / 32) chunks of 32 degrees, l0 = max(|m|, |n|).
-int ensure_cso(gsh_grid_s* g, int lmax, int n, int real, long** dev, long* total) {
-  const long key = ((long)lmax * 4096 + (n + 2048)) * 2 + real;
+// Split-block offsets per slot: one block per 32-degree chunk, or (sym, the
+// parity layout of upper index 0) two per 64-degree chunk.
+int ensure_cso(gsh_grid_s* g, int lmax, int n, int real, int sym, long** dev, long* total) {
+  const long key = (((long)lmax * 4096 + (n + 2048)) * 2 + real) * 2 + sym;
   auto it = g->cso.find(key);
   if (it != g->cso.end()) {
     *dev = it->second.first;
@@ -455,7 +457,7 @@ int ensure_cso(gsh_grid_s* g, int lmax, int n, int real, long** dev, long* total
     h[sl] = acc;
     const int m = sl + m_first;
     const int l0 = std::max(std::abs(m), std::abs(n));
-    acc += (lmax - l0 + 32) / 32;
+    acc += sym ? 2 * ((lmax - l0 + 64) / 64) : (lmax - l0 + 32) / 32;
   }
   h[nslot] = acc;
   long* d = nullptr;
@@ -627,8 +629,9 @@ int ensure_synth_worklist(gsh_grid_s* g, int family, int lmax, int real, int ngr
 // degree and field, not line-aligned) share cache lines, and the next chunk
 // of a group, whose window overlaps the previous one by 8 degrees, follows
 // a band later.
-int ensure_split_worklist(gsh_grid_s* g, int lmax, int n, int real, int ntile, WorkList* out) {
-  auto key = std::make_tuple(31, lmax, n, real, std::vector<int>{ntile});
+int ensure_split_worklist(gsh_grid_s* g, int lmax, int n, int real, int ntile, int sym,
+                          WorkList* out) {
+  auto key = std::make_tuple(sym ? 32 : 31, lmax, n, real, std::vector<int>{ntile});
   auto it = g->worklists.find(key);
   if (it != g->worklists.end()) {
     *out = it->second;
@@ -641,7 +644,8 @@ int ensure_split_worklist(gsh_grid_s* g, int lmax, int n, int real, int ntile, W
   for (int gs = 0; gs < ngs; gs++)
     for (int sl = 8 * gs; sl < std::min(8 * gs + 8, nslot); sl++) {
       const int l0 = std::max(std::abs(sl + m_first), std::abs(n));
-      nch[gs] = std::max(nch[gs], (lmax - l0 + 32) / 32);
+      const int cw = sym ? 64 : 32;  // degrees per chunk
+      nch[gs] = std::max(nch[gs], (lmax - l0 + cw) / cw);
     }
   constexpr int kSplitBand = 4;
   std::vector<uint2> items;
@@ -1056,14 +1060,21 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
       return gsh::launch_leg_synth_f64(p, s, T, precision == GSH_FP32);
     }));
   } else {
-    // Up to kSynthTailRows colatitudes past whole super-blocks go to the tail
-    // kernel rather than to items of their own.
+    // Upper index 0 is synthesized over the northern colatitudes (items of
+    // 128 of them; each writes the mirror rows too), the coefficients in the
+    // parity layout, and the equator of an odd ntheta as a tail row.
+    // Otherwise up to kSynthTailRows colatitudes past whole super-blocks of
+    // 256 go to the tail kernels rather than to items of their own.
     const int nt = g->ntheta();
-    const bool tail = nt > 256 && nt % 256 != 0 && nt % 256 <= gsh::kSynthTailRows;
+    const int sym = n == 0 ? 1 : 0;
+    const bool tail =
+        sym ? nt % 2 == 1 : nt > 256 && nt % 256 != 0 && nt % 256 <= gsh::kSynthTailRows;
+    const int nblk = sym ? (nt / 2 + 127) / 128 : tail ? nt / 256 : ntsb;
     WorkList w;
-    if ((st = ensure_synth_worklist(g, 20, lmax, real, ((p.ncol + 15) / 16 + 7) / 8,
-                                    tail ? nt / 256 : ntsb, &w)))
+    if ((st = ensure_synth_worklist(g, sym ? 21 : 20, lmax, real, ((p.ncol + 15) / 16 + 7) / 8,
+                                    nblk, &w)))
       return st;
+    p.sym = sym;
     p.items = w.d;
     p.items_per = w.per;
     const int ns = precision == GSH_BF16X6 ? 3 : 2;
@@ -1073,15 +1084,15 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
     p.ckoff = d->ckoff;
     long* cso = nullptr;
     long nchunks = 0;
-    if ((st = ensure_cso(g, lmax, n, real, &cso, &nchunks))) return st;
+    if ((st = ensure_cso(g, lmax, n, real, sym, &cso, &nchunks))) return st;
     // Column tiles padded to whole super-groups of 8 (zero columns), so the
     // synthesis reads every tile of its super-group without a bounds check.
     p.ntile = ((p.ncol + 15) / 16 + 7) / 8 * 8;
     p.cso = cso;
     const long nblocks = nchunks * p.ntile;
-    // The tail rows' d values (32 per chunk and row) and partial sums (16 per
-    // block and row) follow the split coefficients in the same buffer.
-    const int trows = tail ? nt % 256 : 0;
+    // The tail rows' d values (32 per block and row) and partial sums (16 per
+    // block, tile and row) follow the split coefficients in the same buffer.
+    const int trows = !tail ? 0 : sym ? 1 : nt % 256;
     const size_t bs_bytes = (size_t)nblocks * ns * 64 * 16;
     const size_t dv_bytes = (size_t)trows * nchunks * 32 * sizeof(double);
     const size_t pt_bytes = (size_t)trows * nblocks * 16 * sizeof(double);
@@ -1091,6 +1102,7 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
     if (trows) {
       char* w2 = static_cast<char*>(g->work2);
       p.tail_rows = trows;
+      p.tail_t0 = sym ? nt / 2 : nt - trows;
       p.tail_dv = reinterpret_cast<const double*>(w2 + bs_bytes);
       p.tail_dstride = nchunks * 32;
       p.tail_part = reinterpret_cast<double*>(w2 + bs_bytes + dv_bytes);
@@ -1098,7 +1110,7 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
     }
     {
       WorkList ws;
-      if ((st = ensure_split_worklist(g, lmax, n, real, p.ntile, &ws))) return st;
+      if ((st = ensure_split_worklist(g, lmax, n, real, p.ntile, sym, &ws))) return st;
       p.items = ws.d;
       p.items_per = ws.per;
     }

@@ -109,15 +109,22 @@ struct LegParams {
   // every l0 + kCk k <= L.
   const double2* ck;
   const long* ckoff;
-  // Split-BF16 synthesis of the last tail_rows colatitudes (0: none), from
-  // the d values tail_dv[r * tail_dstride + 32 (cso[slot] + chunk) + j] of
-  // degree l0 + 32 chunk + j and the per-block partial sums
+  // Split-BF16 synthesis of the tail_rows colatitudes tail_t0 .. (0: none),
+  // from the d values tail_dv[r * tail_dstride + 32 block + j] of the degree
+  // of entry j of each split block and the per-block partial sums
   // tail_part[r * tail_pstride + 16 block + column] that coef_split adds.
   const double* tail_dv;
   double* tail_part;
   long tail_dstride;
   long tail_pstride;
   int tail_rows;
+  int tail_t0;
+  // Split-BF16 synthesis of upper index 0 over the northern colatitudes
+  // (leg_synth_bf16_s): the split blocks go by parity class, two per
+  // 64-degree chunk c of an order, block cso[slot] + 2 c + k holding the
+  // degrees l0 + 64 c + 2 j + ((k + l0 + m) % 2), j = 0..31, of class k
+  // (k = (l + m) % 2).  0: blocks of 32 consecutive degrees.
+  int sym;
 };
 
 
@@ -273,9 +280,9 @@ hipError_t launch_fft_anal_wave(const FFTParams& p, hipStream_t s);
 // Mixed precision kernels (gsh_mixed.hip); ns = 2 (bf16x3) or 3 (bf16x6).
 hipError_t launch_coef_split(const LegParams& p, int ns, hipStream_t s);
 hipError_t launch_leg_synth_bf16(const LegParams& p, int ns, hipStream_t s);
-// Split-BF16 synthesis of the last p.tail_rows colatitudes (at most
-// kSynthTailRows): their d values before coef_split, which then adds the
-// partial sums, and the sums over chunks into F after it.
+// Split-BF16 synthesis of the p.tail_rows colatitudes from p.tail_t0 (at
+// most kSynthTailRows): their d values before coef_split, which then adds the
+// partial sums, and the sums over the blocks into F after it.
 hipError_t launch_tail_dvals(const LegParams& p, hipStream_t s);
 hipError_t launch_tail_reduce(const LegParams& p, hipStream_t s);
 // wide: 128-degree passes (leg_anal_bf16_w; work list of family 3) instead of 64.

@@ -132,11 +132,15 @@ __device__ __forceinline__ void flush(d4& acc, const f4 c) {
 // (degree, field) the 8 orders are 128 contiguous bytes of the coefficient
 // array -- and each wave then takes its order's 32 degrees from LDS.
 // ---------------------------------------------------------------------------
-constexpr int kSplitWin = 40;          // degree window of a workgroup
 constexpr int kSplitRow = 8 * 16 + 2;  // doubles per window degree (8 orders x 16 columns + pad)
 
-template <int NS, int REAL>
+// SYM: the parity layout of p.sym (64-degree chunks, two blocks each, entry j
+// of class k's block being degree l0 + 64 chunk + 2 j + ((k + l0 + m) % 2)).
+template <int NS, int REAL, int SYM>
 __global__ __launch_bounds__(512) void coef_split(LegParams p) {
+  constexpr int CW = SYM ? 64 : 32;     // degrees per chunk
+  constexpr int kSplitWin = CW + 8;     // degree window of a workgroup
+  constexpr int ST = SYM ? 2 : 1;       // degrees between consecutive entries
   __shared__ double win[kSplitWin * kSplitRow];
   const uint2 it = work_item(p.items, p.items_per);
   if (it.x == ~0u) return;
@@ -151,7 +155,7 @@ __global__ __launch_bounds__(512) void coef_split(LegParams p) {
 #pragma unroll
   for (int w = 0; w < 8; w++) {
     const int sl = s0 + w;
-    if (sl < p.nslot) lw0 = min(lw0, max(abs(sl + p.m_first), p.nabs) + 32 * chunk);
+    if (sl < p.nslot) lw0 = min(lw0, max(abs(sl + p.m_first), p.nabs) + CW * chunk);
   }
   // Window: piece (l_off, f, mo) = (tid + 512 r) / 64, (tid / 8) % 8, tid % 8.
   {
@@ -179,41 +183,45 @@ __global__ __launch_bounds__(512) void coef_split(LegParams p) {
   if (sl >= p.nslot) return;
   const int m = sl + p.m_first;
   const int l0 = max(abs(m), p.nabs);
-  if (chunk >= (p.lmax - l0 + 32) / 32) return;  // no such block for this order
-  const long bc = ld_uniform(p.cso, sl) + chunk;  // chunk in the split layout
-  const long b = bc * p.ntile + tile;              // output block
+  if (chunk >= (p.lmax - l0 + CW) / CW) return;  // no such block for this order
   const int kr = lane >> 4, c = lane & 15;
-  const int lo = l0 + 32 * chunk + 8 * kr - lw0;  // window offset of e = 0
-  unsigned planes[NS][4];
 #pragma unroll
-  for (int e2 = 0; e2 < 4; e2++) {
-    const double x0 = win[(lo + 2 * e2) * kSplitRow + 16 * wave + c];
-    const double x1 = win[(lo + 2 * e2 + 1) * kSplitRow + 16 * wave + c];
-    unsigned o[NS];
-    split_pair<NS>(x0, x1, o);
-#pragma unroll
-    for (int s = 0; s < NS; s++) planes[s][e2] = o[s];
-  }
-#pragma unroll
-  for (int s = 0; s < NS; s++)
-    p.bs_out[(b * NS + s) * 64 + lane] =
-        make_uint4(planes[s][0], planes[s][1], planes[s][2], planes[s][3]);
-  // Tail colatitudes: this block's share of their sums, from the FP64
-  // coefficients, one partial per (row, block, column); tail_reduce adds the
-  // chunks up.
-  for (int r = 0; r < p.tail_rows; r++) {
-    const double2* dq =
-        reinterpret_cast<const double2*>(p.tail_dv + r * p.tail_dstride + bc * 32 + 8 * kr);
-    double acc = 0.0;
+  for (int k = 0; k < (SYM ? 2 : 1); k++) {
+    const long bc = ld_uniform(p.cso, sl) + (SYM ? 2 * chunk + k : chunk);  // split block
+    const long b = bc * p.ntile + tile;                                    // output block
+    // Window offset of entry 8 kr (e = 0); entry 8 kr + e is ST e further.
+    const int lo = l0 + CW * chunk + (SYM ? (k + l0 + m) & 1 : 0) + 8 * ST * kr - lw0;
+    unsigned planes[NS][4];
 #pragma unroll
     for (int e2 = 0; e2 < 4; e2++) {
-      const double2 d = dq[e2];
-      acc = fma(d.x, win[(lo + 2 * e2) * kSplitRow + 16 * wave + c], acc);
-      acc = fma(d.y, win[(lo + 2 * e2 + 1) * kSplitRow + 16 * wave + c], acc);
+      const double x0 = win[(lo + ST * 2 * e2) * kSplitRow + 16 * wave + c];
+      const double x1 = win[(lo + ST * (2 * e2 + 1)) * kSplitRow + 16 * wave + c];
+      unsigned o[NS];
+      split_pair<NS>(x0, x1, o);
+#pragma unroll
+      for (int s = 0; s < NS; s++) planes[s][e2] = o[s];
     }
-    acc += __shfl_xor(acc, 16);
-    acc += __shfl_xor(acc, 32);
-    if (lane < 16) p.tail_part[r * p.tail_pstride + b * 16 + c] = acc;
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+      p.bs_out[(b * NS + s) * 64 + lane] =
+          make_uint4(planes[s][0], planes[s][1], planes[s][2], planes[s][3]);
+    // Tail colatitudes: this block's share of their sums, from the FP64
+    // coefficients, one partial per (row, block, column); tail_reduce adds
+    // the blocks up.
+    for (int r = 0; r < p.tail_rows; r++) {
+      const double2* dq =
+          reinterpret_cast<const double2*>(p.tail_dv + r * p.tail_dstride + bc * 32 + 8 * kr);
+      double acc = 0.0;
+#pragma unroll
+      for (int e2 = 0; e2 < 4; e2++) {
+        const double2 d = dq[e2];
+        acc = fma(d.x, win[(lo + ST * 2 * e2) * kSplitRow + 16 * wave + c], acc);
+        acc = fma(d.y, win[(lo + ST * (2 * e2 + 1)) * kSplitRow + 16 * wave + c], acc);
+      }
+      acc += __shfl_xor(acc, 16);
+      acc += __shfl_xor(acc, 32);
+      if (lane < 16) p.tail_part[r * p.tail_pstride + b * 16 + c] = acc;
+    }
   }
 }
 
@@ -442,6 +450,265 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
         for (int r = 0; r < 4; r++)
           if (row0 + g * 16 + r < nt) fp[(g * 16 + r) * rs] = acc[g][t][r];
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Synthesis of upper index 0 over the northern colatitudes.  Gauss-Legendre
+// nodes are symmetric about the equator and d^l_{m0}(pi - theta) =
+// (-1)^(l+m) d^l_{m0}(theta), so with E and O the sums over the degrees of
+// even and of odd l + m at a northern colatitude t, F(t) = E + O and
+// F(nt-1-t) = E - O: half the recurrence values and MFMAs of leg_synth_bf16.
+// The coefficients come in the parity layout (coef_split with sym: two
+// 32-degree blocks per 64-degree chunk, one per class).  Workgroup = 8 waves
+// for (128 northern colatitudes of block tsb) x (128 columns of super-group
+// csg) of one order m, walking the blocks (K steps of one class each) with
+// the B operands double-buffered per block and the bf16 planes of D
+// double-buffered per 64-degree chunk.  While the waves multiply chunk c's
+// two blocks they produce chunk c + 1: wave w runs the recurrence of
+// colatitude block w % 2 (64 colatitudes) for the 16 degrees of quarter
+// w / 2 from the checkpoint there, one recurrence pair per row group, and
+// packs the values of each class into that class's planes (consecutive
+// entries of a class are two degrees apart).  Wave w multiplies colatitude
+// half h = w / 4 (4 row groups) with column tiles 2 (w % 4), 2 (w % 4) + 1,
+// keeping E and O apart.  The equator (odd ntheta) is a tail row.
+// ---------------------------------------------------------------------------
+template <int NS, int REAL>
+__global__ __launch_bounds__(512) void leg_synth_bf16_s(LegParams p) {
+  static_assert(kCk == 16, "four 16-degree quarters per 64-degree chunk");
+  // Planes of a chunk: class k, entries 8 kr .. 8 kr + 7, 128 colatitudes.
+  __shared__ uint4 trb[2][NS][2][4][128];
+  // B operands of a block (8 column tiles x NS planes x 64 lanes).
+  __shared__ uint4 bsh[2][8][NS][64];
+  // Recurrence coefficients of a chunk (64 entries, 2 KiB).
+  __shared__ uint4 rsc[2][128];
+  const int lane = threadIdx.x & 63;
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  // Work item: (order, column super-group, northern block of 128).
+  const uint2 it = work_item(p.items, p.items_per);
+  if (it.x == ~0u) return;
+  const int mw = it.x;
+  const int tsb = it.y & 0xffff;
+  const int csg = it.y >> 16;
+  const int m = order_of(mw, REAL);
+  const int slot = m - p.m_first;
+  const int l0 = max(abs(m), p.nabs);
+  const int nt = p.ntheta;
+  const int nn = nt >> 1;                    // northern colatitudes (the equator apart)
+  const int nC = (p.lmax - l0 + 64) >> 6;    // 64-degree chunks
+  const int nb = 2 * nC;                     // blocks: block 2 c + k is chunk c's class k
+  const int rows = min(128, nn - tsb * 128);  // valid colatitudes of the item
+  const int par0 = (l0 + m) & 1;             // class of the even offsets from l0
+
+  // Producer share: colatitude block pb, degree quarter pq of a chunk.
+  const int pb = wave & 1, pq = wave >> 1;
+  const int tpc = min(tsb * 128 + 64 * pb + lane, nn - 1);  // past the north: the last one
+  const double pcs = p.cosv[tpc];
+  const int nck = (p.L - l0 + kCk) / kCk;  // checkpoints 1 .. nck-1 exist
+  // State at checkpoint k >= 1; past L it clamps to the last checkpoint.  An
+  // order without checkpoints reads entry 0 of the table and never uses it.
+  const double2* ckm =
+      (nck > 1 ? p.ck + ld_uniform(p.ckoff, m + p.L) * (long)nt : p.ck) + tpc;
+  auto ck_state = [&](int k) { return ckm[(long)(nck > 1 ? min(k, nck - 1) - 1 : 0) * nt]; };
+  double pe0, pe1;
+  {
+    // Chunk 0: the seed for quarter 0, checkpoint pq for the others.
+    const double seed = p.seeds[(long)(m + p.L) * nt + tpc];
+    const double2 c1 = ck_state(max(pq, 1));
+    pe0 = pq == 0 || nck < 2 ? seed : c1.x;
+    pe1 = pq == 0 || nck < 2 ? 0.0 : c1.y;
+  }
+
+  const uint4* recq = reinterpret_cast<const uint4*>(p.rec + (ld_uniform(p.roff, m + p.L) - l0));
+  // Entries up to L + kRecPad exist: chunk c's entries l0 + 64 c + 1 .. + 64
+  // for every c < nC.
+  auto fetch_rec = [&](int c) { return recq[2 * (l0 + 64 * min(c, nC - 1) + 1) + (tid & 127)]; };
+
+  // B operand staging: piece j of this thread is uint4 tid + 512 j of the
+  // block's 8 contiguous tiles (ntile is a multiple of 8).
+  const uint4* bbase = p.bs + (ld_uniform(p.cso, slot) * (long)p.ntile + csg * 8) * NS * 64;
+  const int bstride = p.ntile * NS * 64;  // uint4 per block
+  struct Pieces {
+    uint4 v[NS];
+  };
+  auto fetch = [&](int b) {
+    const uint4* cb = bbase + (long)min(b, nb - 1) * bstride;
+    Pieces q;
+#pragma unroll
+    for (int j = 0; j < NS; j++) q.v[j] = cb[tid + 512 * j];
+    return q;
+  };
+  auto store = [&](int buf, const Pieces q) {
+#pragma unroll
+    for (int j = 0; j < NS; j++) (&bsh[buf][0][0][0])[512 * j + tid] = q.v[j];
+  };
+
+  // Consumer role.
+  const int h = wave >> 2;
+  const int tile0 = csg * 8 + (wave & 3) * 2;
+  d4 acc[4][2][2];  // row group, column tile, class
+#pragma unroll
+  for (int g = 0; g < 4; g++)
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+      for (int k = 0; k < 2; k++) acc[g][t][k] = d4{0.0, 0.0, 0.0, 0.0};
+
+  // Production: pair i (0..7) of this wave's quarter is steps 2 i, 2 i + 1,
+  // degrees d = 16 pq + 2 i (+1) of the chunk.  An even pair keeps its two
+  // values; the odd pair after it completes dword i / 2 of the even offsets
+  // (d, d + 2: class par0) and of the odd offsets (d + 1, d + 3).  The last
+  // pair stores both uint4 (entries 8 pq .. + 7 of each class) whole.
+  unsigned pl[NS][2][4];  // [plane][even / odd offsets][dword]
+  double hold0 = 0.0, hold1 = 0.0;
+  auto produce_pair = [&](int rbuf, int tbuf, int i) {
+    const d4* rl = reinterpret_cast<const d4*>(&rsc[rbuf][0]) + 16 * pq;
+    const double v0 = pe0;
+    rec_step(rl[2 * i], pcs, pe0, pe1);
+    const double v1 = pe0;
+    rec_step(rl[2 * i + 1], pcs, pe0, pe1);
+    if ((i & 1) == 0) {
+      hold0 = v0;
+      hold1 = v1;
+    } else {
+      unsigned o[NS];
+      split_pair<NS>(hold0, v0, o);
+#pragma unroll
+      for (int s = 0; s < NS; s++) pl[s][0][i >> 1] = o[s];
+      split_pair<NS>(hold1, v1, o);
+#pragma unroll
+      for (int s = 0; s < NS; s++) pl[s][1][i >> 1] = o[s];
+      if (i == 7) {
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+#pragma unroll
+          for (int e = 0; e < 2; e++)
+            trb[tbuf][s][e ^ par0][pq][64 * pb + lane] =
+                make_uint4(pl[s][e][0], pl[s][e][1], pl[s][e][2], pl[s][e][3]);
+      }
+    }
+  };
+  // A operands of row group g, class k (one uint4 per plane); the group's
+  // MFMAs (two independent chains) with their flush to FP64.
+  auto loadA = [&](int abuf, int k, int g, uint4 (&A)[1][NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) A[0][s] = trb[abuf][s][k][lane >> 4][h * 64 + g * 16 + (lane & 15)];
+  };
+  auto consume_group = [&](int g, int k, const uint4 (&A)[1][NS], const uint4 (&B)[2][NS]) {
+    if (h * 64 + g * 16 < rows) {
+      f4 c[1][2];
+      split_mma<NS, 1>(A, B, c);
+      flush(acc[g][0][k], c[0][0]);
+      flush(acc[g][1][k], c[0][1]);
+    }
+  };
+  auto loadB = [&](int buf, uint4 (&B)[2][NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      B[0][s] = bsh[buf][(wave & 3) * 2][s][lane];
+      B[1][s] = bsh[buf][(wave & 3) * 2 + 1][s][lane];
+    }
+  };
+
+  // Prologue: coefficients of chunks 0 and 1, B operands of block 0,
+  // production of chunk 0.
+  if (tid < 128) {
+    rsc[0][tid] = fetch_rec(0);
+    rsc[1][tid] = fetch_rec(1);
+  }
+  store(0, fetch(0));
+  __syncthreads();
+  const bool pv = 64 * pb < rows;  // this wave's producer block holds colatitudes
+  if (pv) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) produce_pair(0, 0, i);
+  }
+  __syncthreads();
+
+  // Blocks 0 .. nb-2 with the production of the next chunk during a chunk's
+  // two blocks.  The state of chunk c + 1 (checkpoint 4 (c + 1) + pq) is
+  // loaded one chunk before its use.
+  // Block 2 c + k of chunk c (k a constant at each call): B operands of the
+  // next block fetched, production pairs 4 k .. 4 k + 3 of chunk c + 1.
+  double2 nx = ck_state(4 + pq);
+  auto block = [&](int c, const int k) {
+    const int b = 2 * c + k;
+    const int abuf = c & 1;
+    const bool prod = pv && c + 1 < nC;
+    const Pieces q = fetch(b + 1);
+    uint4 qr = make_uint4(0u, 0u, 0u, 0u);
+    if (k == 0) {
+      pe0 = nx.x;
+      pe1 = nx.y;
+      nx = ck_state(4 * (c + 2) + pq);
+    } else if (tid < 128) {
+      qr = fetch_rec(c + 2);
+    }
+    uint4 B[2][NS];
+    loadB(b & 1, B);
+    uint4 Ab[2][1][NS];
+    loadA(abuf, k, 0, Ab[0]);
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      if (g < 3) loadA(abuf, k, g + 1, Ab[(g + 1) & 1]);
+      if (prod) produce_pair(abuf ^ 1, abuf ^ 1, 4 * k + g);
+      consume_group(g, k, Ab[g & 1], B);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    store((b + 1) & 1, q);  // buffers last read in block b - 1
+    // Chunk c + 2's coefficients (rsc[c & 1] was last read in chunk c - 1).
+    if (k == 1 && tid < 128) rsc[abuf][tid] = qr;
+    __syncthreads();
+  };
+  for (int c = 0; c + 1 < nC; c++) {
+    block(c, 0);
+    block(c, 1);
+  }
+  block(nC - 1, 0);
+  {
+    // The last block (class 1 of chunk nC - 1).
+    const int b = nb - 1;
+    const int abuf = (nC - 1) & 1;
+    uint4 B[2][NS];
+    loadB(b & 1, B);
+    uint4 Ab[2][1][NS];
+    loadA(abuf, 1, 0, Ab[0]);
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      if (g < 3) loadA(abuf, 1, g + 1, Ab[(g + 1) & 1]);
+      consume_group(g, 1, Ab[g & 1], B);
+    }
+  }
+
+  // F(t) = E + O at the northern t, F(nt-1-t) = E - O at its mirror (C
+  // layout of the 16x16x32 MFMA: lane holds rows 4 (lane / 16) + r, column
+  // lane % 16).
+  const int ncg16 = (p.ncol + 15) >> 4;
+  const long rs = (long)ncg16 * p.nslot * 16;
+  const int row0 = tsb * 128 + h * 64 + 4 * (lane >> 4);
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    const int col = (tile0 + t) * 16 + (lane & 15);
+    if (col >= p.ncol) continue;
+    const long fi = f_index(row0, slot, col, p.nslot, ncg16);
+    const long fm = f_index(nt - 1 - row0, slot, col, p.nslot, ncg16);
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = row0 + g * 16 + r;
+        if (row >= nn) continue;
+        const double e = acc[g][t][0][r], o = acc[g][t][1][r];
+        const long dn = (long)(g * 16 + r) * rs;  // rows past row0 (north) / before nt-1-row0 (south)
+        if (p.f32out) {
+          reinterpret_cast<float*>(p.F)[fi + dn] = (float)(e + o);
+          reinterpret_cast<float*>(p.F)[fm - dn] = (float)(e - o);
+        } else {
+          p.F[fi + dn] = e + o;
+          p.F[fm - dn] = e - o;
+        }
+      }
   }
 }
 
@@ -1393,16 +1660,22 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_ws(LegParams p) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void tail_dvals(LegParams p, double* dv) {
   const int slot = blockIdx.x;
-  const int t = p.ntheta - p.tail_rows + blockIdx.y;
+  const int t = p.tail_t0 + blockIdx.y;
   const int m = slot + p.m_first;
   const int l0 = max(abs(m), p.nabs);
-  const int nchunk = (p.lmax - l0 + 32) >> 5;
+  const int cw = p.sym ? 64 : 32;  // degrees per chunk of the split layout
+  const int nchunk = (p.lmax - l0 + cw) / cw;
   const int nck = (p.L - l0 + kCk) / kCk;  // checkpoints 1 .. nck-1 exist
   const double cs = p.cosv[t];
   const d4* rl = reinterpret_cast<const d4*>(p.rec + (ld_uniform(p.roff, m + p.L) - l0));
   const double2* ckm = p.ck + ld_uniform(p.ckoff, m + p.L) * (long)p.ntheta + t;
   double* out = dv + blockIdx.y * p.tail_dstride + ld_uniform(p.cso, slot) * 32;
-  for (int sg = threadIdx.x; sg < 2 * nchunk; sg += 64) {
+  // Entry of degree l0 + d: d itself, or in the parity layout block
+  // 2 (d / 64) + k (k = (d + l0 + m) % 2), entry (d % 64) / 2.
+  auto entry = [&](int d) {
+    return p.sym ? 32 * (2 * (d >> 6) + ((d + l0 + m) & 1)) + ((d & 63) >> 1) : d;
+  };
+  for (int sg = threadIdx.x; sg < nchunk * cw / kCk; sg += 64) {
     double e0 = 0.0, e1 = 0.0;
     if (sg == 0) {
       e0 = p.seeds[(long)(m + p.L) * p.ntheta + t];
@@ -1414,7 +1687,7 @@ __global__ __launch_bounds__(64) void tail_dvals(LegParams p, double* dv) {
     const int ls = l0 + kCk * sg;  // entries up to ls + 15 <= lmax + 31 <= L + kRecPad
 #pragma unroll
     for (int j = 0; j < kCk; j++) {
-      out[kCk * sg + j] = e0;
+      out[entry(kCk * sg + j)] = e0;
       if (j + 1 < kCk) rec_step(rl[ls + j + 1], cs, e0, e1);
     }
   }
@@ -1424,11 +1697,13 @@ __global__ __launch_bounds__(64) void tail_dvals(LegParams p, double* dv) {
 // thread, summed over the chunks in order.
 __global__ __launch_bounds__(256) void tail_reduce(LegParams p) {
   const int slot = blockIdx.x;
-  const int t = p.ntheta - p.tail_rows + blockIdx.y;
+  const int t = p.tail_t0 + blockIdx.y;
   const int m = slot + p.m_first;
   const int l0 = max(abs(m), p.nabs);
-  const int nchunk = (p.lmax - l0 + 32) >> 5;
-  const long cstride = (long)p.ntile * 16;  // doubles per chunk
+  // Split blocks of the order: one per 32-degree chunk, or two per 64-degree
+  // chunk in the parity layout.
+  const int nchunk = p.sym ? 2 * ((p.lmax - l0 + 64) >> 6) : (p.lmax - l0 + 32) >> 5;
+  const long cstride = (long)p.ntile * 16;  // doubles per block
   const double* pp = p.tail_part + blockIdx.y * p.tail_pstride + ld_uniform(p.cso, slot) * cstride;
   const int ncg16 = (p.ncol + 15) >> 4;
   for (int col = threadIdx.x; col < p.ncol; col += 256) {
@@ -1444,17 +1719,22 @@ __global__ __launch_bounds__(256) void tail_reduce(LegParams p) {
 
 }  // namespace
 
-hipError_t launch_coef_split(const LegParams& p, int ns, hipStream_t s) {
+template <int SYM>
+hipError_t launch_coef_split_t(const LegParams& p, int ns, hipStream_t s) {
   const dim3 grid(8 * p.items_per);  // one item (8 orders) per workgroup
   if (ns == 3 && p.real)
-    hipLaunchKernelGGL((coef_split<3, 1>), grid, dim3(512), 0, s, p);
+    hipLaunchKernelGGL((coef_split<3, 1, SYM>), grid, dim3(512), 0, s, p);
   else if (ns == 3)
-    hipLaunchKernelGGL((coef_split<3, 0>), grid, dim3(512), 0, s, p);
+    hipLaunchKernelGGL((coef_split<3, 0, SYM>), grid, dim3(512), 0, s, p);
   else if (p.real)
-    hipLaunchKernelGGL((coef_split<2, 1>), grid, dim3(512), 0, s, p);
+    hipLaunchKernelGGL((coef_split<2, 1, SYM>), grid, dim3(512), 0, s, p);
   else
-    hipLaunchKernelGGL((coef_split<2, 0>), grid, dim3(512), 0, s, p);
+    hipLaunchKernelGGL((coef_split<2, 0, SYM>), grid, dim3(512), 0, s, p);
   return hipGetLastError();
+}
+
+hipError_t launch_coef_split(const LegParams& p, int ns, hipStream_t s) {
+  return p.sym ? launch_coef_split_t<1>(p, ns, s) : launch_coef_split_t<0>(p, ns, s);
 }
 
 hipError_t launch_leg_synth_bf16(const LegParams& p0, int ns, hipStream_t s) {
@@ -1463,6 +1743,18 @@ hipError_t launch_leg_synth_bf16(const LegParams& p0, int ns, hipStream_t s) {
   p.n_tsb = (p.ntheta + 255) / 256;
   p.n_cg = (p.ntile + 7) / 8;  // column super-groups of 8 tiles
   const dim3 grid(8 * p.items_per);  // work list: (order, column super-group, theta block)
+  if (p.sym) {
+    if (p.nabs != 0) return hipErrorInvalidValue;  // symmetry of n = 0 only
+    if (ns == 3 && p.real)
+      hipLaunchKernelGGL((leg_synth_bf16_s<3, 1>), grid, dim3(512), 0, s, p);
+    else if (ns == 3)
+      hipLaunchKernelGGL((leg_synth_bf16_s<3, 0>), grid, dim3(512), 0, s, p);
+    else if (p.real)
+      hipLaunchKernelGGL((leg_synth_bf16_s<2, 1>), grid, dim3(512), 0, s, p);
+    else
+      hipLaunchKernelGGL((leg_synth_bf16_s<2, 0>), grid, dim3(512), 0, s, p);
+    return hipGetLastError();
+  }
   if (ns == 3 && p.real)
     hipLaunchKernelGGL((leg_synth_bf16<3, 1>), grid, dim3(512), 0, s, p);
   else if (ns == 3)

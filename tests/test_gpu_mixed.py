@@ -55,7 +55,9 @@ def per_field_rel(torch, a, b):
                                             (512, 2, False, 8), (300, -1, False, 2),
                                             (1024, 0, True, 16), (1024, 2, False, 16),
                                             (100, 1, True, 70), (1024, 0, True, 64),
-                                            (1024, 2, False, 64)])
+                                            (1024, 2, False, 64), (63, 0, True, 4),
+                                            (255, 0, False, 3), (512, 0, False, 20),
+                                            (40, 0, True, 150)])
 def test_mixed_synthesis_error(torch, prec, L, n, real, batch):
     say(f"[mixed synthesis] {prec} L={L} n={n} real={real} batch={batch}")
     g = grid(L)
@@ -77,12 +79,14 @@ def test_mixed_synthesis_last_rows(torch, prec, L, n, real, batch):
     513, 1025), the split-BF16 synthesis takes those last colatitudes out of
     its MFMA items: the coefficient split adds their partial sums per block
     from the FP64 coefficients and the FP64 recurrence values, and a last
-    kernel sums the blocks.  The whole field meets the mode's bar, and those
-    rows equal the FP64 transform to 1e-12 in either mode when the sums reach
-    the ring FFT in FP64 (the LDS ring FFT, here L = 257), and to one FP32
-    rounding of the bins (2^-23) when they reach the wavefront ring FFT in
-    FP32 (L = 256, 512, 1024); a wrong degree, order, column or block shows
-    far above either."""
+    kernel sums the blocks.  Upper index 0 is synthesized over the northern
+    colatitudes (their mirrors from the same sums) and its tail row is the
+    equator.  The whole field meets the mode's bar, and those rows equal the
+    FP64 transform to 1e-12 in either mode when the sums reach the ring FFT
+    in FP64 (the LDS ring FFT, here L = 257), and to one FP32 rounding of the
+    bins (2^-23) when they reach the wavefront ring FFT in FP32 (L = 256,
+    512, 1024); a wrong degree, order, column or block shows far above
+    either."""
     say(f"[mixed synthesis, last rows] {prec} L={L} n={n} real={real} batch={batch}")
     g = grid(L)
     c = torch.from_numpy(G.random_coefficients(L, n, real=real, seed=SEED0 + 7 * L + n,
@@ -95,13 +99,17 @@ def test_mixed_synthesis_last_rows(torch, prec, L, n, real, batch):
     nt = L + 1
     assert 1 <= nt % 256 <= 2
     row_bar = 2.0 ** -23 if 2 * L in (512, 1024, 2048) else 1e-12
-    for t in range(nt // 256 * 256, nt):
+    # Upper index 0 is synthesized over the northern colatitudes, and its
+    # tail row is the equator of an odd ntheta; otherwise the last rows.
+    tail = ([nt // 2] if nt % 2 else []) if n == 0 else list(range(nt // 256 * 256, nt))
+    for t in tail:
         e = per_field_rel(torch, fmx[:, t:t + 1], f64[:, t:t + 1])
         say(f"    row {t}: max {e.max():.2e} (bar {row_bar:.1e})")
         assert e.max() <= row_bar
     # The row before them comes from the MFMA items.
-    t = nt // 256 * 256 - 1
-    assert per_field_rel(torch, fmx[:, t:t + 1], f64[:, t:t + 1]).max() > 1e-12
+    if tail:
+        t = tail[0] - 1
+        assert per_field_rel(torch, fmx[:, t:t + 1], f64[:, t:t + 1]).max() > 1e-12
 
 
 @pytest.mark.parametrize("prec", ["bf16x6", "bf16x3", "fp32"])
