@@ -535,7 +535,8 @@ int order_of_host(int mw, int real) { return real ? mw : ((mw & 1) ? -((mw + 1) 
 // order, group, pass (the passes of one order and group share their input
 // blocks), padded to a multiple of 8 with (~0u, 0); glmax[g] bounds the
 // degrees of group g (empty: lmax for every group).  Passes are kPass
-// degrees, or kWideAnalPass for family 3 (the 128-degree split-BF16 kernel).
+// degrees, or kWideAnalPass for family 3 (the 128-degree split-BF16 kernel);
+// family 5's items are the orders m >= 0 (mw = m) of the paired kernel.
 int ensure_worklist(gsh_grid_s* g, int family, int lmax, int n, int real, int ngroups,
                     const std::vector<int>& glmax, WorkList* out) {
   auto key = std::make_tuple(family, lmax, n, real,
@@ -545,10 +546,11 @@ int ensure_worklist(gsh_grid_s* g, int family, int lmax, int n, int real, int ng
     *out = it->second;
     return GSH_OK;
   }
-  const int nslot = real ? lmax + 1 : 2 * lmax + 1;
+  // Family 5 (orders m and -m in one item) lists the orders m >= 0 in turn.
+  const int nslot = family == 5 ? lmax + 1 : real ? lmax + 1 : 2 * lmax + 1;
   std::vector<uint2> items;
   for (int mw = 0; mw < nslot; mw++) {
-    const int m = order_of_host(mw, real);
+    const int m = family == 5 ? mw : order_of_host(mw, real);
     const int l0 = std::max(std::abs(m), std::abs(n));
     for (int grp = 0; grp < ngroups; grp++) {
       const int lw = glmax.empty() ? lmax : glmax[grp];
@@ -1167,13 +1169,17 @@ int run_forward(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
   p.cg_lmax = o.cg_lmax;
   // FA (FP32 in the split-BF16 modes), plus 8 quads of 64 entries: the
   // split-BF16 analysis reads whole 32-colatitude chunks, up to 8 quads past
-  // the last block.
+  // the last block.  In the split-BF16 modes FA starts gsh::kFAFrontPad
+  // floats into the buffer: the paired-order kernel's mirrored reads of the
+  // last chunk reach up to 31 rows of 16 floats before a block.
   const bool mixed = precision == GSH_BF16X6 || precision == GSH_BF16X3;
-  const size_t fa = (size_t)nslot * ((ncol + 15) / 16) * ((g->ntheta() + 3) / 4) * 64 + 8 * 64;
+  const size_t fa = (size_t)nslot * ((ncol + 15) / 16) * ((g->ntheta() + 3) / 4) * 64 + 8 * 64 +
+                    (mixed ? gsh::kFAFrontPad : 0);
   if ((st = ensure_work(g, (mixed ? sizeof(float) : sizeof(double)) * fa))) return st;
-  q.F = p.F = g->work;
+  q.F = p.F = mixed ? reinterpret_cast<double*>(reinterpret_cast<float*>(g->work) + gsh::kFAFrontPad)
+                    : g->work;
   q.fa32 = mixed ? 1 : 0;
-  p.FA32 = mixed ? reinterpret_cast<const float*>(g->work) : nullptr;
+  p.FA32 = mixed ? reinterpret_cast<const float*>(q.F) : nullptr;
   if ((st = ensure_ck(g, d, n, s))) return st;
   p.ck = d->ck;
   p.ckoff = d->ckoff;
@@ -1191,12 +1197,21 @@ int run_forward(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
     }));
     return GSH_OK;
   }
-  // 128-degree passes unless asked for 64 (o.anal_pass; 0 takes the
-  // default).  Upper index 0 runs over the northern colatitudes only (the
+  // Upper index 0 runs over the northern colatitudes only (the
   // Gauss-Legendre nodes are symmetric about the equator and d^l_{m0} has
-  // parity (-1)^(l+m) under theta -> pi - theta).
+  // parity (-1)^(l+m) under theta -> pi - theta).  Complex fields of other
+  // upper indices run orders m and -m together in 64-degree passes
+  // (d^l_{-m,n}(theta) = (-1)^(l+n) d^l_{m,n}(pi - theta)) by default in
+  // bf16x6, whose six products per value outweigh the paired kernel's extra
+  // B-operand work; everything else defaults to 128-degree passes.
+  // o.anal_pass (64 or 128, 0 for the default) overrides.
   const bool sym = n == 0;
-  const bool wide = (o.anal_pass ? o.anal_pass : kDefaultAnalPass) == gsh::kWideAnalPass;
+  const bool pm_ok = !real && !sym;
+  const int apass = o.anal_pass                           ? o.anal_pass
+                    : pm_ok && precision == GSH_BF16X6 ? gsh::kPass
+                                                          : kDefaultAnalPass;
+  const bool wide = apass == gsh::kWideAnalPass;
+  const bool pm = pm_ok && !wide;
   {
     // 128-column super-groups of the split-BF16 kernel.
     const int nsg = (ncg16 + 7) / 8;
@@ -1207,13 +1222,16 @@ int run_forward(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
       sg.push_back(mx);
     }
     WorkList w;
-    if ((st = ensure_worklist(g, wide ? 3 : 1, lmax, n, real, nsg, sg, &w))) return st;
+    if ((st = ensure_worklist(g, pm ? 5 : wide ? 3 : 1, lmax, n, real, nsg, sg, &w))) return st;
     p.items = w.d;
     p.items_per = w.per;
   }
   const int ns = precision == GSH_BF16X6 ? 3 : 2;
   GSH_HIP(timed(g, GSH_KERNEL_LEGENDRE_ANALYSIS, s,
-                [&] { return gsh::launch_leg_anal_bf16(p, ns, s, (wide ? 1 : 0) + (sym ? 2 : 0)); }));
+                [&] {
+                  return gsh::launch_leg_anal_bf16(p, ns, s,
+                                                   pm ? 4 : (wide ? 1 : 0) + (sym ? 2 : 0));
+                }));
   return GSH_OK;
 }
 

@@ -24,6 +24,9 @@ constexpr int kPass = 64;
 constexpr int kCk = 16;
 // Passes of the wide split-BF16 analysis (leg_anal_bf16_w): 128 degrees.
 constexpr int kWideAnalPass = 128;
+// Floats of the split-BF16 analysis buffer FA32 kept before its first block
+// (32 rows of 16 columns, 2 KiB: the alignment of the buffer is kept).
+constexpr int kFAFrontPad = 512;
 static_assert(kPass % kCk == 0, "passes start at checkpoints");
 
 // Parameters of one Legendre-contraction launch (synthesis, analysis or
@@ -285,9 +288,10 @@ hipError_t launch_leg_synth_bf16(const LegParams& p, int ns, hipStream_t s);
 // partial sums, and the sums over the blocks into F after it.
 hipError_t launch_tail_dvals(const LegParams& p, hipStream_t s);
 hipError_t launch_tail_reduce(const LegParams& p, hipStream_t s);
-// wide: 128-degree passes (leg_anal_bf16_w; work list of family 3) instead of 64.
-// form: 0 passes of 64 degrees, 1 passes of 128 degrees; 2 and 3 the same
-// over the northern colatitudes (upper index 0, symmetric grids).
+// form: 0 passes of 64 degrees, 1 passes of 128 degrees (work list family
+// 3); 2 and 3 the same over the northern colatitudes (upper index 0,
+// symmetric grids); 4 passes of 64 degrees of orders m and -m together
+// (complex fields; work list family 5, orders m >= 0).
 hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s, int form = 0);
 
 // FP32 <-> FP64 coefficient staging of batch plans (gsh_batch.hip); strides

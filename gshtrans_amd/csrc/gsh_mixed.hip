@@ -953,6 +953,227 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
 }
 
 // ---------------------------------------------------------------------------
+// Analysis of complex fields, orders m and -m in one workgroup.  The
+// Gauss-Legendre nodes and weights are symmetric about the equator and
+// d^l_{-m,n}(theta) = (-1)^(l+n) d^l_{m,n}(pi - theta), so
+//   c_{l,-m} = (-1)^(l+n) sum_t d^l_{m,n}(theta_t) FA_{-m}(nt-1-t):
+// the recurrence values of order m >= 0 serve both orders.  Items, production
+// and chunk order are leg_anal_bf16's (order m, a 64-degree pass, 128
+// columns, colatitudes in groups of 128); wave w multiplies the 4 degree
+// tiles with column tile w of order m and column tile w of order -m, whose
+// B operand reads the bins of row nt-1-theta for K index theta.  Per MFMA,
+// half the A-operand LDS reads and half the recurrence work of leg_anal_bf16.
+// The +m sums are leg_anal_bf16's bits; the -m sums group the colatitudes
+// into chunks from the other pole.  Order 0 has no partner: its second tile
+// repeats the first and is not stored.
+// ---------------------------------------------------------------------------
+template <int NS>
+__global__ __launch_bounds__(512) void leg_anal_bf16_pm(LegParams p) {
+  static_assert(kPass == 64 && kCk == 16, "4 degree quarters of 16 per pass");
+  __shared__ __bf16 Dl[2][NS][64][136];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  // Work item: (order m >= 0, pass, column super-group).
+  const uint2 it = work_item(p.items, p.items_per);
+  if (it.x == ~0u) return;
+  const int m = it.x;
+  const int pass = it.y & 0xffff;
+  const int csg = it.y >> 16;
+  const int slot = m - p.m_first, slotn = -m - p.m_first;
+  const int l0 = max(m, p.nabs);
+  const int nt = p.ntheta;
+  const int ngrp = (nt + 127) >> 7;
+  const int lp = l0 + kPass * pass;
+  rec_t* rq = rec_ptr(p.rec, ld_uniform(p.roff, m + p.L) - l0);
+
+  // Production (leg_anal_bf16's): block half bh, degree quarter q.
+  const int bh = wave & 1, q = wave >> 1;
+  const int nck = (p.L - l0 + kCk) / kCk;
+  const int kq = min((kPass / kCk) * pass + q, nck - 1);
+  const double2* ckp = kq == 0 ? p.ck : p.ck + (ld_uniform(p.ckoff, m + p.L) + kq - 1) * (long)nt;
+  const double* sdp = p.seeds + (long)(m + p.L) * nt;
+  double nx_seed, nx_cs;
+  double2 nx_ck;
+  auto prefetch = [&](int g) {
+    const int tc = min((2 * g + bh) * 64 + lane, nt - 1);
+    nx_seed = sdp[tc];
+    nx_ck = ckp[tc];
+    nx_cs = p.cosv[tc];
+  };
+  double pe0, pe1, pcs;
+  auto begin = [&]() {
+    pcs = nx_cs;
+    pe0 = kq == 0 ? nx_seed : nx_ck.x;
+    pe1 = kq == 0 ? 0.0 : nx_ck.y;
+  };
+  auto produce4 = [&](int buf, int part) {
+    const int eq = lp + 16 * q + 1;
+#pragma unroll
+    for (int k = 4 * part; k < 4 * part + 4; k += 2) {
+      const double v0 = pe0;
+      rec_step(rec_entry(rq, eq + k), pcs, pe0, pe1);
+      const double v1 = pe0;
+      rec_step(rec_entry(rq, eq + k + 1), pcs, pe0, pe1);
+      unsigned o[NS];
+      split_pair<NS>(v0, v1, o);
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        const bf16x2 pr = __builtin_bit_cast(bf16x2, o[s]);
+        Dl[buf][s][16 * q + k][bh * 64 + lane] = pr[0];
+        Dl[buf][s][16 * q + k + 1][bh * 64 + lane] = pr[1];
+      }
+    }
+  };
+
+  // B operands: lane (q4, c) of K chunk kc takes colatitudes theta = 32 kc +
+  // 8 q4 + e (e = 0..7) of column 16 tile + c.  Order m reads FA row theta
+  // (leg_anal_bf16's addresses); order -m reads row nt-1-theta, 16 floats
+  // before the previous e within the (slot, column group) block, whose rows
+  // are 16 contiguous floats each.  Rows below 0 (theta >= ntheta: the last
+  // chunk only, at most 31 rows) read the previous block or the buffer's
+  // front pad of kFAFrontPad floats; the split zeroes them.
+  struct RawB {
+    float x[8];
+  };
+  const int tile = csg * 8 + wave;
+  const int tcl = min(tile, p.n_cg - 1);
+  const float* fb =
+      p.FA32 + (((long)slot * p.n_cg + tcl) * p.nquad + 2 * (lane >> 4)) * 64 + (lane & 15);
+  const float* fbn = p.FA32 + ((long)slotn * p.n_cg + tcl) * p.nquad * 64 + (lane & 15);
+  const int rn0 = nt - 1 - 8 * (lane >> 4);  // row of order -m at kc = 0, e = 0
+  auto loadB = [&](int kc, RawB (&X)[2]) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) X[0].x[e] = fb[(long)kc * 512 + 16 * e];
+    const float* rn = fbn + 16 * (rn0 - 32 * kc);
+#pragma unroll
+    for (int e = 0; e < 8; e++) X[1].x[e] = rn[-16 * e];
+  };
+  auto splitB = [&](const RawB& X, uint4 (&B)[NS], int kc, bool last) {
+    unsigned pl[NS][4];
+    const int th = 32 * kc + 8 * (lane >> 4);
+#pragma unroll
+    for (int e2 = 0; e2 < 4; e2++) {
+      float x0 = X.x[2 * e2], x1 = X.x[2 * e2 + 1];
+      if (last) {
+        x0 = th + 2 * e2 < nt ? x0 : 0.0f;
+        x1 = th + 2 * e2 + 1 < nt ? x1 : 0.0f;
+      }
+      unsigned o[NS];
+      split_pair_f<NS>(x0, x1, o);
+#pragma unroll
+      for (int s = 0; s < NS; s++) pl[s][e2] = o[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) B[s] = make_uint4(pl[s][0], pl[s][1], pl[s][2], pl[s][3]);
+  };
+
+  d4 acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; i++) acc[i][0] = acc[i][1] = d4{0.0, 0.0, 0.0, 0.0};
+  // Chunk sums flushed in pairs, as in leg_anal_bf16 (per tile and order).
+  f4 cp[4][2], pp[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) cp[i][j] = pp[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  auto step = [&](int buf, int kc, const RawB (&X)[2], bool last, bool odd) {
+    uint4 B[2][NS];
+    splitB(X[0], B[0], kc, last);
+    splitB(X[1], B[1], kc, last);
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+      uint4 A[2][NS];
+#pragma unroll
+      for (int lt = 0; lt < 2; lt++)
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+          A[lt][s] = *reinterpret_cast<const uint4*>(
+              &Dl[buf][s][(2 * half + lt) * 16 + (lane & 15)][(kc & 3) * 32 + 8 * (lane >> 4)]);
+      f4 c[2][2];
+      split_mma<NS, 2, 2>(A, B, c);
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          const int t = 2 * half + i;
+          if (odd)
+            pp[t][j] = cp[t][j];
+          else
+            flush(acc[t][j], pp[t][j] + cp[t][j]);
+          cp[t][j] = c[i][j];
+        }
+    }
+  };
+
+  prefetch(0);
+  begin();
+#pragma unroll
+  for (int part = 0; part < 4; part++) produce4(0, part);
+  prefetch(1);
+  __syncthreads();
+  // B operands one chunk ahead: chunk c in X[c & 1] (loads clamp to the
+  // last chunk nkc - 1).
+  const int kcl = p.nkc - 1;
+  RawB X[2][2];
+  loadB(0, X[0]);
+  for (int g = 0; g + 1 < ngrp; g++) {
+    const int buf = g & 1;
+    const int kc0 = 4 * g;
+    begin();
+    prefetch(g + 2);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      loadB(min(kc0 + j + 1, kcl), X[(j + 1) & 1]);
+      produce4(buf ^ 1, j);
+      step(buf, kc0 + j, X[j & 1], false, j & 1);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+  }
+  {
+    // Last group: chunks 4 (ngrp-1) .. nkc-1 (X[0] holds the first).
+    const int buf = (ngrp - 1) & 1;
+    const int kc0 = 4 * (ngrp - 1);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (kc0 + j < p.nkc) {
+        if (kc0 + j + 1 < p.nkc) loadB(kc0 + j + 1, X[(j + 1) & 1]);
+        step(buf, kc0 + j, X[j & 1], true, j & 1);
+      }
+    const bool lodd = (p.nkc - 1) & 1;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) flush(acc[i][j], lodd ? pp[i][j] + cp[i][j] : cp[i][j]);
+  }
+
+  // Rows l = lp + 16*lt + 4*(lane/16) + r, column 16*tile + lane%16; order
+  // -m's sums take the sign (-1)^(l+n).
+  const int col = tile * 16 + (lane & 15);
+  if (col >= p.ncol) return;
+  const int lf = field_lmax(p, col >> 1);
+  const long cbase = (long)field_index(p.fmap, col >> 1) * p.cstride;
+#pragma unroll
+  for (int lt = 0; lt < 4; lt++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int l = lp + 16 * lt + 4 * (lane >> 4) + r;
+      if (l > lf) continue;
+      const long ci = (cbase + coef_index(l, m, p.nabs, 0)) * 2 + (col & 1);
+      double v = acc[lt][0][r];
+      if (p.accumulate) v += p.cout[ci];
+      if (p.zero_last && lf == p.L && l == lf && m == lf) v = 0.0;
+      p.cout[ci] = v;
+      if (m > 0) {
+        const long cn = (cbase + coef_index(l, -m, p.nabs, 0)) * 2 + (col & 1);
+        double w = (l + p.n) & 1 ? -acc[lt][1][r] : acc[lt][1][r];
+        if (p.accumulate) w += p.cout[cn];
+        p.cout[cn] = w;
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Analysis for upper index 0 over the northern colatitudes.  Gauss-Legendre
 // nodes are symmetric about the equator and d^l_{m0}(pi - theta) =
 // (-1)^(l+m) d^l_{m0}(theta), so sum_t d_l(theta_t) FA(t) runs over the
@@ -1819,7 +2040,20 @@ hipError_t launch_anal_bf16_w(const LegParams& p0, hipStream_t s, bool sym) {
 hipError_t launch_leg_anal_bf16(const LegParams& p, int ns, hipStream_t s, int form) {
   if (!p.ck || !p.ckoff) return hipErrorInvalidValue;  // checkpoint table required
   if (!p.FA32) return hipErrorInvalidValue;             // FP32 bins required
-  if (form >= 2 && p.nabs != 0) return hipErrorInvalidValue;  // symmetry of n = 0 only
+  if ((form == 2 || form == 3) && p.nabs != 0) return hipErrorInvalidValue;  // n = 0 only
+  if (form == 4) {  // orders m and -m together: complex fields
+    if (p.real) return hipErrorInvalidValue;
+    LegParams q = p;
+    q.n_cg = (q.ncol + 15) / 16;
+    q.nquad = (q.ntheta + 3) / 4;
+    q.nkc = (q.ntheta + 31) / 32;
+    const dim3 grid(8 * q.items_per);  // work list: (order m >= 0, pass, column super-group)
+    if (ns == 3)
+      hipLaunchKernelGGL((leg_anal_bf16_pm<3>), grid, dim3(512), 0, s, q);
+    else
+      hipLaunchKernelGGL((leg_anal_bf16_pm<2>), grid, dim3(512), 0, s, q);
+    return hipGetLastError();
+  }
   if (form == 1 || form == 3)
     return ns == 3 ? launch_anal_bf16_w<3>(p, s, form == 3) : launch_anal_bf16_w<2>(p, s, form == 3);
   return ns == 3 ? launch_anal_bf16_t<3>(p, s, form == 2) : launch_anal_bf16_t<2>(p, s, form == 2);

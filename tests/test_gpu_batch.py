@@ -240,11 +240,15 @@ def test_tuned_choices_are_cached_and_persisted(torch, tmp_path, monkeypatch):
 
 def test_split_bf16_analysis_pass_choice(torch, tmp_path, monkeypatch):
     """The split-BF16 analysis runs in 64- or 128-degree passes, whichever the
-    tuner timed faster per group; both pass lengths accumulate the same
-    products in the same order, so a plan's result does not depend on the
-    choice.  A second cache file that pins every shape to the other pass
-    length (read when GSH_TUNE_CACHE names a new file; its lines replace the
-    process's choices) forces it, and the results stay bit-identical."""
+    tuner timed faster per group.  For upper index 0 both pass lengths
+    accumulate the same products in the same order, so the result does not
+    depend on the choice; complex fields of other upper indices run orders m
+    and -m together in 64-degree passes (the -m sums over mirrored
+    colatitudes, chunked from the other pole), which agree with the
+    128-degree kernel to the split-BF16 error.  A second cache file that pins
+    every shape to the other pass length (read when GSH_TUNE_CACHE names a new
+    file; its lines replace the process's choices) forces it: the n = 0
+    field stays bit-identical, the others within 1e-7 of each other."""
     L = 512
     lmax = np.array([512, 400, 512, 300])
     n = np.array([0, 1, -1, 2])
@@ -275,10 +279,14 @@ def test_split_bf16_analysis_pass_choice(torch, tmp_path, monkeypatch):
     passes2 = [d["analysis_pass"] for d in p2.info()]
     say(f"[pass] forced analysis passes {passes2}")
     assert all(a + b == 192 for a, b in zip(passes, passes2))
-    assert torch.equal(a1, a2)
+    assert torch.equal(a1[0], a2[0])  # n = 0
+    d = float(torch.linalg.vector_norm(a1[1:] - a2[1:]) / torch.linalg.vector_norm(a1[1:]))
+    say(f"[pass] n != 0: 64- vs 128-degree passes {d:.2e}")
+    assert 0 < d < 1e-7
     fp64 = fp.forward(f)
-    e = float(torch.linalg.vector_norm(a1 - fp64) / torch.linalg.vector_norm(fp64))
-    say(f"[pass] bf16x6 vs FP64 {e:.2e}")
-    assert e < 1e-7
+    for a in (a1, a2):
+        e = float(torch.linalg.vector_norm(a - fp64) / torch.linalg.vector_norm(fp64))
+        say(f"[pass] bf16x6 vs FP64 {e:.2e}")
+        assert e < 1e-7
     for p in (fp, p1, p2):
         p.close()
