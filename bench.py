@@ -48,15 +48,15 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "transform_kernels_lmax1024_b64_co
 # batch), as regular expressions over the summary's kernel names.
 FAMILY_KERNELS = {
     # split-BF16 Legendre kernels: the n=0 batch runs the equatorially
-    # symmetric variants (_ws, _s), the n=2 batch the general ones (bf16x6
-    # analysis: orders m and -m paired, _pm)
+    # symmetric variants (_ws, _s), the n=2 batch orders m and -m paired
+    # (_pm; the bf16x3 analysis: the 128-degree kernel _w)
     "legendre_analysis": {"fp64": r"leg_anal_f64<4, false>", "fp32": r"leg_anal_f64<4, true>",
                           "bf16x3": r"leg_anal_bf16_ws?<2, \d>",
                           "bf16x6": r"leg_anal_bf16_(ws<3, \d>|pm<3>)"},
     "legendre_synthesis": {"fp64": r"leg_synth_f64<2, \d, false>",
                            "fp32": r"leg_synth_f64<2, \d, true>",
-                           "bf16x3": r"leg_synth_bf16(_s)?<2, \d>",
-                           "bf16x6": r"leg_synth_bf16(_s)?<3, \d>"},
+                           "bf16x3": r"leg_synth_bf16_(s<2, \d>|pm<2>)",
+                           "bf16x6": r"leg_synth_bf16_(s<3, \d>|pm<3>)"},
     # FA in FP64 (fp64, fp32) or FP32 (split-BF16 modes): last template argument
     "fft_analysis": {"fp64": r"fft_anal_w<32, \d, 0, 0>", "fp32": r"fft_anal_w<32, \d, 0, 0>",
                      "bf16x3": r"fft_anal_w<32, \d, 0, 1>", "bf16x6": r"fft_anal_w<32, \d, 0, 1>"},

@@ -586,14 +586,19 @@ int ensure_synth_worklist(gsh_grid_s* g, int family, int lmax, int real, int ngr
     *out = it->second;
     return GSH_OK;
   }
-  const int nslot = real ? lmax + 1 : 2 * lmax + 1;
+  // Family 22 (orders m and -m in one item) lists the orders m >= 0, an
+  // item of m > 0 weighing twice its degrees.
+  const bool paired = family == 22;
+  const int nslot = real || paired ? lmax + 1 : 2 * lmax + 1;
   std::vector<std::pair<int, int>> units;  // (weight, mw * ngroups + grp)
   units.reserve((size_t)nslot * ngroups);
   // mw is the kernels' order index: complex orders interleave as
-  // 0, -1, 1, -2, 2, ... (order_of_host).
+  // 0, -1, 1, -2, 2, ... (order_of_host); paired items take mw = m.
   for (int mw = 0; mw < nslot; mw++)
     for (int grp = 0; grp < ngroups; grp++)
-      units.emplace_back(lmax - std::abs(order_of_host(mw, real)) + 1, mw * ngroups + grp);
+      units.emplace_back(paired ? (lmax - mw + 1) * (mw > 0 ? 2 : 1)
+                                : lmax - std::abs(order_of_host(mw, real)) + 1,
+                         mw * ngroups + grp);
   std::stable_sort(units.begin(), units.end(),
                    [](const auto& a, const auto& b) { return a.first > b.first; });
   std::vector<uint2> xcd[8];
@@ -1067,16 +1072,23 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
     // parity layout, and the equator of an odd ntheta as a tail row.
     // Otherwise up to kSynthTailRows colatitudes past whole super-blocks of
     // 256 go to the tail kernels rather than to items of their own.
+    // Complex fields of other upper indices run orders m and -m together
+    // (items of 128 colatitudes; order -m's rows mirrored, its coefficients
+    // signed by coef_split), with up to kSynthTailRows colatitudes past whole
+    // blocks of 128 as tail rows (order -m's then sit at the top rows).
     const int nt = g->ntheta();
     const int sym = n == 0 ? 1 : 0;
+    const int pm = !sym && !real ? 1 : 0;
+    const int blk = pm ? 128 : 256;  // colatitudes per item (sym: northern ones)
     const bool tail =
-        sym ? nt % 2 == 1 : nt > 256 && nt % 256 != 0 && nt % 256 <= gsh::kSynthTailRows;
-    const int nblk = sym ? (nt / 2 + 127) / 128 : tail ? nt / 256 : ntsb;
+        sym ? nt % 2 == 1 : nt > blk && nt % blk != 0 && nt % blk <= gsh::kSynthTailRows;
+    const int nblk = sym ? (nt / 2 + 127) / 128 : tail ? nt / blk : (nt + blk - 1) / blk;
     WorkList w;
-    if ((st = ensure_synth_worklist(g, sym ? 21 : 20, lmax, real, ((p.ncol + 15) / 16 + 7) / 8,
-                                    nblk, &w)))
+    if ((st = ensure_synth_worklist(g, sym ? 21 : pm ? 22 : 20, lmax, real,
+                                    ((p.ncol + 15) / 16 + 7) / 8, nblk, &w)))
       return st;
     p.sym = sym;
+    p.pm = pm;
     p.items = w.d;
     p.items_per = w.per;
     const int ns = precision == GSH_BF16X6 ? 3 : 2;
@@ -1094,7 +1106,7 @@ int run_inverse(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
     const long nblocks = nchunks * p.ntile;
     // The tail rows' d values (32 per block and row) and partial sums (16 per
     // block, tile and row) follow the split coefficients in the same buffer.
-    const int trows = !tail ? 0 : sym ? 1 : nt % 256;
+    const int trows = !tail ? 0 : sym ? 1 : nt % blk;
     const size_t bs_bytes = (size_t)nblocks * ns * 64 * 16;
     const size_t dv_bytes = (size_t)trows * nchunks * 32 * sizeof(double);
     const size_t pt_bytes = (size_t)trows * nblocks * 16 * sizeof(double);

@@ -128,6 +128,12 @@ struct LegParams {
   // degrees l0 + 64 c + 2 j + ((k + l0 + m) % 2), j = 0..31, of class k
   // (k = (l + m) % 2).  0: blocks of 32 consecutive degrees.
   int sym;
+  // Split-BF16 synthesis of complex fields with orders m and -m paired
+  // (leg_synth_bf16_pm): coef_split multiplies the coefficients of orders
+  // m < 0 by (-1)^(l+n), tail_dvals gives those orders the d values of order
+  // |m| at the same rows, and tail_reduce writes their sums to the mirrored
+  // rows ntheta-1-t.
+  int pm;
 };
 
 

@@ -171,8 +171,12 @@ __global__ __launch_bounds__(512) void coef_split(LegParams p) {
     for (int r = 0; r < kSplitWin / 8; r++) {
       const int l = lw0 + 8 * r + (tid >> 6);
       v[r] = make_double2(0.0, 0.0);
-      if (l <= lf && l >= abs(m) && l >= p.nabs)
+      if (l <= lf && l >= abs(m) && l >= p.nabs) {
         v[r] = *reinterpret_cast<const double2*>(cp + 2 * coef_index(l, m, p.nabs, REAL));
+        // Paired orders: -m takes (-1)^(l+n) c_{l,-m} (the values of order
+        // |m| at the mirrored colatitude multiply it).
+        if (p.pm && m < 0 && ((l + p.n) & 1)) v[r] = make_double2(-v[r].x, -v[r].y);
+      }
     }
 #pragma unroll
     for (int r = 0; r < kSplitWin / 8; r++)
@@ -707,6 +711,244 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_s(LegParams p) {
         } else {
           p.F[fi + dn] = e + o;
           p.F[fm - dn] = e - o;
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Synthesis of complex fields, orders m and -m in one workgroup.  The
+// Gauss-Legendre nodes are symmetric about the equator and d^l_{-m,n}(theta)
+// = (-1)^(l+n) d^l_{m,n}(pi - theta), so
+//   F_{-m}(nt-1-t) = sum_l d^l_{m,n}(theta_t) (-1)^(l+n) c_{l,-m}:
+// the recurrence values of order m >= 0 at colatitude t serve order -m at
+// the mirrored row, against its coefficients with the sign folded in
+// (coef_split with p.pm).  Workgroup = 8 waves for (128 colatitudes of block
+// tsb) x (128 columns of super-group csg) of orders m and -m: the same MFMAs
+// per recurrence value as leg_synth_bf16 over twice the columns, so half the
+// recurrence work and coefficient reads per MFMA.  The degrees go in
+// 64-degree chunks (bf16 planes of D double-buffered in LDS), each taken as
+// four K blocks of 32 -- (degrees 0-31, m), (0-31, -m), (32-63, m),
+// (32-63, -m) -- with the B operands double-buffered per block.  While the
+// waves multiply chunk c they produce chunk c + 1: wave w runs the
+// recurrence of colatitude block w % 2 (64 colatitudes) for the 16 degrees
+// of quarter w / 2 from the checkpoint there, two recurrence pairs per K
+// block.  Wave w multiplies colatitude half h = w / 4 (4 row groups) with
+// column tiles 2 (w % 4), 2 (w % 4) + 1 of both orders.  Blocks past the
+// order's last 32-degree chunk, and order -0, are skipped.
+// ---------------------------------------------------------------------------
+template <int NS>
+__global__ __launch_bounds__(512) void leg_synth_bf16_pm(LegParams p) {
+  static_assert(kCk == 16, "four 16-degree quarters per 64-degree chunk");
+  // Planes of a chunk: degree half dh, entries 8 kr .. 8 kr + 7, 128 colatitudes.
+  __shared__ uint4 trb[2][NS][2][4][128];
+  // B operands of a block (8 column tiles x NS planes x 64 lanes).
+  __shared__ uint4 bsh[2][8][NS][64];
+  // Recurrence coefficients of a chunk (64 entries, 2 KiB).
+  __shared__ uint4 rsc[2][128];
+  const int lane = threadIdx.x & 63;
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  // Work item: (order m >= 0, column super-group, block of 128 colatitudes).
+  const uint2 it = work_item(p.items, p.items_per);
+  if (it.x == ~0u) return;
+  const int m = it.x;
+  const int tsb = it.y & 0xffff;
+  const int csg = it.y >> 16;
+  const int slot = m - p.m_first, slotn = -m - p.m_first;
+  const int l0 = max(m, p.nabs);
+  const int nt = p.ntheta;
+  const int nC = (p.lmax - l0 + 64) >> 6;   // 64-degree chunks
+  const int n32 = (p.lmax - l0 + 32) >> 5;  // 32-degree coefficient blocks of each order
+  const int rows = min(128, nt - tsb * 128);  // valid colatitudes of the item
+
+  // Producer share: colatitude block pb, degree quarter pq of a chunk.
+  const int pb = wave & 1, pq = wave >> 1;
+  const int tpc = min(tsb * 128 + 64 * pb + lane, nt - 1);  // past ntheta: the last one
+  const double pcs = p.cosv[tpc];
+  const int nck = (p.L - l0 + kCk) / kCk;  // checkpoints 1 .. nck-1 exist
+  const double2* ckm =
+      (nck > 1 ? p.ck + ld_uniform(p.ckoff, m + p.L) * (long)nt : p.ck) + tpc;
+  auto ck_state = [&](int k) { return ckm[(long)(nck > 1 ? min(k, nck - 1) - 1 : 0) * nt]; };
+  double pe0, pe1;
+  {
+    // Chunk 0: the seed for quarter 0, checkpoint pq for the others.
+    const double seed = p.seeds[(long)(m + p.L) * nt + tpc];
+    const double2 c1 = ck_state(max(pq, 1));
+    pe0 = pq == 0 || nck < 2 ? seed : c1.x;
+    pe1 = pq == 0 || nck < 2 ? 0.0 : c1.y;
+  }
+
+  const uint4* recq = reinterpret_cast<const uint4*>(p.rec + (ld_uniform(p.roff, m + p.L) - l0));
+  auto fetch_rec = [&](int c) { return recq[2 * (l0 + 64 * min(c, nC - 1) + 1) + (tid & 127)]; };
+
+  // B operand staging of block b = 4 c + 2 dh + o: coefficient block 2 c + dh
+  // (clamped to the last) of order m (o = 0) or -m (o = 1); piece j of this
+  // thread is uint4 tid + 512 j of the block's 8 contiguous tiles.
+  const uint4* bbp = p.bs + (ld_uniform(p.cso, slot) * (long)p.ntile + csg * 8) * NS * 64;
+  const uint4* bbn = p.bs + (ld_uniform(p.cso, slotn) * (long)p.ntile + csg * 8) * NS * 64;
+  const int bstride = p.ntile * NS * 64;  // uint4 per coefficient block
+  struct Pieces {
+    uint4 v[NS];
+  };
+  auto fetch = [&](int b) {
+    const uint4* cb = ((b & 1) ? bbn : bbp) + (long)min(b >> 1, n32 - 1) * bstride;
+    Pieces q;
+#pragma unroll
+    for (int j = 0; j < NS; j++) q.v[j] = cb[tid + 512 * j];
+    return q;
+  };
+  auto store = [&](int buf, const Pieces q) {
+#pragma unroll
+    for (int j = 0; j < NS; j++) (&bsh[buf][0][0][0])[512 * j + tid] = q.v[j];
+  };
+
+  // Consumer role.
+  const int h = wave >> 2;
+  const int tile0 = csg * 8 + (wave & 3) * 2;
+  d4 acc[4][2][2];  // row group, column tile, order
+#pragma unroll
+  for (int g = 0; g < 4; g++)
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+      for (int o = 0; o < 2; o++) acc[g][t][o] = d4{0.0, 0.0, 0.0, 0.0};
+
+  // Production: pair i (0..7) of this wave's quarter is steps 2 i, 2 i + 1,
+  // degrees 16 pq + 2 i (+1) of the chunk; every 4 pairs fill the uint4 of
+  // each plane holding entries 8 kr .. 8 kr + 7 of degree half pq / 2 (kr =
+  // 2 (pq % 2) + i / 4), stored whole.
+  unsigned pl[NS][4];
+  auto produce_pair = [&](int rbuf, int tbuf, int i) {
+    const d4* rl = reinterpret_cast<const d4*>(&rsc[rbuf][0]) + 16 * pq;
+    const double v0 = pe0;
+    rec_step(rl[2 * i], pcs, pe0, pe1);
+    const double v1 = pe0;
+    rec_step(rl[2 * i + 1], pcs, pe0, pe1);
+    unsigned o[NS];
+    split_pair<NS>(v0, v1, o);
+#pragma unroll
+    for (int s = 0; s < NS; s++) pl[s][i & 3] = o[s];
+    if ((i & 3) == 3) {
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+        trb[tbuf][s][pq >> 1][2 * (pq & 1) + (i >> 2)][64 * pb + lane] =
+            make_uint4(pl[s][0], pl[s][1], pl[s][2], pl[s][3]);
+    }
+  };
+  auto loadA = [&](int abuf, int dh, int g, uint4 (&A)[1][NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) A[0][s] = trb[abuf][s][dh][lane >> 4][h * 64 + g * 16 + (lane & 15)];
+  };
+  auto consume_group = [&](int g, int o, const uint4 (&A)[1][NS], const uint4 (&B)[2][NS]) {
+    if (h * 64 + g * 16 < rows) {
+      f4 c[1][2];
+      split_mma<NS, 1>(A, B, c);
+      flush(acc[g][0][o], c[0][0]);
+      flush(acc[g][1][o], c[0][1]);
+    }
+  };
+  auto loadB = [&](int buf, uint4 (&B)[2][NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      B[0][s] = bsh[buf][(wave & 3) * 2][s][lane];
+      B[1][s] = bsh[buf][(wave & 3) * 2 + 1][s][lane];
+    }
+  };
+
+  // Prologue: coefficients of chunks 0 and 1, B operands of block 0,
+  // production of chunk 0.
+  if (tid < 128) {
+    rsc[0][tid] = fetch_rec(0);
+    rsc[1][tid] = fetch_rec(1);
+  }
+  store(0, fetch(0));
+  __syncthreads();
+  const bool pv = 64 * pb < rows;  // this wave's producer block holds colatitudes
+  if (pv) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) produce_pair(0, 0, i);
+  }
+  __syncthreads();
+
+  // Block 4 c + k of chunk c (k a constant at each call): the B operands of
+  // the next block fetched, production pairs 2 k, 2 k + 1 of chunk c + 1
+  // (none in the last chunk).  The state of chunk c + 1 (checkpoint
+  // 4 (c + 1) + pq) is loaded one chunk before its use.
+  double2 nx = ck_state(4 + pq);
+  auto block = [&](int c, const int k, const bool next) {
+    const int b = 4 * c + k;
+    const int abuf = c & 1;
+    const int dh = k >> 1, o = k & 1;
+    const bool prod = pv && c + 1 < nC;
+    const bool live = 2 * c + dh < n32 && (o == 0 || m > 0);
+    Pieces q;
+    if (next) q = fetch(b + 1);
+    uint4 qr = make_uint4(0u, 0u, 0u, 0u);
+    if (k == 0) {
+      pe0 = nx.x;
+      pe1 = nx.y;
+      nx = ck_state(4 * (c + 2) + pq);
+    } else if (k == 3 && tid < 128) {
+      qr = fetch_rec(c + 2);
+    }
+    if (live) {
+      uint4 B[2][NS];
+      loadB(b & 1, B);
+      uint4 Ab[2][1][NS];
+      loadA(abuf, dh, 0, Ab[0]);
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        if (g < 3) loadA(abuf, dh, g + 1, Ab[(g + 1) & 1]);
+        if (prod && (g & 1)) produce_pair(abuf ^ 1, abuf ^ 1, 2 * k + (g >> 1));
+        consume_group(g, o, Ab[g & 1], B);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    } else if (prod) {
+      produce_pair(abuf ^ 1, abuf ^ 1, 2 * k);
+      produce_pair(abuf ^ 1, abuf ^ 1, 2 * k + 1);
+    }
+    if (next) store((b + 1) & 1, q);  // buffers last read in block b - 1
+    // Chunk c + 2's coefficients (rsc[c & 1] was last read in chunk c - 1).
+    if (k == 3 && tid < 128) rsc[abuf][tid] = qr;
+    __syncthreads();
+  };
+  for (int c = 0; c + 1 < nC; c++) {
+    block(c, 0, true);
+    block(c, 1, true);
+    block(c, 2, true);
+    block(c, 3, true);
+  }
+  block(nC - 1, 0, true);
+  block(nC - 1, 1, true);
+  block(nC - 1, 2, true);
+  block(nC - 1, 3, false);
+
+  // Order m at rows t, order -m at rows nt-1-t (C layout of the 16x16x32
+  // MFMA: lane holds rows 4 (lane / 16) + r, column lane % 16).
+  const int ncg16 = (p.ncol + 15) >> 4;
+  const long rs = (long)ncg16 * p.nslot * 16;
+  const int row0 = tsb * 128 + h * 64 + 4 * (lane >> 4);
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    const int col = (tile0 + t) * 16 + (lane & 15);
+    if (col >= p.ncol) continue;
+    const long fi = f_index(row0, slot, col, p.nslot, ncg16);
+    const long fm = f_index(nt - 1 - row0, slotn, col, p.nslot, ncg16);
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = row0 + g * 16 + r;
+        if (row >= nt) continue;
+        const long dn = (long)(g * 16 + r) * rs;  // rows past row0 / before nt-1-row0
+        const double a = acc[g][t][0][r], an = acc[g][t][1][r];
+        if (p.f32out) {
+          reinterpret_cast<float*>(p.F)[fi + dn] = (float)a;
+          if (m > 0) reinterpret_cast<float*>(p.F)[fm - dn] = (float)an;
+        } else {
+          p.F[fi + dn] = a;
+          if (m > 0) p.F[fm - dn] = an;
         }
       }
   }
@@ -1882,7 +2124,9 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_ws(LegParams p) {
 __global__ __launch_bounds__(64) void tail_dvals(LegParams p, double* dv) {
   const int slot = blockIdx.x;
   const int t = p.tail_t0 + blockIdx.y;
-  const int m = slot + p.m_first;
+  // Paired orders (p.pm): -m takes the values of order |m| at row t, which
+  // multiply its sign-folded coefficients into its sum at row ntheta-1-t.
+  const int m = p.pm ? abs(slot + p.m_first) : slot + p.m_first;
   const int l0 = max(abs(m), p.nabs);
   const int cw = p.sym ? 64 : 32;  // degrees per chunk of the split layout
   const int nchunk = (p.lmax - l0 + cw) / cw;
@@ -1930,7 +2174,8 @@ __global__ __launch_bounds__(256) void tail_reduce(LegParams p) {
   for (int col = threadIdx.x; col < p.ncol; col += 256) {
     double sum = 0.0;
     for (int c = 0; c < nchunk; c++) sum += pp[c * cstride + col];
-    const long fi = f_index(t, slot, col, p.nslot, ncg16);
+    // Paired orders: the sums of orders m < 0 belong to row ntheta-1-t.
+    const long fi = f_index(p.pm && m < 0 ? p.ntheta - 1 - t : t, slot, col, p.nslot, ncg16);
     if (p.f32out)
       reinterpret_cast<float*>(p.F)[fi] = (float)sum;
     else
@@ -1974,6 +2219,14 @@ hipError_t launch_leg_synth_bf16(const LegParams& p0, int ns, hipStream_t s) {
       hipLaunchKernelGGL((leg_synth_bf16_s<2, 1>), grid, dim3(512), 0, s, p);
     else
       hipLaunchKernelGGL((leg_synth_bf16_s<2, 0>), grid, dim3(512), 0, s, p);
+    return hipGetLastError();
+  }
+  if (p.pm) {  // orders m and -m together: complex fields
+    if (p.real) return hipErrorInvalidValue;
+    if (ns == 3)
+      hipLaunchKernelGGL((leg_synth_bf16_pm<3>), grid, dim3(512), 0, s, p);
+    else
+      hipLaunchKernelGGL((leg_synth_bf16_pm<2>), grid, dim3(512), 0, s, p);
     return hipGetLastError();
   }
   if (ns == 3 && p.real)

@@ -81,7 +81,10 @@ def test_mixed_synthesis_last_rows(torch, prec, L, n, real, batch):
     from the FP64 coefficients and the FP64 recurrence values, and a last
     kernel sums the blocks.  Upper index 0 is synthesized over the northern
     colatitudes (their mirrors from the same sums) and its tail row is the
-    equator.  The whole field meets the mode's bar, and those rows equal the
+    equator.  Complex fields of other upper indices pair orders m and -m,
+    order -m at the mirrored rows: their tail rows are the last ones for
+    m >= 0 and the first ones for m < 0, checked on fields of one order sign
+    each.  The whole field meets the mode's bar, and those rows equal the
     FP64 transform to 1e-12 in either mode when the sums reach the ring FFT
     in FP64 (the LDS ring FFT, here L = 257), and to one FP32 rounding of the
     bins (2^-23) when they reach the wavefront ring FFT in FP32 (L = 256,
@@ -99,17 +102,37 @@ def test_mixed_synthesis_last_rows(torch, prec, L, n, real, batch):
     nt = L + 1
     assert 1 <= nt % 256 <= 2
     row_bar = 2.0 ** -23 if 2 * L in (512, 1024, 2048) else 1e-12
-    # Upper index 0 is synthesized over the northern colatitudes, and its
-    # tail row is the equator of an odd ntheta; otherwise the last rows.
-    tail = ([nt // 2] if nt % 2 else []) if n == 0 else list(range(nt // 256 * 256, nt))
-    for t in tail:
-        e = per_field_rel(torch, fmx[:, t:t + 1], f64[:, t:t + 1])
-        say(f"    row {t}: max {e.max():.2e} (bar {row_bar:.1e})")
-        assert e.max() <= row_bar
-    # The row before them comes from the MFMA items.
-    if tail:
-        t = tail[0] - 1
-        assert per_field_rel(torch, fmx[:, t:t + 1], f64[:, t:t + 1]).max() > 1e-12
+
+    def check(fm, fr, tail, mfma_row):
+        for t in tail:
+            e = per_field_rel(torch, fm[:, t:t + 1], fr[:, t:t + 1])
+            say(f"    row {t}: max {e.max():.2e} (bar {row_bar:.1e})")
+            assert e.max() <= row_bar
+        # the row next to them comes from the MFMA items
+        if tail:
+            t = mfma_row
+            assert per_field_rel(torch, fm[:, t:t + 1], fr[:, t:t + 1]).max() > 1e-12
+
+    if n == 0:
+        # synthesized over the northern colatitudes; the tail row is the
+        # equator of an odd ntheta
+        tail = [nt // 2] if nt % 2 else []
+        check(fmx, f64, tail, nt // 2 - 1)
+    elif real:
+        tail = list(range(nt // 256 * 256, nt))
+        check(fmx, f64, tail, tail[0] - 1)
+    else:
+        # orders m, -m paired: order -m's rows mirrored
+        k = nt % 128
+        l = np.arange(abs(n), L + 1)
+        order = np.concatenate([np.arange(-j, j + 1) for j in l])
+        for sign, tail, mfma_row in ((1, list(range(nt - k, nt)), nt - k - 1),
+                                     (-1, list(range(k)), k)):
+            keep = torch.from_numpy((order >= 0) if sign > 0 else (order < 0)).cuda()
+            cs = c * keep
+            say(f"    orders {'m >= 0' if sign > 0 else 'm < 0'} only")
+            check(g.inverse(L, n, cs, real=real, precision=prec), g.inverse(L, n, cs, real=real),
+                  tail, mfma_row)
 
 
 @pytest.mark.parametrize("prec", ["bf16x6", "bf16x3", "fp32"])
