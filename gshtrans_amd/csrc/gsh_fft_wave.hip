@@ -507,7 +507,25 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
     // X[k + N/2] = E[k] - W_N^k O[k] (k = lane + 64 t < N/2).
     constexpr int R2 = R / 2;
     double2 e[R2], o[R2];
-    if (fv) {
+    if (fv && !p.src_cplx && !p.op) {
+      // Plain fields: every load issued before the first use (a branch
+      // between the loads would make each one wait for the last).  A missing
+      // second field reads the first and is zeroed after the loads.
+      const long ia = (long)field_index(p.fmap, f0) * p.fstride + (long)theta * N + 2 * lane;
+      const long ib =
+          f1v ? (long)field_index(p.fmap, f0 + 1) * p.fstride + (long)theta * N + 2 * lane : ia;
+#pragma unroll
+      for (int a = 0; a < R2; a++) {
+        const double2 pa = ld_pair<F32>(p.fin, ia + 128 * a);
+        const double2 pb = ld_pair<F32>(p.fin, ib + 128 * a);
+        e[a] = make_double2(pa.x, pb.x);
+        o[a] = make_double2(pa.y, pb.y);
+      }
+      if (!f1v) {
+#pragma unroll
+        for (int a = 0; a < R2; a++) e[a].y = o[a].y = 0.0;
+      }
+    } else if (fv) {
       const long ia = (long)field_index(p.fmap, f0) * p.fstride + (long)theta * N + 2 * lane;
       const long ib =
           f1v ? (long)field_index(p.fmap, f0 + 1) * p.fstride + (long)theta * N + 2 * lane : ia;
@@ -551,7 +569,11 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
     const long ia = (long)field_index(p.fmap, f0) * p.fstride + (long)theta * N + lane;
     const long ib =
         REAL && f1v ? (long)field_index(p.fmap, f0 + 1) * p.fstride + (long)theta * N + lane : ia;
-    if (!REAL) {
+    if (!REAL && !p.op) {
+      // Plain fields: every load issued before the first use.
+#pragma unroll
+      for (int n1 = 0; n1 < R; n1++) v[n1] = ld_cplx<F32>(p.fin, ia + 64 * n1);
+    } else if (!REAL) {
 #pragma unroll
       for (int n1 = 0; n1 < R; n1++) {
         double2 x = ld_cplx<F32>(p.fin, ia + 64 * n1);
