@@ -166,17 +166,24 @@ __global__ __launch_bounds__(512) void coef_split(LegParams p) {
     const bool ok = sl < p.nslot && fld < p.nfield;
     const int lf = ok ? field_lmax(p, fld) : -1;
     const double* cp = p.cin + (ok ? (long)field_index(p.fmap, fld) * p.cstride * 2 : 0);
+    // Every load is issued unconditionally (degrees outside the order or the
+    // field read its first coefficient and are zeroed after the loads), so
+    // none waits for the one before.  Paired orders: -m takes (-1)^(l+n)
+    // c_{l,-m} (the values of order |m| at the mirrored colatitude multiply
+    // it).
     double2 v[kSplitWin / 8];
 #pragma unroll
     for (int r = 0; r < kSplitWin / 8; r++) {
       const int l = lw0 + 8 * r + (tid >> 6);
-      v[r] = make_double2(0.0, 0.0);
-      if (l <= lf && l >= abs(m) && l >= p.nabs) {
-        v[r] = *reinterpret_cast<const double2*>(cp + 2 * coef_index(l, m, p.nabs, REAL));
-        // Paired orders: -m takes (-1)^(l+n) c_{l,-m} (the values of order
-        // |m| at the mirrored colatitude multiply it).
-        if (p.pm && m < 0 && ((l + p.n) & 1)) v[r] = make_double2(-v[r].x, -v[r].y);
-      }
+      const bool in = l <= lf && l >= abs(m) && l >= p.nabs;
+      v[r] = *reinterpret_cast<const double2*>(cp + (in ? 2 * coef_index(l, m, p.nabs, REAL) : 0));
+    }
+#pragma unroll
+    for (int r = 0; r < kSplitWin / 8; r++) {
+      const int l = lw0 + 8 * r + (tid >> 6);
+      const bool in = l <= lf && l >= abs(m) && l >= p.nabs;
+      const double sg = p.pm && m < 0 && ((l + p.n) & 1) ? -1.0 : 1.0;
+      v[r] = in ? make_double2(sg * v[r].x, sg * v[r].y) : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int r = 0; r < kSplitWin / 8; r++)
