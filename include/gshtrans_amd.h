@@ -243,7 +243,10 @@ int gsh_plan_forward(gsh_plan_t plan, const void* field, long field_stride, void
 int gsh_plan_info(gsh_plan_t plan, int* ngroups, int cap, int* n, int* lmax, int* count,
                   int* synth_tiles, int* fft_inverse, int* fft_forward);
 /* The tuned degrees per pass of each group's split-BF16 analysis (64 or
- * 128; 0 for FP64 / FP32 plans and before the first analysis).  Tuned
+ * 128; 0 for FP64 / FP32 plans and before the first analysis; for complex
+ * fields of upper index n != 0, 64 is the kernel that runs orders m and -m
+ * together, whose -m sums agree with the 128-degree kernel's to the
+ * split-BF16 error, not bit for bit).  Tuned
  * choices are kept per launch shape for the life of the process and, when
  * the environment variable GSH_TUNE_CACHE names a file, read from and
  * appended to it, so later plans of a shape skip the timing. */
