@@ -185,6 +185,8 @@ def run_gpu(args):
         two[m] = {"transforms_per_s": args.steps * 4 * BATCH / dt,
                   "ms_per_step": 1e3 * dt / args.steps}
         log(f"[bench] two streams, {m}: {two[m]['transforms_per_s']:.1f} transforms/s")
+    if two and headline in two and "fp64" in two:
+        two["speedup_vs_fp64"] = two[headline]["transforms_per_s"] / two["fp64"]["transforms_per_s"]
     results["two_streams"] = two or None
 
     # Accuracy of each mode against FP64 per field (relative L2).

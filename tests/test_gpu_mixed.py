@@ -57,7 +57,8 @@ def per_field_rel(torch, a, b):
                                             (100, 1, True, 70), (1024, 0, True, 64),
                                             (1024, 2, False, 64), (63, 0, True, 4),
                                             (255, 0, False, 3), (512, 0, False, 20),
-                                            (40, 0, True, 150)])
+                                            (40, 0, True, 150), (64, 1, False, 5),
+                                            (129, -2, False, 3), (127, 2, False, 3)])
 def test_mixed_synthesis_error(torch, prec, L, n, real, batch):
     say(f"[mixed synthesis] {prec} L={L} n={n} real={real} batch={batch}")
     g = grid(L)
@@ -142,7 +143,8 @@ def test_mixed_synthesis_last_rows(torch, prec, L, n, real, batch):
                                             (100, 1, True, 70), (24, 1, False, 4),
                                             (1024, 0, True, 64), (1024, 2, False, 64),
                                             (63, 0, True, 4), (255, 0, False, 3),
-                                            (512, 0, False, 20)])
+                                            (512, 0, False, 20), (64, 1, False, 5),
+                                            (129, -2, False, 3), (127, 2, False, 3)])
 def test_mixed_analysis_error(torch, prec, L, n, real, batch):
     say(f"[mixed analysis] {prec} L={L} n={n} real={real} batch={batch}")
     g = grid(L)
@@ -161,14 +163,18 @@ def test_mixed_analysis_error(torch, prec, L, n, real, batch):
 
 
 @pytest.mark.parametrize("prec", ["bf16x6", "bf16x3"])
-@pytest.mark.parametrize("L,real", [(64, True), (64, False), (63, True), (1024, True)])
-def test_mixed_analysis_general_field_n0(torch, prec, L, real):
+@pytest.mark.parametrize("L,n,real", [(64, 0, True), (64, 0, False), (63, 0, True),
+                                      (1024, 0, True), (64, 1, False), (63, -2, False),
+                                      (1024, 2, False)])
+def test_mixed_analysis_general_field(torch, prec, L, n, real):
     """Upper index 0 is analysed over the northern colatitudes against the sum
-    and difference of each row and its mirror.  A field of independent
-    random samples (not a synthesized, band-limited one) makes the northern
-    and southern rows unrelated: the split-BF16 analysis must still meet the
-    mode's bar against FP64, for odd (equator row) and even row counts."""
-    say(f"[mixed analysis, random field n=0] {prec} L={L} real={real}")
+    and difference of each row and its mirror; complex fields of other upper
+    indices analyse order -m against the mirrored rows.  A field of
+    independent random samples (not a synthesized, band-limited one) makes
+    the northern and southern rows unrelated: the split-BF16 analysis must
+    still meet the mode's bar against FP64, for odd (equator row) and even
+    row counts."""
+    say(f"[mixed analysis, random field] {prec} L={L} n={n} real={real}")
     g = grid(L)
     gen = torch.Generator(device="cuda").manual_seed(SEED0 + L + real)
     shape = (5, L + 1, 2 * L)
@@ -177,15 +183,19 @@ def test_mixed_analysis_general_field_n0(torch, prec, L, real):
     else:
         f = torch.complex(torch.randn(shape, dtype=torch.float64, device="cuda", generator=gen),
                           torch.randn(shape, dtype=torch.float64, device="cuda", generator=gen))
-    a64 = g.forward(L, 0, f, real=real)
-    amx = g.forward(L, 0, f, real=real, precision=prec)
+    a64 = g.forward(L, n, f, real=real)
+    amx = g.forward(L, n, f, real=real, precision=prec)
     e = per_field_rel(torch, amx, a64)
     say(f"    analysis max {e.max():.2e}")
     assert e.max() <= BARS[prec]
 
 
-def test_mixed_accumulate_and_zero_last(torch):
-    L, n = 32, 0
+@pytest.mark.parametrize("n", [0, 1])
+def test_mixed_accumulate_and_zero_last(torch, n):
+    """Accumulating analysis adds into the output; a complex field at the
+    grid degree zeroes coefficient (L, L) only (order -L at degree L, which
+    the paired-order kernel writes from order L's sums, stays)."""
+    L = 32
     g = grid(L)
     c = torch.from_numpy(G.random_coefficients(L, n, seed=3, batch=2)).cuda()
     f = g.inverse(L, n, c)
