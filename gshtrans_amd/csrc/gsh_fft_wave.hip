@@ -32,10 +32,18 @@ namespace {
 constexpr int lds_per_wave(int R) { return (R >= 32 ? R / 2 : R) * 64; }
 // Wavefronts (rings) per workgroup: 8 complex or 16 real fields of one
 // colatitude, so the workgroup gathers and scatters the order bins of F / FA
-// in whole 128-byte lines (4-wavefront workgroups, two per CU, measured
-// slower: their 64-byte pieces cost more than the overlap gained).  At most
-// 256 VGPRs (two wavefronts per SIMD), as the launch bounds require.
-constexpr int waves_for(int) { return 8; }
+// in whole pieces of 8 rings.  At N = 2048 (256 VGPRs per wave for 32
+// complex doubles per lane, two wavefronts per SIMD) such a workgroup fills
+// a CU and its load, transform and store phases run one after the other;
+// 4-wavefront workgroups run two per CU, whose phases overlap.  They move
+// the bins in pieces of 4 rings, which the XCD pairing of the grid
+// (xcd_pair_map) sends through one L2 with the other 4 rings of the 8:
+// bf16x6 at lMax=1024, B=64, synthesis 0.38 -> 0.32 ms (real pairs) and
+// 0.69 -> 0.64 ms (complex), real-pair analysis 0.52 -> 0.50 ms; the
+// complex analysis, whose FP32 bins would go out in 32-byte pieces, stays
+// at 8 (0.83 against 1.07 ms).
+constexpr int synth_waves(int R) { return R >= 32 ? 4 : 8; }
+constexpr int anal_waves(int R, int real) { return R >= 32 && real ? 4 : 8; }
 
 // cos(pi k / 16) and sin(pi k / 16), k = 0..16.
 constexpr double kCos16[17] = {
@@ -263,12 +271,35 @@ __device__ __forceinline__ void st2(double* p, double2 v) {
 // that every global access of F / FA covers whole 128-byte lines.  Round r of
 // H = N / LPW rounds covers LPW = lds_per_wave(R) bins (ring bins k = r LPW +
 // b for complex rings; bins k = r LPW / 2 + b, xa and xb apart, when the
-// analysis separates a real pair).  Thread tid moves bins b = 64 i + tid / 8
-// of ring c = tid % 8; ring c's entry b sits at lds[c LPW + (b ^ 2c)], which
-// spreads every 16-lane phase of the b128 accesses over all 64 banks, both
-// for the 8 rings x 2 bins of a cooperative access and for the 16
-// consecutive bins a ring reads or writes.
-__device__ __forceinline__ int xch(int c, int b) { return b ^ (2 * c); }
+// analysis separates a real pair).  With W rings per workgroup, thread tid
+// moves bins b = 64 i + tid / W of ring c = tid % W; ring c's entry b sits at
+// lds[c LPW + (b ^ (16 / W) c)], which spreads every 16-lane phase of the
+// b128 accesses over all 64 banks, both for the W rings x 16 / W bins of a
+// cooperative access and for the 16 consecutive bins a ring reads or writes.
+template <int W>
+__device__ __forceinline__ int xch(int c, int b) { return b ^ ((16 / W) * c); }
+
+// Workgroup -> (field group, colatitude).  Units of 8 workgroups -- field
+// groups 2j and 2j+1 at the 4 colatitudes of a quad -- go to the XCDs in
+// turn (workgroup b runs on XCD b % 8), each unit's workgroups back to back
+// on its XCD: the pieces that the two field groups and neighbouring
+// colatitudes write or read of the same lines of F / FA meet in that XCD's
+// L2.  nfg is even; workgroups past the grid return false.
+__device__ __forceinline__ bool xcd_pair_map(int nfg, int ntheta, int& fg, int& theta) {
+  const int b = blockIdx.x;
+  const int x = b & 7, sq = b >> 3;
+  const int u = ((sq >> 3) << 3) | x;  // unit
+  const int np = nfg >> 1;
+  const int quad = u / np;
+  fg = 2 * (u - quad * np) + (sq & 1);
+  theta = 4 * quad + ((sq >> 1) & 3);
+  return theta < ntheta;
+}
+// Grid of xcd_pair_map: whole units for every XCD.
+inline unsigned xcd_pair_grid(int nfg, int ntheta) {
+  const long units = (long)(nfg >> 1) * ((ntheta + 3) >> 2);
+  return static_cast<unsigned>((units + 7) / 8 * 64);
+}
 
 // Synthesis: order bins of F[theta][slot][col] -> backward DFT -> field ring.
 // Every wave stays to the end (the bin exchange synchronises the workgroup);
@@ -284,28 +315,28 @@ __device__ __forceinline__ int xch(int c, int b) { return b ^ (2 * c); }
 //    slot N/2 (lane 0, register R/2) is loaded by the ring itself.  c2r
 //    ignores Im of bins 0 and N/2.
 template <int R, int REAL, int F32, int FA32>
-__global__ __launch_bounds__(64 * waves_for(R), 2) void fft_synth_w(FFTParams p) {
-  constexpr int W = waves_for(R);
+__global__ __launch_bounds__(64 * synth_waves(R), 2) void fft_synth_w(FFTParams p) {
+  constexpr int W = synth_waves(R);
   constexpr int LPW = lds_per_wave(R);
   constexpr int N = 64 * R;
   constexpr int H = N / LPW;      // exchange rounds
   constexpr int CPT = LPW / 64;   // complex rings: bins per thread and round
   constexpr int BH = LPW / 2;     // real pairs: slots per round
   constexpr int CPH = BH / 64;
-  static_assert(W == 8 && LPW % 128 == 0, "8 rings x 16 bytes = one line of F per bin");
+  static_assert((W == 8 || W == 4) && LPW % 128 == 0, "exchange of 4 or 8 rings");
   __shared__ double2 lds_all[W * LPW];
   const int lane = threadIdx.x & 63;
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
-  const int fg = blockIdx.x;
-  const int theta = blockIdx.y;
+  int fg, theta;
+  if (!xcd_pair_map(p.nfg, p.ntheta, fg, theta)) return;
   const int f0 = (fg * W + wave) * (REAL ? 2 : 1);
   const bool fv = f0 < p.nfield;
   const bool f1v = REAL && f0 + 1 < p.nfield;
   const int lmax = p.lmax, m_first = p.m_first;
   const int ncg16 = (p.ncol + 15) >> 4;
   // This thread's share of the cooperative gather: ring c, bins 64 i + bq.
-  const int c = tid & 7, bq = tid >> 3;
+  const int c = tid % W, bq = tid / W;
   const int cf = (fg * W + c) * (REAL ? 2 : 1);
   const bool cv = cf < p.nfield, cv1 = REAL && cf + 1 < p.nfield;
   // Bin (re, im) at element i of F: FP64, or FP32 (FA32: the same indices in
@@ -356,12 +387,12 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_synth_w(FFTParams p)
       for (int i = 0; i < CPT; i++) {
         double2 x = d2(sok[r][i], st[r][i]);
         if (r == RH && i == IH) x = cadd(x, d2(both, xm));
-        lds_all[c * LPW + xch(c, 64 * i + bq)] = x;
+        lds_all[c * LPW + xch<W>(c, 64 * i + bq)] = x;
       }
       __syncthreads();
 #pragma unroll
       for (int j = 0; j < CPT; j++)
-        v[r * CPT + j] = lds_all[wave * LPW + xch(wave, 64 * j + lane)];
+        v[r * CPT + j] = lds_all[wave * LPW + xch<W>(wave, 64 * j + lane)];
     }
   } else {
     Bin sa[H][CPH], sb[H][CPH];
@@ -386,15 +417,15 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_synth_w(FFTParams p)
       if (r > 0) __syncthreads();
 #pragma unroll
       for (int i = 0; i < CPH; i++) {
-        lds_all[c * LPW + xch(c, 64 * i + bq)] = d2(aok[r][i], sa[r][i]);
-        lds_all[c * LPW + BH + xch(c, 64 * i + bq)] = d2(bok[r][i], sb[r][i]);
+        lds_all[c * LPW + xch<W>(c, 64 * i + bq)] = d2(aok[r][i], sa[r][i]);
+        lds_all[c * LPW + BH + xch<W>(c, 64 * i + bq)] = d2(bok[r][i], sb[r][i]);
       }
       __syncthreads();
       const double2* lw = lds_all + wave * LPW;
 #pragma unroll
       for (int j = 0; j < CPH; j++) {  // k = kk = r BH + 64 j + lane < N/2
-        double2 xa = lw[xch(wave, 64 * j + lane)];
-        double2 xb = lw[BH + xch(wave, 64 * j + lane)];
+        double2 xa = lw[xch<W>(wave, 64 * j + lane)];
+        double2 xb = lw[BH + xch<W>(wave, 64 * j + lane)];
         if (r == 0 && j == 0 && lane == 0) {
           xa.y = 0.0;
           xb.y = 0.0;
@@ -408,8 +439,8 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_synth_w(FFTParams p)
       for (int n1 = R - (r + 1) * CPH; n1 <= R - r * CPH && n1 < R; n1++) {
         const int kk = N - 64 * n1 - lane;
         const bool in = kk >= r * BH && kk < (r + 1) * BH;
-        const double2 xa = lw[xch(wave, in ? kk - r * BH : 0)];
-        const double2 xb = lw[BH + xch(wave, in ? kk - r * BH : 0)];
+        const double2 xa = lw[xch<W>(wave, in ? kk - r * BH : 0)];
+        const double2 xb = lw[BH + xch<W>(wave, in ? kk - r * BH : 0)];
         const double2 X = make_double2(xa.x + xb.y, xb.x - xa.y);  // conj xa + i conj xb
         const double2 keep = n1 == R - r * CPH ? v[n1] : make_double2(0.0, 0.0);
         v[n1] = in ? X : keep;
@@ -479,21 +510,21 @@ __device__ __forceinline__ void st_bin(double* F, long i, double2 X) {
 // ring row also zeroes the padding colatitudes of the last quad.  A wave
 // without a field transforms zeros and stores nothing.
 template <int R, int REAL, int F32, int FA32>
-__global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) {
-  constexpr int W = waves_for(R);
+__global__ __launch_bounds__(64 * anal_waves(R, REAL), 2) void fft_anal_w(FFTParams p) {
+  constexpr int W = anal_waves(R, REAL);
   constexpr int LPW = lds_per_wave(R);
   constexpr int N = 64 * R;
   constexpr int H = N / LPW;      // exchange rounds
   constexpr int CPT = LPW / 64;   // complex rings: bins per thread and round
   constexpr int BH = LPW / 2;     // real pairs: bins per round (xa, xb apart)
   constexpr int CPH = BH / 64;
-  static_assert(W == 8 && LPW % 128 == 0, "8 rings x 16 bytes = one line of FA per bin");
+  static_assert((W == 8 || W == 4) && LPW % 128 == 0, "exchange of 4 or 8 rings");
   __shared__ double2 lds_all[W * LPW];
   const int lane = threadIdx.x & 63;
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
-  const int fg = blockIdx.x;
-  const int theta = blockIdx.y;
+  int fg, theta;
+  if (!xcd_pair_map(p.nfg, p.ntheta, fg, theta)) return;
   const int f0 = (fg * W + wave) * (REAL ? 2 : 1);
   const bool fv = f0 < p.nfield;
   const bool f1v = REAL && f0 + 1 < p.nfield;
@@ -612,7 +643,7 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
   const double w = p.wq[theta];
   const int n_cg = p.n_cg, nquad = p.nquad;
   // This thread's share of the cooperative scatter: ring c, bins 64 i + bq.
-  const int c = tid & 7, bq = tid >> 3;
+  const int c = tid % W, bq = tid / W;
   const int cf = (fg * W + c) * (REAL ? 2 : 1);
   const bool cv = cf < p.nfield, cv1 = REAL && cf + 1 < p.nfield;
   if (!REAL) {
@@ -621,13 +652,13 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
       if (r > 0) __syncthreads();  // round r - 1 stored
 #pragma unroll
       for (int j = 0; j < CPT; j++)
-        lw[xch(wave, 64 * j + lane)] = make_double2(v[r * CPT + j].x * w, v[r * CPT + j].y * w);
+        lw[xch<W>(wave, 64 * j + lane)] = make_double2(v[r * CPT + j].x * w, v[r * CPT + j].y * w);
       __syncthreads();
 #pragma unroll
       for (int i = 0; i < CPT; i++) {
         const int b = 64 * i + bq;
         const int k = r * LPW + b;
-        const double2 X = lds_all[c * LPW + xch(c, b)];
+        const double2 X = lds_all[c * LPW + xch<W>(c, b)];
         if (cv && k <= lmax) st_bin<FA32>(p.F, fa_index(k - m_first, 2 * cf, theta, n_cg, nquad), X);
         if (cv && k >= N - lmax)
           st_bin<FA32>(p.F, fa_index(k - N - m_first, 2 * cf, theta, n_cg, nquad), X);
@@ -641,8 +672,8 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
       for (int j = 0; j < CPH; j++) {
         double2 xa, xb;
         separate<R>(v, r * CPH + j, lane, w, p.adjoint, xa, xb);
-        lw[xch(wave, 64 * j + lane)] = xa;
-        lw[BH + xch(wave, 64 * j + lane)] = xb;
+        lw[xch<W>(wave, 64 * j + lane)] = xa;
+        lw[BH + xch<W>(wave, 64 * j + lane)] = xb;
       }
       __syncthreads();
 #pragma unroll
@@ -650,10 +681,10 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
         const int b = 64 * i + bq;
         const int k = r * BH + b;
         if (cv && k <= lmax) {
-          st_bin<FA32>(p.F, fa_index(k, 2 * cf, theta, n_cg, nquad), lds_all[c * LPW + xch(c, b)]);
+          st_bin<FA32>(p.F, fa_index(k, 2 * cf, theta, n_cg, nquad), lds_all[c * LPW + xch<W>(c, b)]);
           if (cv1)
             st_bin<FA32>(p.F, fa_index(k, 2 * cf + 2, theta, n_cg, nquad),
-                         lds_all[c * LPW + BH + xch(c, b)]);
+                         lds_all[c * LPW + BH + xch<W>(c, b)]);
         }
       }
     }
@@ -683,7 +714,8 @@ __global__ __launch_bounds__(64 * waves_for(R), 2) void fft_anal_w(FFTParams p) 
 
 template <int R>
 hipError_t launch_synth_r(const FFTParams& p, hipStream_t s) {
-  const dim3 grid(p.nfg, p.ntheta), block(64 * waves_for(R));
+  // one workgroup per (field group, colatitude), in xcd_pair_map's order
+  const dim3 grid(xcd_pair_grid(p.nfg, p.ntheta)), block(64 * synth_waves(R));
   if (p.fa32) {
     if (p.real && p.f32)
       hipLaunchKernelGGL((fft_synth_w<R, 1, 1, 1>), grid, block, 0, s, p);
@@ -706,7 +738,8 @@ hipError_t launch_synth_r(const FFTParams& p, hipStream_t s) {
 
 template <int R>
 hipError_t launch_anal_r(const FFTParams& p, hipStream_t s) {
-  const dim3 grid(p.nfg, p.ntheta), block(64 * waves_for(R));
+  // one workgroup per (field group, colatitude), in xcd_pair_map's order
+  const dim3 grid(xcd_pair_grid(p.nfg, p.ntheta)), block(64 * (p.real ? anal_waves(R, 1) : anal_waves(R, 0)));
   if (p.fa32) {
     if (p.real && p.f32)
       hipLaunchKernelGGL((fft_anal_w<R, 1, 1, 1>), grid, block, 0, s, p);
@@ -731,15 +764,17 @@ hipError_t launch_anal_r(const FFTParams& p, hipStream_t s) {
 
 bool wave_fft_supported(int N) { return N == 512 || N == 1024 || N == 2048; }
 
-// Field groups of a launch: waves_for(R) rings per workgroup, 2 real fields per ring.
-static int field_groups(const FFTParams& p) {
-  const int per = waves_for(p.N / 64) * (p.real ? 2 : 1);
-  return (p.nfield + per - 1) / per;
+// Field groups of a launch (W rings per workgroup, 2 real fields per ring),
+// rounded up to an even count for xcd_pair_map (the extra group's waves
+// transform zeros and store nothing).
+static int field_groups(const FFTParams& p, int W) {
+  const int per = W * (p.real ? 2 : 1);
+  return ((p.nfield + per - 1) / per + 1) / 2 * 2;
 }
 
 hipError_t launch_fft_synth_wave(const FFTParams& p0, hipStream_t s) {
   FFTParams p = p0;
-  p.nfg = field_groups(p);
+  p.nfg = field_groups(p, synth_waves(p.N / 64));
   switch (p.N) {
     case 512: return launch_synth_r<8>(p, s);
     case 1024: return launch_synth_r<16>(p, s);
@@ -750,7 +785,7 @@ hipError_t launch_fft_synth_wave(const FFTParams& p0, hipStream_t s) {
 
 hipError_t launch_fft_anal_wave(const FFTParams& p0, hipStream_t s) {
   FFTParams p = p0;
-  p.nfg = field_groups(p);
+  p.nfg = field_groups(p, anal_waves(p.N / 64, p.real));
   p.n_cg = (p.ncol + 15) / 16;
   p.nquad = (p.ntheta + 3) / 4;
   switch (p.N) {

@@ -11,13 +11,15 @@ fft_anal_w), re-traced lane by lane in numpy:
 * the real-pair unpacking X_a = (Z[k] + conj Z[N-k]) / 2,
   X_b = (Z[k] - conj Z[N-k]) / 2i with the partner lane / register used by
   the kernel reproduces the FFTs of the two real rings;
-* the exchange of order bins between the 8 rings of a workgroup through LDS
-  (fft_synth_w gather, fft_anal_w scatter): every LDS entry of a round is
-  written exactly once and stays inside the workgroup buffer, every 16-lane
-  phase is free of bank conflicts, each ring receives exactly the bins the
+* the exchange of order bins between the 4 or 8 rings of a workgroup
+  through LDS (fft_synth_w gather, fft_anal_w scatter, with the ring counts
+  of synth_waves / anal_waves): every LDS entry of a round is written
+  exactly once and stays inside the workgroup buffer, every 16-lane phase
+  is free of bank conflicts, each ring receives exactly the bins the
   per-ring formulas give (complex rings, real pairs, lmax below and at N/2,
   rings without a field), and every wave instruction of the cooperative
-  global access covers whole 128-byte lines of F / FA.
+  global access covers whole 128-byte lines of F / FA (the workgroup's half
+  of each line for 4 complex rings).
 """
 import numpy as np
 import pytest
@@ -156,8 +158,18 @@ def lds_per_wave(R):
     return (R // 2 if R >= 32 else R) * 64
 
 
-def xch(c, b):
-    return b ^ (2 * c)
+def synth_waves(R):
+    """Rings per synthesis workgroup (synth_waves in gsh_fft_wave.hip)."""
+    return 4 if R >= 32 else 8
+
+
+def anal_waves(R, real):
+    """Rings per analysis workgroup (anal_waves in gsh_fft_wave.hip)."""
+    return 4 if R >= 32 and real else 8
+
+
+def xch(c, b, W):
+    return b ^ ((16 // W) * c)
 
 
 def check_b128_phases(addr16):
@@ -167,49 +179,52 @@ def check_b128_phases(addr16):
         assert len(set(s.tolist())) == 16
 
 
-def check_whole_lines(lanes, line, piece):
+def check_whole_lines(lanes, line, piece, npieces=8):
     """Global accesses of one wave instruction (or of the xa / xb pair of a
-    real pair): every 128-byte line touched by lanes `lanes` is covered by all
-    8 of its 16-byte pieces."""
+    real pair): every 128-byte line touched by lanes `lanes` is covered by its
+    first `npieces` 16-byte pieces (all 8, or the 4 of a 4-ring workgroup of
+    complex fields, whose paired workgroup moves the other 4 through the same
+    L2)."""
     got = {}
     for t in np.flatnonzero(lanes):
         got.setdefault(int(line[t]), set()).add(int(piece[t]))
     for ln, pcs in got.items():
-        assert pcs == set(range(8)), (ln, sorted(pcs))
+        assert pcs == set(range(npieces)), (ln, sorted(pcs))
 
 
 CASES = [(R, real, part, nf) for R in (8, 16, 32) for real in (False, True)
          for part in (False, True) for nf in ("full", "short")]
 
 
-def case_params(R, real, part, nf):
+def case_params(R, real, part, nf, W=8):
     N = 64 * R
     lmax = N // 2 - (37 if part else 0)
-    nfield = (16 if real else 8) if nf == "full" else (11 if real else 5)
+    nfield = (2 * W if real else W) if nf == "full" else (2 * W - 5 if real else W - 3)
     m_first = 0 if real else -lmax
     nslot = lmax + 1 if real else 2 * lmax + 1
     return N, lmax, nfield, m_first, nslot
 
 
-def coop_threads(real, nfield):
+def coop_threads(real, nfield, W):
     """Cooperative share of thread tid: ring c, bins 64 i + bq; first column
     2 cf of the ring in its workgroup's columns."""
-    tid = np.arange(512)
-    c, bq = tid & 7, tid >> 3
+    tid = np.arange(64 * W)
+    c, bq = tid % W, tid // W
     cf = c * (2 if real else 1)
     return c, bq, cf, cf < nfield, real & (cf + 1 < nfield)
 
 
 @pytest.mark.parametrize("R,real,part,nf", CASES)
 def test_synthesis_bin_gather(R, real, part, nf):
-    N, lmax, nfield, m_first, nslot = case_params(R, real, part, nf)
+    W = synth_waves(R)
+    N, lmax, nfield, m_first, nslot = case_params(R, real, part, nf, W)
     LPW = lds_per_wave(R)
     H, CPT, BH = N // LPW, LPW // 64, LPW // 2
     CPH = BH // 64
     rng = np.random.default_rng(R + 7 * real + 3 * part)
     ncg = 2 if real else 1
     F = rng.standard_normal((ncg, nslot, 16))   # one colatitude: F[cg][slot][16]
-    c, bq, cf, cv, cv1 = coop_threads(real, nfield)
+    c, bq, cf, cv, cv1 = coop_threads(real, nfield, W)
     col = 2 * cf
     lane = np.arange(64)
 
@@ -223,14 +238,16 @@ def test_synthesis_bin_gather(R, real, part, nf):
         """mask / colv / slot over 512 threads, or 1024 for the xa, xb pair."""
         if nf != "full":
             return
-        n = len(mask) // 512
-        for wv in range(8):
+        T = 64 * W
+        n = len(mask) // T
+        for wv in range(W):
             sel = np.zeros(len(mask), bool)
             for h in range(n):
-                sel[512 * h + 64 * wv:512 * h + 64 * wv + 64] = True
-            check_whole_lines(mask & sel, (colv // 16) * nslot + slot, (colv % 16) // 2)
+                sel[T * h + 64 * wv:T * h + 64 * wv + 64] = True
+            check_whole_lines(mask & sel, (colv // 16) * nslot + slot, (colv % 16) // 2,
+                              min(8, W * (2 if real else 1)))
 
-    v = np.full((8, R, 64), np.nan + 0j)
+    v = np.full((W, R, 64), np.nan + 0j)
     if not real:
         st = {}
         for r in range(H):
@@ -245,19 +262,19 @@ def test_synthesis_bin_gather(R, real, part, nf):
         both = cv & (bq == 0) & (N // 2 <= lmax)
         st[RH, IH] = st[RH, IH] + ld2(col, np.where(both, -N // 2 - m_first, 0), both)
         for r in range(H):
-            lds = np.full(8 * LPW, np.nan + 0j)
-            written = np.zeros(8 * LPW, int)
+            lds = np.full(W * LPW, np.nan + 0j)
+            written = np.zeros(W * LPW, int)
             for i in range(CPT):
-                addr = c * LPW + xch(c, 64 * i + bq)
-                assert addr.min() >= 0 and addr.max() < 8 * LPW
-                for wv in range(8):
+                addr = c * LPW + xch(c, 64 * i + bq, W)
+                assert addr.min() >= 0 and addr.max() < W * LPW
+                for wv in range(W):
                     check_b128_phases(addr[64 * wv:64 * wv + 64])
                 lds[addr] = st[r, i]
                 np.add.at(written, addr, 1)
             assert (written == 1).all()
-            for w in range(8):
+            for w in range(W):
                 for j in range(CPT):
-                    a = w * LPW + xch(w, 64 * j + lane)
+                    a = w * LPW + xch(w, 64 * j + lane, W)
                     check_b128_phases(a)
                     v[w, r * CPT + j] = lds[a]
     else:
@@ -271,21 +288,21 @@ def test_synthesis_bin_gather(R, real, part, nf):
                             np.concatenate([ks, ks]))
                 sa[r, i], sb[r, i] = ld2(col, ks, ma), ld2(col + 2, ks, mb)
         for r in range(H):
-            lds = np.full(8 * LPW, np.nan + 0j)
-            written = np.zeros(8 * LPW, int)
+            lds = np.full(W * LPW, np.nan + 0j)
+            written = np.zeros(W * LPW, int)
             for i in range(CPH):
-                addr = c * LPW + xch(c, 64 * i + bq)
-                assert addr.min() >= 0 and addr.max() + BH < 8 * LPW
-                for wv in range(8):
+                addr = c * LPW + xch(c, 64 * i + bq, W)
+                assert addr.min() >= 0 and addr.max() + BH < W * LPW
+                for wv in range(W):
                     check_b128_phases(addr[64 * wv:64 * wv + 64])
                     check_b128_phases(addr[64 * wv:64 * wv + 64] + BH)
                 lds[addr], lds[addr + BH] = sa[r, i], sb[r, i]
                 np.add.at(written, addr, 1)
                 np.add.at(written, addr + BH, 1)
             assert (written == 1).all()
-            for w in range(8):
+            for w in range(W):
                 for j in range(CPH):
-                    a = w * LPW + xch(w, 64 * j + lane)
+                    a = w * LPW + xch(w, 64 * j + lane, W)
                     check_b128_phases(a)
                     xa, xb = lds[a].copy(), lds[a + BH].copy()
                     if r == 0 and j == 0:
@@ -294,13 +311,13 @@ def test_synthesis_bin_gather(R, real, part, nf):
                 for n1 in range(R - (r + 1) * CPH, min(R - r * CPH, R - 1) + 1):
                     kk = N - 64 * n1 - lane
                     m = (kk >= r * BH) & (kk < (r + 1) * BH)
-                    a = w * LPW + xch(w, np.where(m, kk - r * BH, 0))
+                    a = w * LPW + xch(w, np.where(m, kk - r * BH, 0), W)
                     assert (a >= w * LPW).all() and (a < w * LPW + BH).all()
                     xa, xb = lds[a], lds[a + BH]
                     # first touch of the register unless round r - 1 set it
                     keep = v[w, n1] if n1 == R - r * CPH else np.zeros(64, complex)
                     v[w, n1] = np.where(m, np.conj(xa) + 1j * np.conj(xb), keep)
-        for w in range(8):   # slot N/2: lane 0 of register R/2, loaded by the ring
+        for w in range(W):   # slot N/2: lane 0 of register R/2, loaded by the ring
             f0 = 2 * w
             na = nb = 0
             if f0 < nfield and N // 2 <= lmax:
@@ -309,7 +326,7 @@ def test_synthesis_bin_gather(R, real, part, nf):
                 nb = F[cg, N // 2, cc + 2] if f0 + 1 < nfield else 0
             v[w, R // 2, 0] = na + 1j * nb
     # Reference: the bins of ring w from the per-ring formulas.
-    for w in range(8):
+    for w in range(W):
         f0 = w * (2 if real else 1)
         if f0 >= nfield:
             continue
@@ -336,12 +353,13 @@ def test_synthesis_bin_gather(R, real, part, nf):
 
 @pytest.mark.parametrize("R,real,part,nf", CASES)
 def test_analysis_bin_scatter(R, real, part, nf):
-    N, lmax, nfield, m_first, nslot = case_params(R, real, part, nf)
+    W = anal_waves(R, real)
+    N, lmax, nfield, m_first, nslot = case_params(R, real, part, nf, W)
     LPW = lds_per_wave(R)
     H, CPT, BH = N // LPW, LPW // 64, LPW // 2
     CPH = BH // 64
     rng = np.random.default_rng(50 + R + 7 * real + 3 * part)
-    c, bq, cf, cv, cv1 = coop_threads(real, nfield)
+    c, bq, cf, cv, cv1 = coop_threads(real, nfield, W)
     lane = np.arange(64)
     ncg = 2 if real else 1
     FA = {}   # (slot, column) -> value
@@ -355,30 +373,31 @@ def test_analysis_bin_scatter(R, real, part, nf):
     def whole_lines(masks_slots_cols):
         if nf != "full":
             return
-        for wv in range(8):
+        for wv in range(W):
             line, piece, sel = [], [], []
             for mask, slot, colv in masks_slots_cols:
                 sl = slice(64 * wv, 64 * wv + 64)
                 sel.append(mask[sl])
                 line.append((colv[sl] // 16) * nslot + slot[sl])
                 piece.append((colv[sl] % 16) // 2)
-            check_whole_lines(np.concatenate(sel), np.concatenate(line), np.concatenate(piece))
+            check_whole_lines(np.concatenate(sel), np.concatenate(line), np.concatenate(piece),
+                              min(8, W * (2 if real else 1)))
 
     if not real:
-        X = rng.standard_normal((8, N)) + 1j * rng.standard_normal((8, N))   # weighted bins
+        X = rng.standard_normal((W, N)) + 1j * rng.standard_normal((W, N))   # weighted bins
         for r in range(H):
-            lds = np.full(8 * LPW, np.nan + 0j)
-            for w in range(8):
+            lds = np.full(W * LPW, np.nan + 0j)
+            for w in range(W):
                 for j in range(CPT):
-                    a = w * LPW + xch(w, 64 * j + lane)
+                    a = w * LPW + xch(w, 64 * j + lane, W)
                     check_b128_phases(a)
                     assert np.isnan(lds[a]).all()
                     lds[a] = X[w, 64 * (r * CPT + j) + lane]
             for i in range(CPT):
                 b = 64 * i + bq
                 k = r * LPW + b
-                a = c * LPW + xch(c, b)
-                for wv in range(8):
+                a = c * LPW + xch(c, b, W)
+                for wv in range(W):
                     check_b128_phases(a[64 * wv:64 * wv + 64])
                 val = lds[a]
                 assert not np.isnan(val).any()
@@ -389,30 +408,30 @@ def test_analysis_bin_scatter(R, real, part, nf):
                 whole_lines([(m1, s1, 2 * cf)])
                 whole_lines([(m2, s2, 2 * cf)])
         expect = {}
-        for w in range(min(8, nfield)):
+        for w in range(min(W, nfield)):
             for k in range(N):
                 if k <= lmax:
                     expect[(k - m_first, 2 * w)] = X[w, k]
                 if k >= N - lmax:
                     expect[(k - N - m_first, 2 * w)] = X[w, k]
     else:
-        XA = rng.standard_normal((8, N // 2)) + 1j * rng.standard_normal((8, N // 2))
-        XB = rng.standard_normal((8, N // 2)) + 1j * rng.standard_normal((8, N // 2))
+        XA = rng.standard_normal((W, N // 2)) + 1j * rng.standard_normal((W, N // 2))
+        XB = rng.standard_normal((W, N // 2)) + 1j * rng.standard_normal((W, N // 2))
         for r in range(H):
-            lds = np.full(8 * LPW, np.nan + 0j)
-            for w in range(8):
+            lds = np.full(W * LPW, np.nan + 0j)
+            for w in range(W):
                 for j in range(CPH):
                     kl = 64 * (r * CPH + j) + lane
-                    a = w * LPW + xch(w, 64 * j + lane)
+                    a = w * LPW + xch(w, 64 * j + lane, W)
                     check_b128_phases(a)
                     check_b128_phases(a + BH)
                     lds[a], lds[a + BH] = XA[w, kl], XB[w, kl]
             for i in range(CPH):
                 b = 64 * i + bq
                 k = r * BH + b
-                a = c * LPW + xch(c, b)
-                assert a.max() + BH < 8 * LPW
-                for wv in range(8):
+                a = c * LPW + xch(c, b, W)
+                assert a.max() + BH < W * LPW
+                for wv in range(W):
                     check_b128_phases(a[64 * wv:64 * wv + 64])
                 m = cv & (k <= lmax)
                 mb = m & cv1
@@ -420,7 +439,7 @@ def test_analysis_bin_scatter(R, real, part, nf):
                 store(mb, k, 2 * cf + 2, lds[a + BH])
                 whole_lines([(m, k, 2 * cf), (mb, k, 2 * cf + 2)])
         expect = {}
-        for w in range(8):
+        for w in range(W):
             f0 = 2 * w
             for k in range(min(lmax + 1, N // 2)):
                 if f0 < nfield:
@@ -468,3 +487,43 @@ def test_real_pair_radix2_split(R):
     for k1 in range(1, R2):
         assert 2 * k1 < R
     assert 7 * 7 * R2 * 2 < N
+
+
+# ---------------------------------------------------------------------------
+# Workgroup order of the wavefront ring FFTs (xcd_pair_map, xcd_pair_grid).
+# ---------------------------------------------------------------------------
+def xcd_pair_grid(nfg, ntheta):
+    units = (nfg >> 1) * ((ntheta + 3) >> 2)
+    return (units + 7) // 8 * 64
+
+
+def xcd_pair_map(b, nfg, ntheta):
+    x, sq = b & 7, b >> 3
+    u = ((sq >> 3) << 3) | x
+    np_ = nfg >> 1
+    quad = u // np_
+    fg = 2 * (u - quad * np_) + (sq & 1)
+    theta = 4 * quad + ((sq >> 1) & 3)
+    return (fg, theta) if theta < ntheta else None
+
+
+@pytest.mark.parametrize("nfg,ntheta", [(2, 1), (2, 3), (4, 5), (8, 257), (16, 1025),
+                                        (6, 65), (10, 130), (16, 513)])
+def test_xcd_pair_map(nfg, ntheta):
+    """Every (field group, colatitude) runs exactly once; the two field groups
+    of a pair and the four colatitudes of a quad run on one XCD (b % 8),
+    within 8 consecutive workgroups of that XCD."""
+    where = {}
+    for b in range(xcd_pair_grid(nfg, ntheta)):
+        r = xcd_pair_map(b, nfg, ntheta)
+        if r is None:
+            continue
+        fg, t = r
+        assert 0 <= fg < nfg and 0 <= t < ntheta
+        assert r not in where
+        where[r] = (b & 7, (b >> 3) >> 3)   # XCD, unit slot on that XCD
+    assert len(where) == nfg * ntheta
+    for (fg, t), loc in where.items():
+        for other in ((fg ^ 1, t), (fg, t - t % 4), (fg ^ 1, t - t % 4)):
+            if other in where:
+                assert where[other] == loc
