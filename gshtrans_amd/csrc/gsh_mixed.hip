@@ -473,9 +473,10 @@ __global__ __launch_bounds__(512) void leg_synth_bf16(LegParams p) {
 // The coefficients come in the parity layout (coef_split with sym: two
 // 32-degree blocks per 64-degree chunk, one per class).  Workgroup = 8 waves
 // for (128 northern colatitudes of block tsb) x (128 columns of super-group
-// csg) of one order m, walking the blocks (K steps of one class each) with
-// the B operands double-buffered per block and the bf16 planes of D
-// double-buffered per 64-degree chunk.  While the waves multiply chunk c's
+// csg) of one order m, walking the blocks (K steps of one class each), each
+// wave loading its B operands straight from the split coefficients one block
+// ahead, the bf16 planes of D double-buffered in LDS per 64-degree chunk
+// (one barrier per chunk).  While the waves multiply chunk c's
 // two blocks they produce chunk c + 1: wave w runs the recurrence of
 // colatitude block w % 2 (64 colatitudes) for the 16 degrees of quarter
 // w / 2 from the checkpoint there, one recurrence pair per row group, and
@@ -489,8 +490,6 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_s(LegParams p) {
   static_assert(kCk == 16, "four 16-degree quarters per 64-degree chunk");
   // Planes of a chunk: class k, entries 8 kr .. 8 kr + 7, 128 colatitudes.
   __shared__ uint4 trb[2][NS][2][4][128];
-  // B operands of a block (8 column tiles x NS planes x 64 lanes).
-  __shared__ uint4 bsh[2][8][NS][64];
   // Recurrence coefficients of a chunk (64 entries, 2 KiB).
   __shared__ uint4 rsc[2][128];
   const int lane = threadIdx.x & 63;
@@ -536,23 +535,18 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_s(LegParams p) {
   // for every c < nC.
   auto fetch_rec = [&](int c) { return recq[2 * (l0 + 64 * min(c, nC - 1) + 1) + (tid & 127)]; };
 
-  // B operand staging: piece j of this thread is uint4 tid + 512 j of the
-  // block's 8 contiguous tiles (ntile is a multiple of 8).
+  // B operands of block b (class b % 2 of chunk b / 2): each wave loads its
+  // two column tiles' planes straight from the split coefficients (16 bytes
+  // a lane, 1 KiB a wave instruction) one block ahead.
   const uint4* bbase = p.bs + (ld_uniform(p.cso, slot) * (long)p.ntile + csg * 8) * NS * 64;
   const int bstride = p.ntile * NS * 64;  // uint4 per block
-  struct Pieces {
-    uint4 v[NS];
-  };
-  auto fetch = [&](int b) {
-    const uint4* cb = bbase + (long)min(b, nb - 1) * bstride;
-    Pieces q;
+  const int toff = (wave & 3) * 2 * NS * 64 + lane;
+  auto fetchB = [&](int b, uint4 (&B)[2][NS]) {
+    const uint4* cb = bbase + (long)min(b, nb - 1) * bstride + toff;
 #pragma unroll
-    for (int j = 0; j < NS; j++) q.v[j] = cb[tid + 512 * j];
-    return q;
-  };
-  auto store = [&](int buf, const Pieces q) {
+    for (int t = 0; t < 2; t++)
 #pragma unroll
-    for (int j = 0; j < NS; j++) (&bsh[buf][0][0][0])[512 * j + tid] = q.v[j];
+      for (int s = 0; s < NS; s++) B[t][s] = cb[(t * NS + s) * 64];
   };
 
   // Consumer role.
@@ -568,10 +562,9 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_s(LegParams p) {
 
   // Production: pair i (0..7) of this wave's quarter is steps 2 i, 2 i + 1,
   // degrees d = 16 pq + 2 i (+1) of the chunk.  An even pair keeps its two
-  // values; the odd pair after it completes dword i / 2 of the even offsets
-  // (d, d + 2: class par0) and of the odd offsets (d + 1, d + 3).  The last
-  // pair stores both uint4 (entries 8 pq .. + 7 of each class) whole.
-  unsigned pl[NS][2][4];  // [plane][even / odd offsets][dword]
+  // values; the odd pair after it stores dword i / 2 of the even offsets
+  // (d, d + 2: class par0) and of the odd offsets (d + 1, d + 3) of entries
+  // 8 pq .. + 7 of each class.
   double hold0 = 0.0, hold1 = 0.0;
   auto produce_pair = [&](int rbuf, int tbuf, int i) {
     const d4* rl = reinterpret_cast<const d4*>(&rsc[rbuf][0]) + 16 * pq;
@@ -583,20 +576,13 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_s(LegParams p) {
       hold0 = v0;
       hold1 = v1;
     } else {
-      unsigned o[NS];
-      split_pair<NS>(hold0, v0, o);
+      unsigned o0[NS], o1[NS];
+      split_pair<NS>(hold0, v0, o0);
+      split_pair<NS>(hold1, v1, o1);
 #pragma unroll
-      for (int s = 0; s < NS; s++) pl[s][0][i >> 1] = o[s];
-      split_pair<NS>(hold1, v1, o);
-#pragma unroll
-      for (int s = 0; s < NS; s++) pl[s][1][i >> 1] = o[s];
-      if (i == 7) {
-#pragma unroll
-        for (int s = 0; s < NS; s++)
-#pragma unroll
-          for (int e = 0; e < 2; e++)
-            trb[tbuf][s][e ^ par0][pq][64 * pb + lane] =
-                make_uint4(pl[s][e][0], pl[s][e][1], pl[s][e][2], pl[s][e][3]);
+      for (int s = 0; s < NS; s++) {
+        reinterpret_cast<unsigned*>(&trb[tbuf][s][par0][pq][64 * pb + lane])[i >> 1] = o0[s];
+        reinterpret_cast<unsigned*>(&trb[tbuf][s][par0 ^ 1][pq][64 * pb + lane])[i >> 1] = o1[s];
       }
     }
   };
@@ -614,13 +600,6 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_s(LegParams p) {
       flush(acc[g][1][k], c[0][1]);
     }
   };
-  auto loadB = [&](int buf, uint4 (&B)[2][NS]) {
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-      B[0][s] = bsh[buf][(wave & 3) * 2][s][lane];
-      B[1][s] = bsh[buf][(wave & 3) * 2 + 1][s][lane];
-    }
-  };
 
   // Prologue: coefficients of chunks 0 and 1, B operands of block 0,
   // production of chunk 0.
@@ -628,7 +607,8 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_s(LegParams p) {
     rsc[0][tid] = fetch_rec(0);
     rsc[1][tid] = fetch_rec(1);
   }
-  store(0, fetch(0));
+  uint4 Bc[2][NS];
+  fetchB(0, Bc);
   __syncthreads();
   const bool pv = 64 * pb < rows;  // this wave's producer block holds colatitudes
   if (pv) {
@@ -637,17 +617,18 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_s(LegParams p) {
   }
   __syncthreads();
 
-  // Blocks 0 .. nb-2 with the production of the next chunk during a chunk's
-  // two blocks.  The state of chunk c + 1 (checkpoint 4 (c + 1) + pq) is
-  // loaded one chunk before its use.
-  // Block 2 c + k of chunk c (k a constant at each call): B operands of the
-  // next block fetched, production pairs 4 k .. 4 k + 3 of chunk c + 1.
+  // Block 2 c + k of chunk c (k a constant at each call): the B operands of
+  // the next block fetched into registers, production pairs 4 k .. 4 k + 3
+  // of chunk c + 1 (none in the last chunk).  One barrier per chunk, after
+  // its class-1 block.  The state of chunk c + 1 (checkpoint 4 (c + 1) + pq)
+  // is loaded one chunk before its use.
   double2 nx = ck_state(4 + pq);
-  auto block = [&](int c, const int k) {
+  auto block = [&](int c, const int k, const bool next) {
     const int b = 2 * c + k;
     const int abuf = c & 1;
     const bool prod = pv && c + 1 < nC;
-    const Pieces q = fetch(b + 1);
+    uint4 Bn[2][NS];
+    if (next) fetchB(b + 1, Bn);
     uint4 qr = make_uint4(0u, 0u, 0u, 0u);
     if (k == 0) {
       pe0 = nx.x;
@@ -656,41 +637,33 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_s(LegParams p) {
     } else if (tid < 128) {
       qr = fetch_rec(c + 2);
     }
-    uint4 B[2][NS];
-    loadB(b & 1, B);
     uint4 Ab[2][1][NS];
     loadA(abuf, k, 0, Ab[0]);
 #pragma unroll
     for (int g = 0; g < 4; g++) {
       if (g < 3) loadA(abuf, k, g + 1, Ab[(g + 1) & 1]);
       if (prod) produce_pair(abuf ^ 1, abuf ^ 1, 4 * k + g);
-      consume_group(g, k, Ab[g & 1], B);
+      consume_group(g, k, Ab[g & 1], Bc);
       __builtin_amdgcn_sched_barrier(0);
     }
-    store((b + 1) & 1, q);  // buffers last read in block b - 1
-    // Chunk c + 2's coefficients (rsc[c & 1] was last read in chunk c - 1).
-    if (k == 1 && tid < 128) rsc[abuf][tid] = qr;
-    __syncthreads();
+    if (next) {
+#pragma unroll
+      for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int s = 0; s < NS; s++) Bc[t][s] = Bn[t][s];
+    }
+    if (k == 1) {
+      // Chunk c + 2's coefficients (rsc[c & 1] was last read in chunk c - 1).
+      if (tid < 128) rsc[abuf][tid] = qr;
+      __syncthreads();
+    }
   };
   for (int c = 0; c + 1 < nC; c++) {
-    block(c, 0);
-    block(c, 1);
+    block(c, 0, true);
+    block(c, 1, true);
   }
-  block(nC - 1, 0);
-  {
-    // The last block (class 1 of chunk nC - 1).
-    const int b = nb - 1;
-    const int abuf = (nC - 1) & 1;
-    uint4 B[2][NS];
-    loadB(b & 1, B);
-    uint4 Ab[2][1][NS];
-    loadA(abuf, 1, 0, Ab[0]);
-#pragma unroll
-    for (int g = 0; g < 4; g++) {
-      if (g < 3) loadA(abuf, 1, g + 1, Ab[(g + 1) & 1]);
-      consume_group(g, 1, Ab[g & 1], B);
-    }
-  }
+  block(nC - 1, 0, true);
+  block(nC - 1, 1, false);
 
   // F(t) = E + O at the northern t, F(nt-1-t) = E - O at its mirror (C
   // layout of the 16x16x32 MFMA: lane holds rows 4 (lane / 16) + r, column
@@ -736,7 +709,9 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_s(LegParams p) {
 // recurrence work and coefficient reads per MFMA.  The degrees go in
 // 64-degree chunks (bf16 planes of D double-buffered in LDS), each taken as
 // four K blocks of 32 -- (degrees 0-31, m), (0-31, -m), (32-63, m),
-// (32-63, -m) -- with the B operands double-buffered per block.  While the
+// (32-63, -m) -- each wave loading its B operands straight from the split
+// coefficients one block ahead (no LDS staging, one barrier per chunk:
+// 1.86 -> 1.77 ms at lMax=1024, B=64).  While the
 // waves multiply chunk c they produce chunk c + 1: wave w runs the
 // recurrence of colatitude block w % 2 (64 colatitudes) for the 16 degrees
 // of quarter w / 2 from the checkpoint there, two recurrence pairs per K
@@ -749,8 +724,6 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_pm(LegParams p) {
   static_assert(kCk == 16, "four 16-degree quarters per 64-degree chunk");
   // Planes of a chunk: degree half dh, entries 8 kr .. 8 kr + 7, 128 colatitudes.
   __shared__ uint4 trb[2][NS][2][4][128];
-  // B operands of a block (8 column tiles x NS planes x 64 lanes).
-  __shared__ uint4 bsh[2][8][NS][64];
   // Recurrence coefficients of a chunk (64 entries, 2 KiB).
   __shared__ uint4 rsc[2][128];
   const int lane = threadIdx.x & 63;
@@ -789,25 +762,20 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_pm(LegParams p) {
   const uint4* recq = reinterpret_cast<const uint4*>(p.rec + (ld_uniform(p.roff, m + p.L) - l0));
   auto fetch_rec = [&](int c) { return recq[2 * (l0 + 64 * min(c, nC - 1) + 1) + (tid & 127)]; };
 
-  // B operand staging of block b = 4 c + 2 dh + o: coefficient block 2 c + dh
-  // (clamped to the last) of order m (o = 0) or -m (o = 1); piece j of this
-  // thread is uint4 tid + 512 j of the block's 8 contiguous tiles.
+  // B operands of block b = 4 c + 2 dh + o: coefficient block 2 c + dh
+  // (clamped to the last) of order m (o = 0) or -m (o = 1); each wave loads
+  // its two column tiles' planes straight from the split buffer (16 bytes a
+  // lane, 1 KiB a wave instruction) one block ahead.
   const uint4* bbp = p.bs + (ld_uniform(p.cso, slot) * (long)p.ntile + csg * 8) * NS * 64;
   const uint4* bbn = p.bs + (ld_uniform(p.cso, slotn) * (long)p.ntile + csg * 8) * NS * 64;
   const int bstride = p.ntile * NS * 64;  // uint4 per coefficient block
-  struct Pieces {
-    uint4 v[NS];
-  };
-  auto fetch = [&](int b) {
-    const uint4* cb = ((b & 1) ? bbn : bbp) + (long)min(b >> 1, n32 - 1) * bstride;
-    Pieces q;
+  const int toff = (wave & 3) * 2 * NS * 64 + lane;
+  auto fetchB = [&](int b, uint4 (&B)[2][NS]) {
+    const uint4* cb = ((b & 1) ? bbn : bbp) + (long)min(b >> 1, n32 - 1) * bstride + toff;
 #pragma unroll
-    for (int j = 0; j < NS; j++) q.v[j] = cb[tid + 512 * j];
-    return q;
-  };
-  auto store = [&](int buf, const Pieces q) {
+    for (int t = 0; t < 2; t++)
 #pragma unroll
-    for (int j = 0; j < NS; j++) (&bsh[buf][0][0][0])[512 * j + tid] = q.v[j];
+      for (int s = 0; s < NS; s++) B[t][s] = cb[(t * NS + s) * 64];
   };
 
   // Consumer role.
@@ -825,7 +793,6 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_pm(LegParams p) {
   // degrees 16 pq + 2 i (+1) of the chunk; every 4 pairs fill the uint4 of
   // each plane holding entries 8 kr .. 8 kr + 7 of degree half pq / 2 (kr =
   // 2 (pq % 2) + i / 4), stored whole.
-  unsigned pl[NS][4];
   auto produce_pair = [&](int rbuf, int tbuf, int i) {
     const d4* rl = reinterpret_cast<const d4*>(&rsc[rbuf][0]) + 16 * pq;
     const double v0 = pe0;
@@ -835,13 +802,9 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_pm(LegParams p) {
     unsigned o[NS];
     split_pair<NS>(v0, v1, o);
 #pragma unroll
-    for (int s = 0; s < NS; s++) pl[s][i & 3] = o[s];
-    if ((i & 3) == 3) {
-#pragma unroll
-      for (int s = 0; s < NS; s++)
-        trb[tbuf][s][pq >> 1][2 * (pq & 1) + (i >> 2)][64 * pb + lane] =
-            make_uint4(pl[s][0], pl[s][1], pl[s][2], pl[s][3]);
-    }
+    for (int s = 0; s < NS; s++)
+      reinterpret_cast<unsigned*>(
+          &trb[tbuf][s][pq >> 1][2 * (pq & 1) + (i >> 2)][64 * pb + lane])[i & 3] = o[s];
   };
   auto loadA = [&](int abuf, int dh, int g, uint4 (&A)[1][NS]) {
 #pragma unroll
@@ -855,13 +818,6 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_pm(LegParams p) {
       flush(acc[g][1][o], c[0][1]);
     }
   };
-  auto loadB = [&](int buf, uint4 (&B)[2][NS]) {
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-      B[0][s] = bsh[buf][(wave & 3) * 2][s][lane];
-      B[1][s] = bsh[buf][(wave & 3) * 2 + 1][s][lane];
-    }
-  };
 
   // Prologue: coefficients of chunks 0 and 1, B operands of block 0,
   // production of chunk 0.
@@ -869,7 +825,8 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_pm(LegParams p) {
     rsc[0][tid] = fetch_rec(0);
     rsc[1][tid] = fetch_rec(1);
   }
-  store(0, fetch(0));
+  uint4 Bc[2][NS];
+  fetchB(0, Bc);
   __syncthreads();
   const bool pv = 64 * pb < rows;  // this wave's producer block holds colatitudes
   if (pv) {
@@ -879,8 +836,10 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_pm(LegParams p) {
   __syncthreads();
 
   // Block 4 c + k of chunk c (k a constant at each call): the B operands of
-  // the next block fetched, production pairs 2 k, 2 k + 1 of chunk c + 1
-  // (none in the last chunk).  The state of chunk c + 1 (checkpoint
+  // the next block fetched into registers, production pairs 2 k, 2 k + 1 of
+  // chunk c + 1 (none in the last chunk).  One barrier per chunk: the planes
+  // of chunk c + 1 and the coefficients of chunk c + 2 are complete, and
+  // chunk c's planes free, after it.  The state of chunk c + 1 (checkpoint
   // 4 (c + 1) + pq) is loaded one chunk before its use.
   double2 nx = ck_state(4 + pq);
   auto block = [&](int c, const int k, const bool next) {
@@ -889,8 +848,8 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_pm(LegParams p) {
     const int dh = k >> 1, o = k & 1;
     const bool prod = pv && c + 1 < nC;
     const bool live = 2 * c + dh < n32 && (o == 0 || m > 0);
-    Pieces q;
-    if (next) q = fetch(b + 1);
+    uint4 Bn[2][NS];
+    if (next) fetchB(b + 1, Bn);
     uint4 qr = make_uint4(0u, 0u, 0u, 0u);
     if (k == 0) {
       pe0 = nx.x;
@@ -900,25 +859,30 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_pm(LegParams p) {
       qr = fetch_rec(c + 2);
     }
     if (live) {
-      uint4 B[2][NS];
-      loadB(b & 1, B);
       uint4 Ab[2][1][NS];
       loadA(abuf, dh, 0, Ab[0]);
 #pragma unroll
       for (int g = 0; g < 4; g++) {
         if (g < 3) loadA(abuf, dh, g + 1, Ab[(g + 1) & 1]);
         if (prod && (g & 1)) produce_pair(abuf ^ 1, abuf ^ 1, 2 * k + (g >> 1));
-        consume_group(g, o, Ab[g & 1], B);
+        consume_group(g, o, Ab[g & 1], Bc);
         __builtin_amdgcn_sched_barrier(0);
       }
     } else if (prod) {
       produce_pair(abuf ^ 1, abuf ^ 1, 2 * k);
       produce_pair(abuf ^ 1, abuf ^ 1, 2 * k + 1);
     }
-    if (next) store((b + 1) & 1, q);  // buffers last read in block b - 1
-    // Chunk c + 2's coefficients (rsc[c & 1] was last read in chunk c - 1).
-    if (k == 3 && tid < 128) rsc[abuf][tid] = qr;
-    __syncthreads();
+    if (next) {
+#pragma unroll
+      for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int s = 0; s < NS; s++) Bc[t][s] = Bn[t][s];
+    }
+    if (k == 3) {
+      // Chunk c + 2's coefficients (rsc[c & 1] was last read in chunk c - 1).
+      if (tid < 128) rsc[abuf][tid] = qr;
+      __syncthreads();
+    }
   };
   for (int c = 0; c + 1 < nC; c++) {
     block(c, 0, true);
