@@ -149,12 +149,13 @@ def run_gpu(args):
                       "kernels": kt}
         log(f"[bench] {m}: {tps:.1f} transforms/s, {1e3 * dt / args.steps:.2f} ms/step")
 
-    # --two-streams: the real and complex batches are independent jobs; the
-    # same step with each batch on its own HIP stream (default priority) and
-    # grid handle (its own work buffers), so one batch's HBM-bound ring FFTs
-    # overlap the other's Legendre kernels.  Reported beside the sequential
-    # figures, which carry the per-kernel timings (off by default so that a
-    # profile of the default run holds one-stream kernel durations only).
+    # Two streams (unless --no-two-streams): the real and complex batches are
+    # independent jobs; the same step with each batch on its own HIP stream
+    # (default priority) and grid handle (its own work buffers), so one
+    # batch's HBM-bound ring FFTs overlap the other's Legendre kernels.
+    # Reported beside the one-stream figures, which are value and carry the
+    # per-kernel timings (profiles run with --no-two-streams, so that their
+    # kernel durations are one-stream ones too).
     if args.two_streams:
         gc = G.GaussLegendreGrid(L, NMAX)
         gc.prepare(2)
@@ -517,9 +518,10 @@ def main():
     ap.add_argument("--no-batching", action="store_true",
                     help="skip the batching workload (plan graph replay against a loop of "
                          "per-field calls)")
-    ap.add_argument("--two-streams", action="store_true",
-                    help="also time FP64 and the headline mode with the real and complex "
-                         "batches on two HIP streams")
+    ap.add_argument("--no-two-streams", dest="two_streams", action="store_false",
+                    help="skip the extra timing of FP64 and the headline mode with the real "
+                         "and complex batches on two HIP streams (reported beside value, "
+                         "never as value)")
     ap.add_argument("--config-rows", action="store_true",
                     help="only time the CPU-baseline rows of the other workloads (lMax=64, "
                          "lMax=512 for n=-2..2, the lMax=512 adjoint) beside the FP64 GPU "
