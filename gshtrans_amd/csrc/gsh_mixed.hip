@@ -1503,25 +1503,33 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_s(LegParams p) {
   const float* ft = p.FA32 + ((long)slot * p.n_cg + min(tile, p.n_cg - 1)) * p.nquad * 64 + (lane & 15);
   const float* fb = ft + 2 * (lane >> 4) * 64;
   // Chunk kc's colatitudes t = 32 kc + 8 (lane / 16) + e (X) and their
-  // mirrors nt - 1 - t (Y; a mirror below 0, past the north, reads t = 0).
+  // mirrors nt - 1 - t (Y), 16 floats apart per row within the (slot, column
+  // group) block.  A mirror row below 0 (t past the north, the last chunk
+  // only, at most 31 rows) reads the previous block or the front pad of FA;
+  // combine zeroes it.
   auto loadB = [&](int kc, RawB& X, RawB& Y) {
+    const float* fm = ft + 16 * (nt - 1 - 32 * kc - 8 * (lane >> 4));
 #pragma unroll
     for (int e = 0; e < 8; e++) {
       X.x[e] = fb[(long)kc * 512 + 16 * e];
-      const int tm = max(nt - 1 - (32 * kc + 8 * (lane >> 4) + e), 0);
-      Y.x[e] = ft[(long)(tm >> 2) * 64 + (tm & 3) * 16];
+      Y.x[e] = fm[-16 * e];
     }
   };
   // B operands of a chunk: FA(t) + FA(nt - 1 - t) (sign +1) or
   // FA(t) - FA(nt - 1 - t) (sign -1) for northern t below the equator, FA(t)
-  // (+1) and 0 (-1) at the equator, 0 past the north.
-  auto combine = [&](const RawB& X, const RawB& Y, int kc, float sign, RawB& Z) {
+  // (+1) and 0 (-1) at the equator, 0 past the north.  Only the chunks of
+  // the last group (`last`) hold the equator or rows past the north.
+  auto combine = [&](const RawB& X, const RawB& Y, int kc, float sign, RawB& Z, bool last) {
     const int th = 32 * kc + 8 * (lane >> 4);
 #pragma unroll
     for (int e = 0; e < 8; e++) {
-      const int t = th + e;
-      const float y = 2 * t == nt - 1 ? (sign > 0.f ? 0.0f : X.x[e]) : Y.x[e];
-      Z.x[e] = t < nts ? X.x[e] + sign * y : 0.0f;
+      if (!last) {
+        Z.x[e] = X.x[e] + sign * Y.x[e];
+      } else {
+        const int t = th + e;
+        const float y = 2 * t == nt - 1 ? (sign > 0.f ? 0.0f : X.x[e]) : Y.x[e];
+        Z.x[e] = t < nts ? X.x[e] + sign * y : 0.0f;
+      }
     }
   };
   auto splitB = [&](const RawB& X, uint4 (&B)[1][NS], int kc, bool last) {
@@ -1552,12 +1560,12 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_s(LegParams p) {
   f4 cp[4][1];
 #pragma unroll
   for (int i = 0; i < 4; i++) cp[i][0] = f4{0.f, 0.f, 0.f, 0.f};
-  auto step = [&](int buf, int kc, const RawB& X, const RawB& Y) {
+  auto step = [&](int buf, int kc, const RawB& X, const RawB& Y, bool last) {
     f4 c[4][1];
 #pragma unroll
     for (int h = 0; h < 2; h++) {  // parity class h: tiles 2 h, 2 h + 1
       RawB Z;
-      combine(X, Y, kc, h == 0 ? 1.0f : -1.0f, Z);
+      combine(X, Y, kc, h == 0 ? 1.0f : -1.0f, Z, last);
       uint4 B[1][NS];
       splitB(Z, B, kc, false);
       uint4 A[2][NS];
@@ -1606,7 +1614,7 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_s(LegParams p) {
     for (int j = 0; j < 4; j++) {
       loadB(min(kc0 + j + kAhead, kcl), X[(j + kAhead) & 3], Y[(j + kAhead) & 3]);
       produce4(buf ^ 1, j);
-      step(buf, kc0 + j, X[j], Y[j]);
+      step(buf, kc0 + j, X[j], Y[j], false);
       __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
@@ -1615,14 +1623,14 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_s(LegParams p) {
     // Last group: chunks 4 (ngrp-1) .. nkc-1.
     const int buf = (ngrp - 1) & 1;
     const int kc0 = 4 * (ngrp - 1);
-    step(buf, kc0, X[0], Y[0]);
+    step(buf, kc0, X[0], Y[0], true);
     if (kc0 + 1 < nkc) {
-      step(buf, kc0 + 1, X[1], Y[1]);
+      step(buf, kc0 + 1, X[1], Y[1], true);
       if (kc0 + 2 < nkc) {
-        step(buf, kc0 + 2, X[2], Y[2]);
+        step(buf, kc0 + 2, X[2], Y[2], true);
         if (kc0 + 3 < nkc) {
           loadB(kc0 + 3, X[3], Y[3]);
-          step(buf, kc0 + 3, X[3], Y[3]);
+          step(buf, kc0 + 3, X[3], Y[3], true);
         }
       }
     }
@@ -1946,25 +1954,33 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_ws(LegParams p) {
   const float* ft = p.FA32 + ((long)slot * p.n_cg + min(tile, p.n_cg - 1)) * p.nquad * 64 + (lane & 15);
   const float* fb = ft + 2 * (lane >> 4) * 64;
   // Chunk kc's colatitudes t = 32 kc + 8 (lane / 16) + e (X) and their
-  // mirrors nt - 1 - t (Y; a mirror below 0, past the north, reads t = 0).
+  // mirrors nt - 1 - t (Y), 16 floats apart per row within the (slot, column
+  // group) block.  A mirror row below 0 (t past the north, the last chunk
+  // only, at most 31 rows) reads the previous block or the front pad of FA;
+  // combine zeroes it.
   auto loadB = [&](int kc, RawB& X, RawB& Y) {
+    const float* fm = ft + 16 * (nt - 1 - 32 * kc - 8 * (lane >> 4));
 #pragma unroll
     for (int e = 0; e < 8; e++) {
       X.x[e] = fb[(long)kc * 512 + 16 * e];
-      const int tm = max(nt - 1 - (32 * kc + 8 * (lane >> 4) + e), 0);
-      Y.x[e] = ft[(long)(tm >> 2) * 64 + (tm & 3) * 16];
+      Y.x[e] = fm[-16 * e];
     }
   };
   // B operands of a chunk: FA(t) + FA(nt - 1 - t) (sign +1) or
   // FA(t) - FA(nt - 1 - t) (sign -1) for northern t below the equator, FA(t)
-  // (+1) and 0 (-1) at the equator, 0 past the north.
-  auto combine = [&](const RawB& X, const RawB& Y, int kc, float sign, RawB& Z) {
+  // (+1) and 0 (-1) at the equator, 0 past the north.  Only the chunks of
+  // the last group (`last`) hold the equator or rows past the north.
+  auto combine = [&](const RawB& X, const RawB& Y, int kc, float sign, RawB& Z, bool last) {
     const int th = 32 * kc + 8 * (lane >> 4);
 #pragma unroll
     for (int e = 0; e < 8; e++) {
-      const int t = th + e;
-      const float y = 2 * t == nt - 1 ? (sign > 0.f ? 0.0f : X.x[e]) : Y.x[e];
-      Z.x[e] = t < nts ? X.x[e] + sign * y : 0.0f;
+      if (!last) {
+        Z.x[e] = X.x[e] + sign * Y.x[e];
+      } else {
+        const int t = th + e;
+        const float y = 2 * t == nt - 1 ? (sign > 0.f ? 0.0f : X.x[e]) : Y.x[e];
+        Z.x[e] = t < nts ? X.x[e] + sign * y : 0.0f;
+      }
     }
   };
   auto splitB = [&](const RawB& X, uint4 (&B)[1][NS], int kc, bool last) {
@@ -1995,13 +2011,13 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_ws(LegParams p) {
   f4 cp[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) cp[i] = f4{0.f, 0.f, 0.f, 0.f};
-  auto step = [&](int buf, int kc, const RawB& X, const RawB& Y) {
+  auto step = [&](int buf, int kc, const RawB& X, const RawB& Y, bool last) {
 #pragma unroll
     for (int half = 0; half < 2; half++) {  // parity class half: tiles 4 half .. + 3
       uint4 B[1][NS];
       {
         RawB Z;
-        combine(X, Y, kc, half == 0 ? 1.0f : -1.0f, Z);
+        combine(X, Y, kc, half == 0 ? 1.0f : -1.0f, Z, last);
         splitB(Z, B, kc, false);
       }
       uint4 A[4][NS];
@@ -2041,7 +2057,7 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_ws(LegParams p) {
     for (int j = 0; j < 2; j++) {
       loadB(min(kc0 + j + 1, kcl), X[(j + 1) & 1], Y[(j + 1) & 1]);
       produce8(buf ^ 1, j);
-      step(buf, kc0 + j, X[j], Y[j]);
+      step(buf, kc0 + j, X[j], Y[j], false);
       __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
@@ -2051,8 +2067,8 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_ws(LegParams p) {
     const int buf = (ngrp - 1) & 1;
     const int kc0 = 2 * (ngrp - 1);
     if (kc0 + 1 < nkc) loadB(kc0 + 1, X[1], Y[1]);
-    step(buf, kc0, X[0], Y[0]);
-    if (kc0 + 1 < nkc) step(buf, kc0 + 1, X[1], Y[1]);
+    step(buf, kc0, X[0], Y[0], true);
+    if (kc0 + 1 < nkc) step(buf, kc0 + 1, X[1], Y[1], true);
 #pragma unroll
     for (int i = 0; i < 8; i++) flush(acc[i][0], cp[i]);
   }
