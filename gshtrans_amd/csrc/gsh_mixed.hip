@@ -708,16 +708,15 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_s(LegParams p) {
 // per recurrence value as leg_synth_bf16 over twice the columns, so half the
 // recurrence work and coefficient reads per MFMA.  The degrees go in
 // 64-degree chunks (bf16 planes of D double-buffered in LDS), each taken as
-// four K blocks of 32 -- (degrees 0-31, m), (0-31, -m), (32-63, m),
-// (32-63, -m) -- each wave loading its B operands straight from the split
-// coefficients one block ahead (no LDS staging, one barrier per chunk:
-// 1.86 -> 1.77 ms at lMax=1024, B=64).  While the
-// waves multiply chunk c they produce chunk c + 1: wave w runs the
-// recurrence of colatitude block w % 2 (64 colatitudes) for the 16 degrees
-// of quarter w / 2 from the checkpoint there, two recurrence pairs per K
-// block.  Wave w multiplies colatitude half h = w / 4 (4 row groups) with
-// column tiles 2 (w % 4), 2 (w % 4) + 1 of both orders.  Blocks past the
-// order's last 32-degree chunk, and order -0, are skipped.
+// two K blocks of 32 degrees.  Wave w multiplies column tile w of both
+// orders over all 128 colatitudes (8 row groups; each A operand feeds the
+// two orders' MFMA chains), loading its B operands straight from the split
+// coefficients one block ahead (no LDS staging, one barrier per chunk).
+// While the waves multiply chunk c they produce chunk c + 1: wave w runs
+// the recurrence of colatitude block w % 2 (64 colatitudes) for the 16
+// degrees of quarter w / 2 from the checkpoint there, four recurrence pairs
+// per K block.  Blocks past the order's last 32-degree chunk are skipped;
+// order -0's sums are not stored.
 // ---------------------------------------------------------------------------
 template <int NS>
 __global__ __launch_bounds__(512) void leg_synth_bf16_pm(LegParams p) {
@@ -769,25 +768,25 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_pm(LegParams p) {
   const uint4* bbp = p.bs + (ld_uniform(p.cso, slot) * (long)p.ntile + csg * 8) * NS * 64;
   const uint4* bbn = p.bs + (ld_uniform(p.cso, slotn) * (long)p.ntile + csg * 8) * NS * 64;
   const int bstride = p.ntile * NS * 64;  // uint4 per coefficient block
-  const int toff = (wave & 3) * 2 * NS * 64 + lane;
-  auto fetchB = [&](int b, uint4 (&B)[2][NS]) {
-    const uint4* cb = ((b & 1) ? bbn : bbp) + (long)min(b >> 1, n32 - 1) * bstride + toff;
+  const int toff = wave * NS * 64 + lane;  // column tile csg * 8 + wave
+  // B operands of degree half dh of chunk c for both orders: B[o] of order m
+  // (o = 0) or -m (o = 1).
+  auto fetchB = [&](int c, int dh, uint4 (&B)[2][NS]) {
+    const long off = (long)min(2 * c + dh, n32 - 1) * bstride + toff;
 #pragma unroll
-    for (int t = 0; t < 2; t++)
-#pragma unroll
-      for (int s = 0; s < NS; s++) B[t][s] = cb[(t * NS + s) * 64];
+    for (int s = 0; s < NS; s++) {
+      B[0][s] = bbp[off + s * 64];
+      B[1][s] = bbn[off + s * 64];
+    }
   };
 
-  // Consumer role.
-  const int h = wave >> 2;
-  const int tile0 = csg * 8 + (wave & 3) * 2;
-  d4 acc[4][2][2];  // row group, column tile, order
+  // Consumer role: column tile w of both orders, all 128 colatitudes.
+  const int tile = csg * 8 + wave;
+  d4 acc[8][2];  // row group, order
 #pragma unroll
-  for (int g = 0; g < 4; g++)
+  for (int g = 0; g < 8; g++)
 #pragma unroll
-    for (int t = 0; t < 2; t++)
-#pragma unroll
-      for (int o = 0; o < 2; o++) acc[g][t][o] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int o = 0; o < 2; o++) acc[g][o] = d4{0.0, 0.0, 0.0, 0.0};
 
   // Production: pair i (0..7) of this wave's quarter is steps 2 i, 2 i + 1,
   // degrees 16 pq + 2 i (+1) of the chunk; every 4 pairs fill the uint4 of
@@ -808,14 +807,14 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_pm(LegParams p) {
   };
   auto loadA = [&](int abuf, int dh, int g, uint4 (&A)[1][NS]) {
 #pragma unroll
-    for (int s = 0; s < NS; s++) A[0][s] = trb[abuf][s][dh][lane >> 4][h * 64 + g * 16 + (lane & 15)];
+    for (int s = 0; s < NS; s++) A[0][s] = trb[abuf][s][dh][lane >> 4][g * 16 + (lane & 15)];
   };
-  auto consume_group = [&](int g, int o, const uint4 (&A)[1][NS], const uint4 (&B)[2][NS]) {
-    if (h * 64 + g * 16 < rows) {
+  auto consume_group = [&](int g, const uint4 (&A)[1][NS], const uint4 (&B)[2][NS]) {
+    if (g * 16 < rows) {
       f4 c[1][2];
       split_mma<NS, 1>(A, B, c);
-      flush(acc[g][0][o], c[0][0]);
-      flush(acc[g][1][o], c[0][1]);
+      flush(acc[g][0], c[0][0]);
+      flush(acc[g][1], c[0][1]);
     }
   };
 
@@ -826,7 +825,7 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_pm(LegParams p) {
     rsc[1][tid] = fetch_rec(1);
   }
   uint4 Bc[2][NS];
-  fetchB(0, Bc);
+  fetchB(0, 0, Bc);
   __syncthreads();
   const bool pv = 64 * pb < rows;  // this wave's producer block holds colatitudes
   if (pv) {
@@ -835,50 +834,49 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_pm(LegParams p) {
   }
   __syncthreads();
 
-  // Block 4 c + k of chunk c (k a constant at each call): the B operands of
-  // the next block fetched into registers, production pairs 2 k, 2 k + 1 of
-  // chunk c + 1 (none in the last chunk).  One barrier per chunk: the planes
-  // of chunk c + 1 and the coefficients of chunk c + 2 are complete, and
-  // chunk c's planes free, after it.  The state of chunk c + 1 (checkpoint
-  // 4 (c + 1) + pq) is loaded one chunk before its use.
+  // Block 2 c + dh of chunk c (dh a constant at each call): degree half dh
+  // for both orders, the B operands of the next block fetched into
+  // registers, production pairs 4 dh .. 4 dh + 3 of chunk c + 1 (none in the
+  // last chunk).  One barrier per chunk: the planes of chunk c + 1 and the
+  // coefficients of chunk c + 2 are complete, and chunk c's planes free,
+  // after it.  The state of chunk c + 1 (checkpoint 4 (c + 1) + pq) is
+  // loaded one chunk before its use.
   double2 nx = ck_state(4 + pq);
-  auto block = [&](int c, const int k, const bool next) {
-    const int b = 4 * c + k;
+  auto block = [&](int c, const int dh, const bool next) {
     const int abuf = c & 1;
-    const int dh = k >> 1, o = k & 1;
     const bool prod = pv && c + 1 < nC;
-    const bool live = 2 * c + dh < n32 && (o == 0 || m > 0);
+    const bool live = 2 * c + dh < n32;
     uint4 Bn[2][NS];
-    if (next) fetchB(b + 1, Bn);
+    if (next) fetchB(c + dh, dh ^ 1, Bn);
     uint4 qr = make_uint4(0u, 0u, 0u, 0u);
-    if (k == 0) {
+    if (dh == 0) {
       pe0 = nx.x;
       pe1 = nx.y;
       nx = ck_state(4 * (c + 2) + pq);
-    } else if (k == 3 && tid < 128) {
+    } else if (tid < 128) {
       qr = fetch_rec(c + 2);
     }
     if (live) {
       uint4 Ab[2][1][NS];
       loadA(abuf, dh, 0, Ab[0]);
 #pragma unroll
-      for (int g = 0; g < 4; g++) {
-        if (g < 3) loadA(abuf, dh, g + 1, Ab[(g + 1) & 1]);
-        if (prod && (g & 1)) produce_pair(abuf ^ 1, abuf ^ 1, 2 * k + (g >> 1));
-        consume_group(g, o, Ab[g & 1], Bc);
+      for (int g = 0; g < 8; g++) {
+        if (g < 7) loadA(abuf, dh, g + 1, Ab[(g + 1) & 1]);
+        if (prod && (g & 1)) produce_pair(abuf ^ 1, abuf ^ 1, 4 * dh + (g >> 1));
+        consume_group(g, Ab[g & 1], Bc);
         __builtin_amdgcn_sched_barrier(0);
       }
     } else if (prod) {
-      produce_pair(abuf ^ 1, abuf ^ 1, 2 * k);
-      produce_pair(abuf ^ 1, abuf ^ 1, 2 * k + 1);
+#pragma unroll
+      for (int i = 0; i < 4; i++) produce_pair(abuf ^ 1, abuf ^ 1, 4 * dh + i);
     }
     if (next) {
 #pragma unroll
-      for (int t = 0; t < 2; t++)
+      for (int o = 0; o < 2; o++)
 #pragma unroll
-        for (int s = 0; s < NS; s++) Bc[t][s] = Bn[t][s];
+        for (int s = 0; s < NS; s++) Bc[o][s] = Bn[o][s];
     }
-    if (k == 3) {
+    if (dh == 1) {
       // Chunk c + 2's coefficients (rsc[c & 1] was last read in chunk c - 1).
       if (tid < 128) rsc[abuf][tid] = qr;
       __syncthreads();
@@ -887,42 +885,35 @@ __global__ __launch_bounds__(512) void leg_synth_bf16_pm(LegParams p) {
   for (int c = 0; c + 1 < nC; c++) {
     block(c, 0, true);
     block(c, 1, true);
-    block(c, 2, true);
-    block(c, 3, true);
   }
   block(nC - 1, 0, true);
-  block(nC - 1, 1, true);
-  block(nC - 1, 2, true);
-  block(nC - 1, 3, false);
+  block(nC - 1, 1, false);
 
   // Order m at rows t, order -m at rows nt-1-t (C layout of the 16x16x32
   // MFMA: lane holds rows 4 (lane / 16) + r, column lane % 16).
   const int ncg16 = (p.ncol + 15) >> 4;
   const long rs = (long)ncg16 * p.nslot * 16;
-  const int row0 = tsb * 128 + h * 64 + 4 * (lane >> 4);
+  const int row0 = tsb * 128 + 4 * (lane >> 4);
+  const int col = tile * 16 + (lane & 15);
+  if (col >= p.ncol) return;
+  const long fi = f_index(row0, slot, col, p.nslot, ncg16);
+  const long fm = f_index(nt - 1 - row0, slotn, col, p.nslot, ncg16);
 #pragma unroll
-  for (int t = 0; t < 2; t++) {
-    const int col = (tile0 + t) * 16 + (lane & 15);
-    if (col >= p.ncol) continue;
-    const long fi = f_index(row0, slot, col, p.nslot, ncg16);
-    const long fm = f_index(nt - 1 - row0, slotn, col, p.nslot, ncg16);
+  for (int g = 0; g < 8; g++)
 #pragma unroll
-    for (int g = 0; g < 4; g++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int row = row0 + g * 16 + r;
-        if (row >= nt) continue;
-        const long dn = (long)(g * 16 + r) * rs;  // rows past row0 / before nt-1-row0
-        const double a = acc[g][t][0][r], an = acc[g][t][1][r];
-        if (p.f32out) {
-          reinterpret_cast<float*>(p.F)[fi + dn] = (float)a;
-          if (m > 0) reinterpret_cast<float*>(p.F)[fm - dn] = (float)an;
-        } else {
-          p.F[fi + dn] = a;
-          if (m > 0) p.F[fm - dn] = an;
-        }
+    for (int r = 0; r < 4; r++) {
+      const int row = row0 + g * 16 + r;
+      if (row >= nt) continue;
+      const long dn = (long)(g * 16 + r) * rs;  // rows past row0 / before nt-1-row0
+      const double a = acc[g][0][r], an = acc[g][1][r];
+      if (p.f32out) {
+        reinterpret_cast<float*>(p.F)[fi + dn] = (float)a;
+        if (m > 0) reinterpret_cast<float*>(p.F)[fm - dn] = (float)an;
+      } else {
+        p.F[fi + dn] = a;
+        if (m > 0) p.F[fm - dn] = an;
       }
-  }
+    }
 }
 
 // ---------------------------------------------------------------------------
