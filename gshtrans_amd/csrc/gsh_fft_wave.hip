@@ -647,6 +647,13 @@ __global__ __launch_bounds__(64 * anal_waves(R, REAL), 2) void fft_anal_w(FFTPar
   const int cf = (fg * W + c) * (REAL ? 2 : 1);
   const bool cv = cf < p.nfield, cv1 = REAL && cf + 1 < p.nfield;
   if (!REAL) {
+    // Bin k goes to order k (k <= lmax) or k - N (k >= N - lmax), at
+    // fa_index(slot, 2 cf, theta) = slot * (n_cg nquad 64) + that of slot 0;
+    // bin N/2 (k = N/2, lane bq = 0 of round RH, piece IH) goes to both when
+    // lmax = N/2.  One store per bin, one branch.
+    constexpr int RH = (N / 2) / LPW, IH = ((N / 2) % LPW) / 64;
+    const long sst = (long)n_cg * nquad * 64;
+    const long cb0 = fa_index(0, 2 * cf, theta, n_cg, nquad);
 #pragma unroll
     for (int r = 0; r < H; r++) {
       if (r > 0) __syncthreads();  // round r - 1 stored
@@ -659,9 +666,11 @@ __global__ __launch_bounds__(64 * anal_waves(R, REAL), 2) void fft_anal_w(FFTPar
         const int b = 64 * i + bq;
         const int k = r * LPW + b;
         const double2 X = lds_all[c * LPW + xch<W>(c, b)];
-        if (cv && k <= lmax) st_bin<FA32>(p.F, fa_index(k - m_first, 2 * cf, theta, n_cg, nquad), X);
-        if (cv && k >= N - lmax)
-          st_bin<FA32>(p.F, fa_index(k - N - m_first, 2 * cf, theta, n_cg, nquad), X);
+        const bool lo = k <= lmax;
+        const int slot = lo ? k - m_first : k - N - m_first;
+        if (cv && (lo || k >= N - lmax)) st_bin<FA32>(p.F, cb0 + slot * sst, X);
+        if (r == RH && i == IH && cv && bq == 0 && N / 2 <= lmax)
+          st_bin<FA32>(p.F, cb0 + (-N / 2 - m_first) * sst, X);
       }
     }
   } else {
