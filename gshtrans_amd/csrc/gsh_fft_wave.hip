@@ -133,12 +133,27 @@ __device__ __forceinline__ void wave_fft(double2 (&v)[R], double2* lds,
   constexpr int H = R >= 32 ? 2 : 1;  // transpose rounds
   constexpr int LR = log2c(R);
 
-  // Stage A.
+  // Stage A.  The twiddles W^(lane k1), W = exp(-2 pi i / (TS N)), from the
+  // table at k1 = 1 and at multiples of 8 only: W^(lane (8 q + j)) =
+  // W^(8 q lane) (W^lane)^j, j < 8 (at most 7 rounded products per twiddle),
+  // so a lane reads R/8 + 1 table entries instead of R - 1.
   dft_reg<R, SIGN>(v);
+  {
+    double2 wj[8];
+    wj[1] = twa[TS * 64 + lane];
 #pragma unroll
-  for (int k1 = 1; k1 < R; k1++) {
-    double2 w = twa[(TS * k1) * 64 + lane];
-    v[bitrev(k1, LR)] = SIGN < 0 ? cmul(v[bitrev(k1, LR)], w) : cmulc(v[bitrev(k1, LR)], w);
+    for (int j = 2; j < 8; j++) wj[j] = cmul(wj[j - 1], wj[1]);
+#pragma unroll
+    for (int q = 0; q < R / 8; q++) {
+      const double2 wq = q == 0 ? make_double2(1.0, 0.0) : twa[(TS * 8 * q) * 64 + lane];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int k1 = 8 * q + j;
+        if (k1 == 0) continue;
+        const double2 w = q == 0 ? wj[j] : (j == 0 ? wq : cmul(wq, wj[j]));
+        v[bitrev(k1, LR)] = SIGN < 0 ? cmul(v[bitrev(k1, LR)], w) : cmulc(v[bitrev(k1, LR)], w);
+      }
+    }
   }
 
   // T1 + B1: lane (b, cp) = (lane % 8, lane / 8) gathers k1 = 8 g + cp,
