@@ -177,12 +177,17 @@ __device__ __forceinline__ void wave_fft(double2 (&v)[R], double2* lds,
   }
 #pragma unroll
   for (int g = 0; g < G; g++) dft_reg<8, SIGN>(u[g]);
+  {
+    // exp(-2 pi i b c / 64) as the c-th power of the table's exp(-2 pi i b / 64).
+    const double2 w1 = tw[b * R * TS];
+    double2 w = w1;
 #pragma unroll
-  for (int c = 1; c < 8; c++) {
-    const double2 w = tw[b * c * R * TS];  // exp(-2 pi i b c / 64)
+    for (int c = 1; c < 8; c++) {
+      if (c > 1) w = cmul(w, w1);
 #pragma unroll
-    for (int g = 0; g < G; g++)
-      u[g][bitrev(c, 3)] = SIGN < 0 ? cmul(u[g][bitrev(c, 3)], w) : cmulc(u[g][bitrev(c, 3)], w);
+      for (int g = 0; g < G; g++)
+        u[g][bitrev(c, 3)] = SIGN < 0 ? cmul(u[g][bitrev(c, 3)], w) : cmulc(u[g][bitrev(c, 3)], w);
+    }
   }
 
   // T2 + B2: lane (k1o, cm) = (lane % R, lane / R) gathers c = cm + P cq,
@@ -604,9 +609,13 @@ __global__ __launch_bounds__(64 * anal_waves(R, REAL), 2) void fft_anal_w(FFTPar
     }
     wave_fft<R2, -1, 2>(e, lw, p.twa, p.tw, lane);
     wave_fft<R2, -1, 2>(o, lw, p.twa, p.tw, lane);
+    // W_N^(lane + 64 t) = W_N^lane exp(-2 pi i t / R) (N = 64 R), the second
+    // factor a constant rotation by -pi (32 t / R) / 16.
+    static_assert(R <= 32, "rotation in steps of pi / 16");
+    const double2 wl = p.tw[lane];
 #pragma unroll
     for (int t = 0; t < R2; t++) {
-      const double2 wo = cmul(o[t], p.tw[lane + 64 * t]);
+      const double2 wo = cmul(o[t], rot16<-1>(wl, (32 / R) * t));
       v[t] = cadd(e[t], wo);
       v[t + R2] = csub(e[t], wo);
     }
