@@ -531,12 +531,21 @@ constexpr int kDefaultAnalPass = gsh::kWideAnalPass;
 
 int order_of_host(int mw, int real) { return real ? mw : ((mw & 1) ? -((mw + 1) >> 1) : (mw >> 1)); }
 
-// Work list of an analysis launch: items (mw, pass | group << 16) ordered by
-// order, group, pass (the passes of one order and group share their input
-// blocks), padded to a multiple of 8 with (~0u, 0); glmax[g] bounds the
+// Orders per band of an analysis work list (kAnalBand below).
+constexpr int kAnalBand = 8;
+
+// Work list of an analysis launch: items (mw, pass | group << 16) in bands
+// of kAnalBand neighbouring order indices mw, each band ordered by group,
+// pass, order; padded to a multiple of 8 with (~0u, 0).  glmax[g] bounds the
 // degrees of group g (empty: lmax for every group).  Passes are kPass
 // degrees, or kWideAnalPass for family 3 (the 128-degree split-BF16 kernel);
 // family 5's items are the orders m >= 0 (mw = m) of the paired kernel.
+// An XCD runs a contiguous eighth of the list, so the workgroups on it at
+// one time hold a few passes of each order of a band: the passes of one order
+// and group read the same input blocks (from L2 after the first), and the
+// orders of a band write neighbouring 16-byte coefficients of each degree
+// (the coefficient layout runs over m within a degree), which meet in L2 as
+// whole lines rather than leave as partial-line writes.
 int ensure_worklist(gsh_grid_s* g, int family, int lmax, int n, int real, int ngroups,
                     const std::vector<int>& glmax, WorkList* out) {
   auto key = std::make_tuple(family, lmax, n, real,
@@ -548,20 +557,20 @@ int ensure_worklist(gsh_grid_s* g, int family, int lmax, int n, int real, int ng
   }
   // Family 5 (orders m and -m in one item) lists the orders m >= 0 in turn.
   const int nslot = family == 5 ? lmax + 1 : real ? lmax + 1 : 2 * lmax + 1;
+  const int plen = family == 3 ? gsh::kWideAnalPass : gsh::kPass;  // degrees per pass
   std::vector<uint2> items;
-  for (int mw = 0; mw < nslot; mw++) {
-    const int m = family == 5 ? mw : order_of_host(mw, real);
-    const int l0 = std::max(std::abs(m), std::abs(n));
+  for (int b0 = 0; b0 < nslot; b0 += kAnalBand)
     for (int grp = 0; grp < ngroups; grp++) {
       const int lw = glmax.empty() ? lmax : glmax[grp];
-      if (l0 > lw) continue;
-      const int plen = family == 3 ? gsh::kWideAnalPass : gsh::kPass;  // degrees per pass
-      const int np = (lw - l0 + plen) / plen;
-      for (int k = 0; k < np; k++)
-        items.push_back(make_uint2(static_cast<unsigned>(mw),
-                                   static_cast<unsigned>(k) | (static_cast<unsigned>(grp) << 16)));
+      for (int k = 0; k <= lw / plen; k++)
+        for (int mw = b0; mw < std::min(nslot, b0 + kAnalBand); mw++) {
+          const int m = family == 5 ? mw : order_of_host(mw, real);
+          const int l0 = std::max(std::abs(m), std::abs(n));
+          if (l0 <= lw && k < (lw - l0 + plen) / plen)
+            items.push_back(make_uint2(static_cast<unsigned>(mw),
+                                       static_cast<unsigned>(k) | (static_cast<unsigned>(grp) << 16)));
+        }
     }
-  }
   while (items.size() % 8 != 0 || items.empty()) items.push_back(make_uint2(~0u, 0u));
   WorkList w;
   w.per = static_cast<int>(items.size() / 8);
