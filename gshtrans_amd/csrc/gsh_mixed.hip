@@ -118,6 +118,31 @@ __device__ __forceinline__ void flush(d4& acc, const f4 c) {
   acc[3] += (double)c[3];
 }
 
+// Analysis epilogues.  Column 16 tile + lane % 16 of the sums is the real
+// (even column) or imaginary (odd column) part of field col / 2, so the two
+// lanes of a field hold halves of the same coefficients.  trade_halves takes
+// rows r and r + 1 of the pair and returns the whole coefficient of row r to
+// the even lane and that of row r + 1 to the odd lane, which each store 16
+// bytes (store_coef) instead of 8.
+__device__ __forceinline__ double2 trade_halves(double a0, double a1, bool odd) {
+  const double recv = __shfl_xor(odd ? a0 : a1, 1);
+  return odd ? make_double2(recv, a1) : make_double2(a0, recv);
+}
+
+// Writes (adds, under p.accumulate) coefficient (l, m) of the field whose
+// coefficients start at cbase; under p.zero_last the coefficient (L, L) of a
+// field truncated at L is zero.
+__device__ __forceinline__ void store_coef(const LegParams& p, long cbase, int l, int m, int lf,
+                                           int real, double2 v) {
+  double2* c = reinterpret_cast<double2*>(p.cout) + cbase + coef_index(l, m, p.nabs, real);
+  if (p.accumulate) {
+    const double2 o = *c;
+    v = make_double2(v.x + o.x, v.y + o.y);
+  }
+  if (p.zero_last && lf == p.L && l == lf && m == lf) v = make_double2(0.0, 0.0);
+  *c = v;
+}
+
 // ---------------------------------------------------------------------------
 // Coefficient split: synthesis B operands.  Block (cso[slot] + chunk, tile,
 // s) holds, for lane = 16*kr + c, the bf16 plane s of the coefficients of
@@ -1139,20 +1164,17 @@ __global__ __launch_bounds__(512) void leg_anal_bf16(LegParams p) {
 
   // Rows l = lp + 16*lt + 4*(lane/16) + r, column 16*tile + lane%16.
   const int col = tile * 16 + (lane & 15);
-  if (col >= p.ncol) return;
-  const int lf = field_lmax(p, col >> 1);
-  const long cbase = (long)field_index(p.fmap, col >> 1) * p.cstride;
+  const bool odd = lane & 1;
+  const bool cv = col < p.ncol;  // the same for both lanes of a field (ncol is even)
+  const int lf = cv ? field_lmax(p, col >> 1) : -1;
+  const long cbase = cv ? (long)field_index(p.fmap, col >> 1) * p.cstride : 0;
 #pragma unroll
   for (int lt = 0; lt < 4; lt++)
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int l = lp + 16 * lt + 4 * (lane >> 4) + r;
-      if (l > lf) continue;
-      const long ci = (cbase + coef_index(l, m, p.nabs, REAL)) * 2 + (col & 1);
-      double v = acc[lt][0][r];
-      if (p.accumulate) v += p.cout[ci];
-      if (p.zero_last && lf == p.L && l == lf && m == lf) v = 0.0;
-      p.cout[ci] = v;
+    for (int rp = 0; rp < 2; rp++) {
+      const double2 v = trade_halves(acc[lt][0][2 * rp], acc[lt][0][2 * rp + 1], odd);
+      const int l = lp + 16 * lt + 4 * (lane >> 4) + 2 * rp + odd;
+      if (l <= lf) store_coef(p, cbase, l, m, lf, REAL, v);
     }
 }
 
@@ -1354,27 +1376,23 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_pm(LegParams p) {
   // Rows l = lp + 16*lt + 4*(lane/16) + r, column 16*tile + lane%16; order
   // -m's sums take the sign (-1)^(l+n).
   const int col = tile * 16 + (lane & 15);
-  if (col >= p.ncol) return;
-  const int lf = field_lmax(p, col >> 1);
-  const long cbase = (long)field_index(p.fmap, col >> 1) * p.cstride;
+  const bool odd = lane & 1;
+  const bool cv = col < p.ncol;  // the same for both lanes of a field (ncol is even)
+  const int lf = cv ? field_lmax(p, col >> 1) : -1;
+  const long cbase = cv ? (long)field_index(p.fmap, col >> 1) * p.cstride : 0;
 #pragma unroll
-  for (int lt = 0; lt < 4; lt++)
+  for (int j = 0; j < 2; j++) {
+    if (j == 1 && m == 0) break;
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int l = lp + 16 * lt + 4 * (lane >> 4) + r;
-      if (l > lf) continue;
-      const long ci = (cbase + coef_index(l, m, p.nabs, 0)) * 2 + (col & 1);
-      double v = acc[lt][0][r];
-      if (p.accumulate) v += p.cout[ci];
-      if (p.zero_last && lf == p.L && l == lf && m == lf) v = 0.0;
-      p.cout[ci] = v;
-      if (m > 0) {
-        const long cn = (cbase + coef_index(l, -m, p.nabs, 0)) * 2 + (col & 1);
-        double w = (l + p.n) & 1 ? -acc[lt][1][r] : acc[lt][1][r];
-        if (p.accumulate) w += p.cout[cn];
-        p.cout[cn] = w;
+    for (int lt = 0; lt < 4; lt++)
+#pragma unroll
+      for (int rp = 0; rp < 2; rp++) {
+        double2 v = trade_halves(acc[lt][j][2 * rp], acc[lt][j][2 * rp + 1], odd);
+        const int l = lp + 16 * lt + 4 * (lane >> 4) + 2 * rp + odd;
+        if (j == 1 && ((l + p.n) & 1)) v = make_double2(-v.x, -v.y);
+        if (l <= lf) store_coef(p, cbase, l, j ? -m : m, lf, 0, v);
       }
-    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1633,21 +1651,18 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_s(LegParams p) {
   // + (lt / 2 + par0) % 2 with i its index in the class; column 16 tile +
   // lane % 16.
   const int col = tile * 16 + (lane & 15);
-  if (col >= p.ncol) return;
-  const int lf = field_lmax(p, col >> 1);
-  const long cbase = (long)field_index(p.fmap, col >> 1) * p.cstride;
+  const bool odd = lane & 1;
+  const bool cv = col < p.ncol;  // the same for both lanes of a field (ncol is even)
+  const int lf = cv ? field_lmax(p, col >> 1) : -1;
+  const long cbase = cv ? (long)field_index(p.fmap, col >> 1) * p.cstride : 0;
 #pragma unroll
   for (int lt = 0; lt < 4; lt++)
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int i = 16 * (lt & 1) + 4 * (lane >> 4) + r;
+    for (int rp = 0; rp < 2; rp++) {
+      const double2 v = trade_halves(acc[lt][0][2 * rp], acc[lt][0][2 * rp + 1], odd);
+      const int i = 16 * (lt & 1) + 4 * (lane >> 4) + 2 * rp + odd;
       const int l = lp + 2 * i + (((lt >> 1) + par0) & 1);
-      if (l > lf) continue;
-      const long ci = (cbase + coef_index(l, m, p.nabs, REAL)) * 2 + (col & 1);
-      double v = acc[lt][0][r];
-      if (p.accumulate) v += p.cout[ci];
-      if (p.zero_last && lf == p.L && l == lf && m == lf) v = 0.0;
-      p.cout[ci] = v;
+      if (l <= lf) store_coef(p, cbase, l, m, lf, REAL, v);
     }
 }
 
@@ -1833,20 +1848,17 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_w(LegParams p) {
   }
 
   const int col = tile * 16 + (lane & 15);
-  if (col >= p.ncol) return;
-  const int lf = field_lmax(p, col >> 1);
-  const long cbase = (long)field_index(p.fmap, col >> 1) * p.cstride;
+  const bool odd = lane & 1;
+  const bool cv = col < p.ncol;  // the same for both lanes of a field (ncol is even)
+  const int lf = cv ? field_lmax(p, col >> 1) : -1;
+  const long cbase = cv ? (long)field_index(p.fmap, col >> 1) * p.cstride : 0;
 #pragma unroll
   for (int lt = 0; lt < 8; lt++)
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int l = lp + 16 * lt + 4 * (lane >> 4) + r;
-      if (l > lf) continue;
-      const long ci = (cbase + coef_index(l, m, p.nabs, REAL)) * 2 + (col & 1);
-      double v = acc[lt][0][r];
-      if (p.accumulate) v += p.cout[ci];
-      if (p.zero_last && lf == p.L && l == lf && m == lf) v = 0.0;
-      p.cout[ci] = v;
+    for (int rp = 0; rp < 2; rp++) {
+      const double2 v = trade_halves(acc[lt][0][2 * rp], acc[lt][0][2 * rp + 1], odd);
+      const int l = lp + 16 * lt + 4 * (lane >> 4) + 2 * rp + odd;
+      if (l <= lf) store_coef(p, cbase, l, m, lf, REAL, v);
     }
 }
 
@@ -2065,23 +2077,20 @@ __global__ __launch_bounds__(512) void leg_anal_bf16_ws(LegParams p) {
   }
 
   const int col = tile * 16 + (lane & 15);
-  if (col >= p.ncol) return;
-  const int lf = field_lmax(p, col >> 1);
-  const long cbase = (long)field_index(p.fmap, col >> 1) * p.cstride;
+  const bool odd = lane & 1;
+  const bool cv = col < p.ncol;  // the same for both lanes of a field (ncol is even)
+  const int lf = cv ? field_lmax(p, col >> 1) : -1;
+  const long cbase = cv ? (long)field_index(p.fmap, col >> 1) * p.cstride : 0;
   // Row 16 lt + 4 (lane / 16) + r of class lt / 4 is degree lp + d, d =
   // 2 i + (lt / 4 + par0) % 2 with i its index in the class.
 #pragma unroll
   for (int lt = 0; lt < 8; lt++)
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int i = 16 * (lt & 3) + 4 * (lane >> 4) + r;
+    for (int rp = 0; rp < 2; rp++) {
+      const double2 v = trade_halves(acc[lt][0][2 * rp], acc[lt][0][2 * rp + 1], odd);
+      const int i = 16 * (lt & 3) + 4 * (lane >> 4) + 2 * rp + odd;
       const int l = lp + 2 * i + (((lt >> 2) + par0) & 1);
-      if (l > lf) continue;
-      const long ci = (cbase + coef_index(l, m, p.nabs, REAL)) * 2 + (col & 1);
-      double v = acc[lt][0][r];
-      if (p.accumulate) v += p.cout[ci];
-      if (p.zero_last && lf == p.L && l == lf && m == lf) v = 0.0;
-      p.cout[ci] = v;
+      if (l <= lf) store_coef(p, cbase, l, m, lf, REAL, v);
     }
 }
 
