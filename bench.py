@@ -48,10 +48,9 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "transform_kernels_lmax1024_b64_co
 # batch), as regular expressions over the summary's kernel names.
 FAMILY_KERNELS = {
     # split-BF16 Legendre kernels: the n=0 batch runs the equatorially
-    # symmetric variants (_ws, _s), the n=2 batch orders m and -m paired
-    # (_pm; the bf16x3 analysis: the 128-degree kernel _w)
+    # symmetric variants (_ws, _s), the n=2 batch orders m and -m paired (_pm)
     "legendre_analysis": {"fp64": r"leg_anal_f64<4, false>", "fp32": r"leg_anal_f64<4, true>",
-                          "bf16x3": r"leg_anal_bf16_ws?<2, \d>",
+                          "bf16x3": r"leg_anal_bf16_(ws<2, \d>|pm<2>)",
                           "bf16x6": r"leg_anal_bf16_(ws<3, \d>|pm<3>)"},
     "legendre_synthesis": {"fp64": r"leg_synth_f64<2, \d, false>",
                            "fp32": r"leg_synth_f64<2, \d, true>",

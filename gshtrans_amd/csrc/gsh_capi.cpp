@@ -1222,15 +1222,12 @@ int run_forward(gsh_grid_s* g, NData* d, int lmax, int n, int real, int precisio
   // Gauss-Legendre nodes are symmetric about the equator and d^l_{m0} has
   // parity (-1)^(l+m) under theta -> pi - theta).  Complex fields of other
   // upper indices run orders m and -m together in 64-degree passes
-  // (d^l_{-m,n}(theta) = (-1)^(l+n) d^l_{m,n}(pi - theta)) by default in
-  // bf16x6, whose six products per value outweigh the paired kernel's extra
-  // B-operand work; everything else defaults to 128-degree passes.
-  // o.anal_pass (64 or 128, 0 for the default) overrides.
+  // (d^l_{-m,n}(theta) = (-1)^(l+n) d^l_{m,n}(pi - theta)) by default;
+  // everything else defaults to 128-degree passes.  o.anal_pass (64 or 128,
+  // 0 for the default) overrides.
   const bool sym = n == 0;
   const bool pm_ok = !real && !sym;
-  const int apass = o.anal_pass                           ? o.anal_pass
-                    : pm_ok && precision == GSH_BF16X6 ? gsh::kPass
-                                                          : kDefaultAnalPass;
+  const int apass = o.anal_pass ? o.anal_pass : pm_ok ? gsh::kPass : kDefaultAnalPass;
   const bool wide = apass == gsh::kWideAnalPass;
   const bool pm = pm_ok && !wide;
   {
